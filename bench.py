@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Benchmark of the SimpleNavierStokesSolver time-step hot path on one MI355X.
+
+Workload: the large single instance, 4096 x 8192 cells, uniform grid, lx=1,
+ly=2, Ra=1e11, Pr=0.7, FP64, Crank-Nicolson diffusion in x and explicit in y,
+seeded temperature noise of amplitude 1e-3 (u = 0 initially).  One step is
+what the reference's time loop does per iteration (src/main.c:28-35): the
+time-step decision (a device reduction over ux, uy read back to the host)
+and one three-substage RK3 integration.  Inputs are resident in HBM before
+the timed region starts.
+
+Printed JSON (one line): value = cell-steps/s, plus
+  roofline      -- the dominant kernel, timed with HIP events on the
+                   instance's stream during the timed steps, against the
+                   8 TB/s HBM peak, using that kernel's algorithmic bytes;
+  step_roofline -- the whole step against 600 B/cell-step of compulsory
+                   traffic (75 FP64 words per cell and step);
+  cpu_baseline  -- the CPU oracle (a C port of the reference algorithm,
+                   one core) timed on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+B_ALG_STEP = 600.0  # bytes per cell-step, explicit-y treatments
+
+# Algorithmic (compulsory) HBM bytes per interior cell for one launch of each
+# kernel of the step: each array the kernel must read or write, once.
+KERNEL_BYTES_PER_CELL = {
+    "force": 8 * 2,            # T -> tempforcex
+    "src_temp": 8 * 5,         # ux, uy, T -> A, G
+    "src_vel": 8 * 8,          # ux, uy, p, tf -> A, G for ux and uy
+    "cnx": 8 * 5,              # A, B, G, field -> field (one field)
+    "incr": 8 * 5,
+    "poisson_fwd": 8 * 3,      # ux, uy -> spectrum (jtot/2+1 complex per column)
+    "poisson_x": 8 * 2,        # spectrum in place
+    "poisson_inv": 8 * 2,      # spectrum -> psi
+    "correct": 8 * 5,          # ux, uy, psi -> ux, uy
+    "pressure": 8 * 3,         # p, psi -> p
+    "dt_reduce": 8 * 2,        # ux, uy
+    "pred": 8 * 10,            # fused predictor: ux, uy, p, T, A(3) -> T', ux*, uy*, A(3)
+    "corr": 8 * 7,             # fused correction: ux*, uy*, psi, p -> ux, uy, p
+}
+KERNEL_NAMES = list(KERNEL_BYTES_PER_CELL.keys())
+
+WORKLOAD = {
+    "workload": "rayleigh_benard_2d_large_single_instance",
+    "itot": 4096, "jtot": 8192, "lx": 1.0, "ly": 2.0, "grid": "uniform",
+    "Ra": 1e11, "Pr": 0.7, "implicitx": 1, "implicity": 0,
+    "ic": "u=0, T=1e-3*(2U-1) seeded (seed 3)", "seed": 3, "amplitude": 1e-3,
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--itot", type=int, default=WORKLOAD["itot"])
+    ap.add_argument("--jtot", type=int, default=WORKLOAD["jtot"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--per-kernel", action="store_true", help="print per-kernel timings to stderr")
+    return ap.parse_args()
+
+
+def make_instance(sns, itot, jtot):
+    w = WORKLOAD
+    inst = sns.Instance(itot, jtot, ly=w["ly"], stretch=float(itot), Ra=w["Ra"], Pr=w["Pr"],
+                        implicitx=w["implicitx"], implicity=w["implicity"])
+    inst.state_alloc()
+    t0 = sns.initial_temperature(itot, jtot, inst.g["xc"], w["seed"], w["amplitude"], False)
+    inst.state_upload("temp", t0)
+    inst.update_boundaries_state = None
+    return inst
+
+
+def step(sns, inst):
+    w = WORKLOAD
+    raw = inst.state_dt_adv_raw()
+    dt = sns.decide_dt(inst.g, w["Ra"], w["Pr"], raw, w["implicitx"], w["implicity"])
+    inst.set_dt(dt)
+    inst.integrate()
+    return dt
+
+
+def per_kernel_times(sns, inst, steps):
+    """Average duration of each kernel over `steps` steps (HIP events)."""
+    out = {}
+    for name in KERNEL_NAMES:
+        inst.timing_enable(name)
+        for _ in range(steps):
+            step(sns, inst)
+        n, ms = inst.timing_read()
+        inst.timing_disable()
+        if n:
+            out[name] = (n, ms)
+    return out
+
+
+def cpu_baseline(itot, jtot, nsteps):
+    from oracle import pyoracle as po
+    w = WORKLOAD
+    o = po.Oracle(itot, jtot, ly=w["ly"], stretch=float(itot), Ra=w["Ra"], Pr=w["Pr"])
+    o.set_treatment(w["implicitx"], w["implicity"])
+    o.init_temperature(w["seed"], w["amplitude"], linear=False)
+    o.apply_boundaries()
+    t = time.perf_counter()
+    o.run(nsteps)
+    el = time.perf_counter() - t
+    return {"value": itot * jtot * nsteps / el, "unit": "cell-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": "%d RK3 step(s) (time-step decision + 3 substages) of the same %dx%d "
+                      "workload from the same initial state, CPU oracle (C, one thread, "
+                      "gcc -O2)" % (nsteps, itot, jtot),
+            "seconds": el}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(multi-GPU sharding is described in DESIGN.md, not implemented)" % a.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import simplenavierstokessolver_amd as sns
+    itot, jtot = a.itot, a.jtot
+    cells = itot * jtot
+    inst = make_instance(sns, itot, jtot)
+    for _ in range(a.warmup):
+        step(sns, inst)
+    inst.sync()
+    # timed region, with the dominant-kernel events recorded live on the stream
+    kt = per_kernel_times(sns, inst, 1)
+    dominant = max(kt.items(), key=lambda kv: kv[1][1])[0] if kt else None
+    inst.sync()
+    if dominant:
+        inst.timing_enable(dominant)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step(sns, inst)
+    inst.sync()
+    el = time.perf_counter() - t0
+    nlaunch, kms = inst.timing_read() if dominant else (0, 0.0)
+    inst.timing_disable()
+    value = cells * a.steps / el
+    res = {
+        "metric": "FP64 cell-steps/s @4096x8192 (%HBM roofline)",
+        "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": 1e3 * el / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": dict(WORKLOAD, itot=itot, jtot=jtot),
+    }
+    if dominant and nlaunch:
+        avg_s = kms / nlaunch * 1e-3
+        bytes_launch = KERNEL_BYTES_PER_CELL[dominant] * cells
+        ach = bytes_launch / avg_s / 1e9
+        res["roofline"] = {"bound": "hbm", "kernel": dominant, "achieved": ach,
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                           "traffic": None, "avg_launch_ms": kms / nlaunch,
+                           "bytes_per_launch": bytes_launch}
+    step_gbs = value * B_ALG_STEP / 1e9
+    res["step_roofline"] = {"bytes_per_cell_step": B_ALG_STEP, "achieved": step_gbs,
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS}
+    if a.per_kernel:
+        kt = per_kernel_times(sns, inst, 2)
+        tot = sum(v[1] for v in kt.values())
+        for k, (n, ms) in sorted(kt.items(), key=lambda kv: -kv[1][1]):
+            gbs = KERNEL_BYTES_PER_CELL[k] * cells / (ms / n * 1e-3) / 1e9
+            print("%-12s launches %4d  avg %8.3f ms  share %5.1f%%  %7.0f GB/s" %
+                  (k, n, ms / n, 100 * ms / tot, gbs), file=sys.stderr)
+    inst.close()
+    if not a.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline(itot, jtot, a.cpu_steps)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

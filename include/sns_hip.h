@@ -1,0 +1,133 @@
+/*
+ * sns_hip.h -- thin C-ABI between the C host layer and the gfx950 kernels
+ * (libsns_hip.so).  Plain C types only: sizes, scalars and device pointers.
+ *
+ * One snsg_ctx is one simulation instance on one GPU: it owns a HIP stream,
+ * the device copies of the grid metrics, the time step, the Poisson and
+ * Crank-Nicolson scratch and the FFT plan.  Field arrays are device memory
+ * in the reference's layouts (include/arrays/fluid.h, temperature.h of the
+ * reference; see the layout comment in include/sns_arrays.h), allocated with
+ * snsg_malloc and passed explicitly, so every entry point below maps onto
+ * one reference function with the same arguments.
+ *
+ * Return values: 0 on success, otherwise a non-zero code (a hipError_t value
+ * or one of SNSG_E_*); snsg_last_error() gives a message.  Work is enqueued
+ * on the instance's stream; snsg_synchronize() waits for it.
+ *
+ * Entry points and the reference interface each one replaces:
+ *   snsg_fluid_update_boundaries_{ux,uy,p}   include/fluid.h:16-18
+ *   snsg_temperature_update_boundaries_temp  include/temperature.h:20
+ *   snsg_temperature_compute_force           include/temperature.h:17
+ *   snsg_temperature_update_temp             include/temperature.h:18
+ *   snsg_fluid_update_velocity               include/fluid.h:20
+ *   snsg_fluid_compute_potential             include/fluid.h:21
+ *   snsg_fluid_correct_velocity              include/fluid.h:22
+ *   snsg_fluid_update_pressure               include/fluid.h:23
+ *   snsg_param_decide_dt                     include/param.h:59 (device reduction part)
+ *   snsg_transpose                           include/parallel.h:21 (one rank)
+ *   snsg_integrate                           src/main.c:79-101 (static integrate())
+ */
+#ifndef SNS_HIP_H
+#define SNS_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SNSG_E_ARG      (-1)   /* invalid argument */
+#define SNSG_E_SIZE     (-2)   /* grid size not supported by the kernels */
+#define SNSG_E_NOMEM    (-3)   /* device allocation failed */
+
+typedef struct snsg_ctx snsg_ctx;
+
+typedef struct {
+  int itot, jtot;                 /* cells in x (walls) and y (periodic) */
+  double ly;                      /* domain length in y (lx is 1) */
+  double Ra, Pr;
+  int with_temperature, with_thermal_forcing;
+  int implicitx, implicity;       /* diffusive treatment */
+  double alpha[3], beta[3], gamma[3];  /* RK3 coefficients */
+} snsg_config;
+
+/* Device-resident state advanced by the fused step (snsg_integrate).  The
+ * layouts are the reference's; temp and temp_alt are exchanged by every
+ * substage (the caller's pointers are updated). */
+typedef struct {
+  double *ux, *uy, *p, *psi;
+  double *temp, *temp_alt;
+  double *srcuxa, *srcuya, *srctempa;   /* RK history of the explicit terms */
+  double *ux_star, *uy_star;            /* predictor output, (jtot+2) rows */
+} snsg_state;
+
+/* ---- instance management */
+int snsg_create(int device, const snsg_config *cfg, const double *dxf, const double *dxc,
+                snsg_ctx **ctx);
+int snsg_destroy(snsg_ctx *ctx);
+int snsg_set_treatment(snsg_ctx *ctx, int implicitx, int implicity);
+int snsg_set_dt(snsg_ctx *ctx, double dt);
+int snsg_synchronize(snsg_ctx *ctx);
+const char *snsg_last_error(void);
+int snsg_device_count(int *count);
+
+/* ---- memory (zero-initialised device allocations) */
+int snsg_malloc(snsg_ctx *ctx, size_t bytes, void **dptr);
+int snsg_free(snsg_ctx *ctx, void *dptr);
+int snsg_upload(snsg_ctx *ctx, void *dst, const void *src, size_t bytes);
+int snsg_download(snsg_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* ---- boundary values */
+int snsg_fluid_update_boundaries_ux(snsg_ctx *ctx, double *ux);
+int snsg_fluid_update_boundaries_uy(snsg_ctx *ctx, double *uy);
+int snsg_fluid_update_boundaries_p(snsg_ctx *ctx, double *p);
+int snsg_temperature_update_boundaries_temp(snsg_ctx *ctx, double *temp);
+
+/* ---- temperature.  The reference copies SRCTEMPB <- SRCTEMPA before
+ * recomputing SRCTEMPA (src/temperature/update.c:31); here srctempb must
+ * already hold the previous SRCTEMPA (the host layer exchanges the two
+ * pointers) and srctempa is overwritten. */
+int snsg_temperature_compute_force(snsg_ctx *ctx, const double *temp, double *tempforcex);
+int snsg_temperature_update_temp(snsg_ctx *ctx, int rkstep, const double *ux, const double *uy,
+                                 double *temp, double *srctempa, const double *srctempb,
+                                 double *srctempg);
+
+/* ---- fluid (same convention for the ux / uy source histories) */
+int snsg_fluid_update_velocity(snsg_ctx *ctx, int rkstep, double *ux, double *uy,
+                               const double *p, const double *tempforcex, double *srcuxa,
+                               const double *srcuxb, double *srcuxg, double *srcuya,
+                               const double *srcuyb, double *srcuyg);
+int snsg_fluid_compute_potential(snsg_ctx *ctx, int rkstep, const double *ux, const double *uy,
+                                 double *psi);
+int snsg_fluid_correct_velocity(snsg_ctx *ctx, int rkstep, const double *psi, double *ux,
+                                double *uy);
+int snsg_fluid_update_pressure(snsg_ctx *ctx, int rkstep, const double *psi, double *p);
+
+/* ---- time step: *dt_adv receives min(5e-2, min DXC(i)/max(|UX|,1e-8),
+ * min dy/max(|UY|,1e-8)), the advective limit of src/param/decide_dt.c:25-45
+ * before its safety factor.  Blocks until the value is on the host. */
+int snsg_param_decide_dt(snsg_ctx *ctx, const double *ux, const double *uy, double *dt_adv);
+
+/* ---- global transpose on one rank: recv[i*g_jsize + j] = send[j*g_isize + i],
+ * elemsize 4, 8 or 16 bytes (src/parallel/transpose.c:6-85) */
+int snsg_transpose(snsg_ctx *ctx, int g_isize, int g_jsize, size_t elemsize, const void *send,
+                   void *recv);
+
+/* ---- one RK3 step with fused kernels, using the dt set by snsg_set_dt */
+int snsg_state_alloc(snsg_ctx *ctx, snsg_state *st);
+int snsg_state_free(snsg_ctx *ctx, snsg_state *st);
+int snsg_integrate(snsg_ctx *ctx, snsg_state *st);
+
+/* ---- per-kernel timing with HIP events on the instance's stream.  While
+ * enabled, every launch of a kernel whose name contains `filter` (NULL: all)
+ * is bracketed by two events; snsg_timing_read returns the number of such
+ * launches and their summed duration in milliseconds. */
+int snsg_timing_enable(snsg_ctx *ctx, const char *filter);
+int snsg_timing_disable(snsg_ctx *ctx);
+int snsg_timing_read(snsg_ctx *ctx, int *launches, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SNS_HIP_H */

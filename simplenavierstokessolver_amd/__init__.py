@@ -1,0 +1,383 @@
+"""MI355X (gfx950) implementation of the SimpleNavierStokesSolver time-step hot path.
+
+Native parts (built in-tree by ``simplenavierstokessolver_amd.build``):
+
+* ``lib/libsns_hip.so`` -- hand-written HIP kernels and the C-ABI declared in
+  ``include/sns_hip.h``;
+* ``lib/libsns.so`` -- the C host layer with the reference's call surface
+  (``param_*``, ``fluid_*``, ``temperature_*``, ``fileio_*``);
+* ``lib/sns_run`` -- the driver with the reference's time loop.
+
+This module is a ctypes front end used by the tests and the benchmark.  It
+never falls back to a CPU path: if the native library is missing, loading
+fails with an error.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_PKG, "lib")
+HIP_LIB = os.path.join(LIB_DIR, "libsns_hip.so")
+HOST_LIB = os.path.join(LIB_DIR, "libsns.so")
+DRIVER = os.path.join(LIB_DIR, "sns_run")
+
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+
+class SnsgConfig(C.Structure):
+    _fields_ = [("itot", C.c_int), ("jtot", C.c_int), ("ly", C.c_double),
+                ("Ra", C.c_double), ("Pr", C.c_double),
+                ("with_temperature", C.c_int), ("with_thermal_forcing", C.c_int),
+                ("implicitx", C.c_int), ("implicity", C.c_int),
+                ("alpha", C.c_double * 3), ("beta", C.c_double * 3), ("gamma", C.c_double * 3)]
+
+
+class SnsgState(C.Structure):
+    _fields_ = [(n, _vp) for n in ["ux", "uy", "p", "psi", "temp", "temp_alt",
+                                   "srcuxa", "srcuya", "srctempa", "ux_star", "uy_star"]]
+
+
+class GpuError(RuntimeError):
+    pass
+
+
+_hip = None
+_host = None
+
+
+def _load_hip():
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_LIB):
+            raise GpuError("native library %s is missing: run simplenavierstokessolver_amd.build"
+                           % HIP_LIB)
+        L = C.CDLL(HIP_LIB)
+        L.snsg_last_error.restype = C.c_char_p
+        L.snsg_create.argtypes = [C.c_int, C.POINTER(SnsgConfig), _dp, _dp, C.POINTER(_vp)]
+        L.snsg_destroy.argtypes = [_vp]
+        L.snsg_set_treatment.argtypes = [_vp, C.c_int, C.c_int]
+        L.snsg_set_dt.argtypes = [_vp, C.c_double]
+        L.snsg_synchronize.argtypes = [_vp]
+        L.snsg_device_count.argtypes = [C.POINTER(C.c_int)]
+        L.snsg_malloc.argtypes = [_vp, C.c_size_t, C.POINTER(_vp)]
+        L.snsg_free.argtypes = [_vp, _vp]
+        L.snsg_upload.argtypes = [_vp, _vp, _vp, C.c_size_t]
+        L.snsg_download.argtypes = [_vp, _vp, _vp, C.c_size_t]
+        for n in ["snsg_fluid_update_boundaries_ux", "snsg_fluid_update_boundaries_uy",
+                  "snsg_fluid_update_boundaries_p", "snsg_temperature_update_boundaries_temp"]:
+            getattr(L, n).argtypes = [_vp, _vp]
+        L.snsg_temperature_compute_force.argtypes = [_vp, _vp, _vp]
+        L.snsg_temperature_update_temp.argtypes = [_vp, C.c_int] + [_vp] * 6
+        L.snsg_fluid_update_velocity.argtypes = [_vp, C.c_int] + [_vp] * 10
+        L.snsg_fluid_compute_potential.argtypes = [_vp, C.c_int, _vp, _vp, _vp]
+        L.snsg_fluid_correct_velocity.argtypes = [_vp, C.c_int, _vp, _vp, _vp]
+        L.snsg_fluid_update_pressure.argtypes = [_vp, C.c_int, _vp, _vp]
+        L.snsg_param_decide_dt.argtypes = [_vp, _vp, _vp, _dp]
+        L.snsg_transpose.argtypes = [_vp, C.c_int, C.c_int, C.c_size_t, _vp, _vp]
+        L.snsg_state_alloc.argtypes = [_vp, C.POINTER(SnsgState)]
+        L.snsg_state_free.argtypes = [_vp, C.POINTER(SnsgState)]
+        L.snsg_integrate.argtypes = [_vp, C.POINTER(SnsgState)]
+        L.snsg_timing_enable.argtypes = [_vp, C.c_char_p]
+        L.snsg_timing_disable.argtypes = [_vp]
+        L.snsg_timing_read.argtypes = [_vp, C.POINTER(C.c_int), _dp]
+        _hip = L
+    return _hip
+
+
+def _load_host():
+    global _host
+    if _host is None:
+        _load_hip()
+        if not os.path.exists(HOST_LIB):
+            raise GpuError("native library %s is missing: run simplenavierstokessolver_amd.build"
+                           % HOST_LIB)
+        L = C.CDLL(HOST_LIB)
+        L.sns_set_coordinate.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                         _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.sns_rk_coefs.argtypes = [_dp, _dp, _dp]
+        L.sns_initial_temperature.argtypes = [C.c_int, C.c_int, _dp, C.c_uint64, C.c_double,
+                                              C.c_int, _dp]
+        _host = L
+    return _host
+
+
+def hip_lib():
+    return _load_hip()
+
+
+def host_lib():
+    return _load_host()
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise GpuError("%s failed (%d): %s" % (what, rc, _load_hip().snsg_last_error().decode()))
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def device_count():
+    n = C.c_int(0)
+    _check(_load_hip().snsg_device_count(C.byref(n)), "snsg_device_count")
+    return n.value
+
+
+def grid(itot, jtot, ly=2.0, stretch=3.0, lx=1.0):
+    """Coordinates from the host layer (src/param/init.c:124-195 restated)."""
+    L = _load_host()
+    g = {"xf": np.zeros(itot + 1), "xc": np.zeros(itot + 2), "dxf": np.zeros(itot),
+         "dxc": np.zeros(itot + 1), "yf": np.zeros(jtot + 1), "yc": np.zeros(jtot)}
+    dy = C.c_double(0.0)
+    L.sns_set_coordinate(itot, jtot, lx, ly, stretch, _p(g["xf"]), _p(g["xc"]), _p(g["dxf"]),
+                         _p(g["dxc"]), _p(g["yf"]), _p(g["yc"]), C.byref(dy))
+    g["dy"] = dy.value
+    return g
+
+
+def rk_coefs():
+    a, b, c = np.zeros(3), np.zeros(3), np.zeros(3)
+    _load_host().sns_rk_coefs(_p(a), _p(b), _p(c))
+    return a, b, c
+
+
+def initial_temperature(itot, jtot, xc, seed, amplitude, linear=False):
+    """Interior of the seeded initial temperature in the (jtot+2, itot+2) layout."""
+    t = np.zeros((jtot + 2, itot + 2))
+    _load_host().sns_initial_temperature(itot, jtot, _p(np.ascontiguousarray(xc)), seed,
+                                         amplitude, int(linear), _p(t))
+    return t
+
+
+FIELD_SHAPES = {
+    "ux": lambda i, j: (j + 2, i + 1), "uy": lambda i, j: (j + 2, i + 2),
+    "p": lambda i, j: (j + 2, i + 2), "psi": lambda i, j: (j + 2, i + 2),
+    "temp": lambda i, j: (j + 2, i + 2), "tempforcex": lambda i, j: (j, i - 1),
+    "srcuxa": lambda i, j: (j, i - 1), "srcuxb": lambda i, j: (j, i - 1),
+    "srcuxg": lambda i, j: (j, i - 1), "srcuya": lambda i, j: (j, i),
+    "srcuyb": lambda i, j: (j, i), "srcuyg": lambda i, j: (j, i),
+    "srctempa": lambda i, j: (j, i), "srctempb": lambda i, j: (j, i),
+    "srctempg": lambda i, j: (j, i),
+}
+
+
+class Instance:
+    """One simulation on one GPU: the reference's arrays as device buffers and
+    the reference-named operations on them (through include/sns_hip.h)."""
+
+    def __init__(self, itot, jtot, ly=2.0, stretch=3.0, Ra=1e8, Pr=1.0, implicitx=0,
+                 implicity=0, with_temperature=True, with_thermal_forcing=True, device=0):
+        self.L = _load_hip()
+        self.itot, self.jtot = itot, jtot
+        self.g = grid(itot, jtot, ly, stretch)
+        self.alpha, self.beta, self.gamma = rk_coefs()
+        cfg = SnsgConfig()
+        cfg.itot, cfg.jtot, cfg.ly, cfg.Ra, cfg.Pr = itot, jtot, ly, Ra, Pr
+        cfg.with_temperature = int(with_temperature)
+        cfg.with_thermal_forcing = int(with_thermal_forcing)
+        cfg.implicitx, cfg.implicity = int(implicitx), int(implicity)
+        for k in range(3):
+            cfg.alpha[k], cfg.beta[k], cfg.gamma[k] = self.alpha[k], self.beta[k], self.gamma[k]
+        self.cfg = cfg
+        self.Ra, self.Pr, self.ly = Ra, Pr, ly
+        self.ctx = _vp()
+        _check(self.L.snsg_create(device, C.byref(cfg), _p(self.g["dxf"]), _p(self.g["dxc"]),
+                                  C.byref(self.ctx)), "snsg_create")
+        self.buf = {}
+        self.state = None
+
+    # ---- lifetime
+    def close(self):
+        if self.ctx:
+            if self.state is not None:
+                self.L.snsg_state_free(self.ctx, C.byref(self.state))
+                self.state = None
+            for v in self.buf.values():
+                self.L.snsg_free(self.ctx, v)
+            self.buf = {}
+            self.L.snsg_destroy(self.ctx)
+            self.ctx = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- memory
+    def shape(self, name):
+        return FIELD_SHAPES[name](self.itot, self.jtot)
+
+    def alloc(self, *names):
+        for n in names or FIELD_SHAPES.keys():
+            if n not in self.buf:
+                nbytes = 8 * int(np.prod(self.shape(n)))
+                ptr = _vp()
+                _check(self.L.snsg_malloc(self.ctx, nbytes, C.byref(ptr)), "snsg_malloc")
+                self.buf[n] = ptr
+        return self
+
+    def upload(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == self.shape(name), (name, a.shape, self.shape(name))
+        self.alloc(name)
+        _check(self.L.snsg_upload(self.ctx, self.buf[name], a.ctypes.data, a.nbytes), "upload")
+
+    def download(self, name):
+        a = np.zeros(self.shape(name))
+        _check(self.L.snsg_download(self.ctx, a.ctypes.data, self.buf[name], a.nbytes), "download")
+        return a
+
+    def sync(self):
+        _check(self.L.snsg_synchronize(self.ctx), "snsg_synchronize")
+
+    # ---- parameters
+    def set_dt(self, dt):
+        self.dt = float(dt)
+        _check(self.L.snsg_set_dt(self.ctx, self.dt), "snsg_set_dt")
+
+    def set_treatment(self, implicitx, implicity):
+        _check(self.L.snsg_set_treatment(self.ctx, int(implicitx), int(implicity)),
+               "snsg_set_treatment")
+
+    # ---- reference functions (host-layer conventions: B <- A by pointer exchange)
+    def update_boundaries(self, name):
+        fn = {"ux": "snsg_fluid_update_boundaries_ux", "uy": "snsg_fluid_update_boundaries_uy",
+              "p": "snsg_fluid_update_boundaries_p", "psi": "snsg_fluid_update_boundaries_p",
+              "temp": "snsg_temperature_update_boundaries_temp"}[name]
+        _check(getattr(self.L, fn)(self.ctx, self.buf[name]), fn)
+
+    def compute_force(self):
+        _check(self.L.snsg_temperature_compute_force(self.ctx, self.buf["temp"],
+                                                     self.buf["tempforcex"]), "compute_force")
+
+    def _swap(self, a, b):
+        self.buf[a], self.buf[b] = self.buf[b], self.buf[a]
+
+    def update_temp(self, rkstep):
+        self._swap("srctempa", "srctempb")
+        b = self.buf
+        _check(self.L.snsg_temperature_update_temp(self.ctx, rkstep, b["ux"], b["uy"], b["temp"],
+                                                   b["srctempa"], b["srctempb"], b["srctempg"]),
+               "update_temp")
+
+    def update_velocity(self, rkstep):
+        self._swap("srcuxa", "srcuxb")
+        self._swap("srcuya", "srcuyb")
+        b = self.buf
+        _check(self.L.snsg_fluid_update_velocity(self.ctx, rkstep, b["ux"], b["uy"], b["p"],
+                                                 b["tempforcex"], b["srcuxa"], b["srcuxb"],
+                                                 b["srcuxg"], b["srcuya"], b["srcuyb"],
+                                                 b["srcuyg"]), "update_velocity")
+
+    def compute_potential(self, rkstep):
+        b = self.buf
+        _check(self.L.snsg_fluid_compute_potential(self.ctx, rkstep, b["ux"], b["uy"], b["psi"]),
+               "compute_potential")
+
+    def correct_velocity(self, rkstep):
+        b = self.buf
+        _check(self.L.snsg_fluid_correct_velocity(self.ctx, rkstep, b["psi"], b["ux"], b["uy"]),
+               "correct_velocity")
+
+    def update_pressure(self, rkstep):
+        b = self.buf
+        _check(self.L.snsg_fluid_update_pressure(self.ctx, rkstep, b["psi"], b["p"]),
+               "update_pressure")
+
+    def dt_adv_raw(self, ux="ux", uy="uy"):
+        v = C.c_double(0.0)
+        _check(self.L.snsg_param_decide_dt(self.ctx, self.buf[ux], self.buf[uy], C.byref(v)),
+               "decide_dt")
+        return v.value
+
+    def transpose(self, g_isize, g_jsize, elemsize, send_ptr, recv_ptr):
+        _check(self.L.snsg_transpose(self.ctx, g_isize, g_jsize, elemsize, send_ptr, recv_ptr),
+               "transpose")
+
+    def integrate_reference_order(self):
+        """One RK3 step through the reference-named operations (src/main.c:79-101)."""
+        for rk in range(3):
+            for n in ("ux", "uy", "p", "temp"):
+                self.update_boundaries(n)
+            if self.cfg.with_temperature:
+                self.compute_force()
+                self.update_temp(rk)
+            self.update_velocity(rk)
+            self.compute_potential(rk)
+            self.correct_velocity(rk)
+            self.update_pressure(rk)
+
+    # ---- fused step on instance-owned state
+    def state_alloc(self):
+        if self.state is None:
+            self.state = SnsgState()
+            _check(self.L.snsg_state_alloc(self.ctx, C.byref(self.state)), "snsg_state_alloc")
+        return self.state
+
+    def state_upload(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        _check(self.L.snsg_upload(self.ctx, getattr(self.state, name), a.ctypes.data, a.nbytes),
+               "state upload")
+
+    def state_download(self, name, shape_name=None):
+        a = np.zeros(self.shape(shape_name or name))
+        _check(self.L.snsg_download(self.ctx, a.ctypes.data, getattr(self.state, name), a.nbytes),
+               "state download")
+        return a
+
+    def integrate(self):
+        _check(self.L.snsg_integrate(self.ctx, C.byref(self.state)), "snsg_integrate")
+
+    def state_dt_adv_raw(self):
+        v = C.c_double(0.0)
+        _check(self.L.snsg_param_decide_dt(self.ctx, self.state.ux, self.state.uy, C.byref(v)),
+               "decide_dt")
+        return v.value
+
+    # ---- timing
+    def timing_enable(self, name_filter=None):
+        _check(self.L.snsg_timing_enable(self.ctx, name_filter.encode() if name_filter else None),
+               "timing_enable")
+
+    def timing_read(self):
+        n, ms = C.c_int(0), C.c_double(0.0)
+        _check(self.L.snsg_timing_read(self.ctx, C.byref(n), C.byref(ms)), "timing_read")
+        return n.value, ms.value
+
+    def timing_disable(self):
+        _check(self.L.snsg_timing_disable(self.ctx), "timing_disable")
+
+
+def decide_dt(g, Ra, Pr, dt_adv_raw, implicitx, implicity):
+    """Host part of src/param/decide_dt.c:11-101 with a forced treatment:
+    the advective bound (device reduction) times 0.7, the diffusive bounds,
+    and the reference's selection rule fed a cost table that picks the
+    forced treatment.  Evaluated in C by the host layer's formula restated
+    here with the same operation order."""
+    import math
+    dt_adv = dt_adv_raw * 0.7
+    diffusivity = max(math.sqrt(Pr) / math.sqrt(Ra), 1. / math.sqrt(Pr) / math.sqrt(Ra))
+    dx = 1.0
+    for v in g["dxc"][1:-1]:
+        dx = min(dx, v)
+    dt_dif_x = 0.25 / diffusivity * dx ** 2.0 * 0.9
+    dt_dif_y = 0.25 / diffusivity * g["dy"] ** 2.0 * 0.9
+    if implicitx and implicity:
+        return dt_adv
+    if implicitx:
+        return min(dt_adv, dt_dif_y)
+    if implicity:
+        return min(dt_adv, dt_dif_x)
+    return min(dt_adv, min(dt_dif_x, dt_dif_y))

@@ -1,0 +1,551 @@
+// sns_capi.hip -- instance context and the extern "C" entry points of
+// libsns_hip.so (declared in include/sns_hip.h).
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+#include "sns_hip.h"
+#include "sns_launch.h"
+
+using namespace sns;
+
+#ifndef M_PI
+#define M_PI 3.1415926535897932
+#endif
+
+struct snsg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  snsg_config cfg{};
+  Geo geo{};
+  Treat treat{};
+  double *d_dxf = nullptr, *d_dxc = nullptr;
+  double *d_dt = nullptr;
+  double *d_eigen = nullptr;
+  double2 *d_tw = nullptr;
+  double2 *d_psihat = nullptr;
+  double *d_q = nullptr, *d_qy = nullptr;
+  unsigned long long *d_red = nullptr;
+  unsigned long long *h_red = nullptr;  // pinned
+  FftPlan plan{};
+  // fused-step scratch (RK history B/G kept on device for the composed path)
+  double *d_srcb[3] = {nullptr, nullptr, nullptr};
+  double *d_srcg[3] = {nullptr, nullptr, nullptr};
+  double *d_tf = nullptr;
+  // timing
+  bool timing = false;
+  std::string filter;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+};
+
+static thread_local std::string g_last_error;
+
+static int set_err(int code, const char *what) {
+  char buf[512];
+  snprintf(buf, sizeof buf, "%s (code %d%s%s)", what, code, code > 0 ? ": " : "",
+           code > 0 ? hipGetErrorString((hipError_t)code) : "");
+  g_last_error = buf;
+  return code;
+}
+
+#define HIPCHECK(expr)                                          \
+  do {                                                          \
+    hipError_t e_ = (expr);                                     \
+    if (e_ != hipSuccess) return set_err((int)e_, #expr);       \
+  } while (0)
+
+static int check_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err((int)e, what);
+  return 0;
+}
+
+// Runs `fn` (which enqueues kernels on ctx->stream) and, when timing is on
+// and the name matches, brackets it with two events.
+template <typename F>
+static int run(snsg_ctx *ctx, const char *name, F fn) {
+  const bool t = ctx->timing && (ctx->filter.empty() || strstr(name, ctx->filter.c_str()));
+  if (t) {
+    while (ctx->ev_pool.size() < ctx->ev_used + 2) {
+      hipEvent_t e;
+      HIPCHECK(hipEventCreate(&e));
+      ctx->ev_pool.push_back(e);
+    }
+    HIPCHECK(hipEventRecord(ctx->ev_pool[ctx->ev_used], ctx->stream));
+  }
+  fn();
+  int rc = check_launch(name);
+  if (rc) return rc;
+  if (t) {
+    HIPCHECK(hipEventRecord(ctx->ev_pool[ctx->ev_used + 1], ctx->stream));
+    ctx->ev_used += 2;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------- creation
+
+static bool fft_plan(int n, int itot, FftPlan &pl) {
+  if (n < 2 || n > 8192) return false;
+  int rest = n, np = 0;
+  unsigned long long code = 0;
+  auto push = [&](int r) {
+    code |= (unsigned long long)r << (4 * np);
+    np++;
+    rest /= r;
+  };
+  while (rest % 8 == 0 && np < 16) push(8);
+  while (rest % 4 == 0 && np < 16) push(4);
+  while (rest % 2 == 0 && np < 16) push(2);
+  while (rest % 3 == 0 && np < 16) push(3);
+  while (rest % 5 == 0 && np < 16) push(5);
+  if (rest != 1) return false;
+  pl.n = n;
+  pl.npass = np;
+  pl.code = code;
+  const int maxS = 8192 / n;
+  const int half = (itot + 1) / 2;
+  int S = (half + 255) / 256;
+  if (S < 1) S = 1;
+  if (S > maxS) S = maxS;
+  pl.S = S;
+  return true;
+}
+
+static int ctx_alloc(void **p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes ? bytes : 8);
+  if (e != hipSuccess) return set_err((int)e, "hipMalloc");
+  e = hipMemset(*p, 0, bytes ? bytes : 8);
+  if (e != hipSuccess) return set_err((int)e, "hipMemset");
+  return 0;
+}
+
+extern "C" int snsg_device_count(int *count) {
+  HIPCHECK(hipGetDeviceCount(count));
+  return 0;
+}
+
+extern "C" const char *snsg_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int snsg_create(int device, const snsg_config *cfg, const double *dxf,
+                           const double *dxc, snsg_ctx **out) {
+  if (!cfg || !dxf || !dxc || !out) return set_err(SNSG_E_ARG, "snsg_create: null argument");
+  const int itot = cfg->itot, jtot = cfg->jtot;
+  if (itot < 4 || jtot < 4) return set_err(SNSG_E_SIZE, "snsg_create: itot and jtot must be >= 4");
+  if (itot > 8192) return set_err(SNSG_E_SIZE, "snsg_create: itot > 8192 is not supported");
+  FftPlan pl;
+  if (!fft_plan(jtot, itot, pl))
+    return set_err(SNSG_E_SIZE,
+                   "snsg_create: jtot must be <= 8192 with prime factors 2, 3 and 5 only");
+  HIPCHECK(hipSetDevice(device));
+  snsg_ctx *c = new snsg_ctx();
+  c->device = device;
+  c->cfg = *cfg;
+  c->plan = pl;
+  int rc = 0;
+  auto fail = [&](int code) {
+    snsg_destroy(c);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(set_err(SNSG_E_ARG, "hipStreamCreate"));
+  if ((rc = ctx_alloc((void **)&c->d_dxf, sizeof(double) * itot))) return fail(rc);
+  if ((rc = ctx_alloc((void **)&c->d_dxc, sizeof(double) * (itot + 1)))) return fail(rc);
+  if ((rc = ctx_alloc((void **)&c->d_dt, sizeof(double)))) return fail(rc);
+  if ((rc = ctx_alloc((void **)&c->d_eigen, sizeof(double) * (jtot / 2 + 1)))) return fail(rc);
+  if ((rc = ctx_alloc((void **)&c->d_tw, sizeof(double2) * jtot))) return fail(rc);
+  if ((rc = ctx_alloc((void **)&c->d_psihat, sizeof(double2) * (size_t)itot * (jtot / 2 + 1))))
+    return fail(rc);
+  if ((rc = ctx_alloc((void **)&c->d_q, sizeof(double) * (size_t)(itot + 1) * jtot))) return fail(rc);
+  if ((rc = ctx_alloc((void **)&c->d_qy, sizeof(double) * (size_t)(itot + 1) * jtot))) return fail(rc);
+  if ((rc = ctx_alloc((void **)&c->d_red, sizeof(unsigned long long)))) return fail(rc);
+  if (hipHostMalloc((void **)&c->h_red, sizeof(unsigned long long), hipHostMallocDefault) !=
+      hipSuccess)
+    return fail(set_err(SNSG_E_NOMEM, "hipHostMalloc"));
+  if (hipMemcpy(c->d_dxf, dxf, sizeof(double) * itot, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_dxc, dxc, sizeof(double) * (itot + 1), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(set_err(SNSG_E_ARG, "upload grid"));
+  // host-evaluated constants, with the reference's operations
+  const double sqrtPr = sqrt(cfg->Pr), sqrtRa = sqrt(cfg->Ra);
+  const double dy = cfg->ly / jtot;  // src/param/init.c:187
+  Geo &g = c->geo;
+  g.itot = itot;
+  g.jtot = jtot;
+  g.dy = dy;
+  g.dxf = c->d_dxf;
+  g.dxc = c->d_dxc;
+  g.sqrtPr = sqrtPr;
+  g.sqrtRa = sqrtRa;
+  g.kappa_u = sqrtPr / sqrtRa;
+  g.kappa_T = 1. / sqrtPr / sqrtRa;
+  g.two_sqrtRa = 2. * sqrtRa;
+  g.two_sqrtPr_sqrtRa = 2. * sqrtPr * sqrtRa;
+  g.two_Re = 2. * (sqrtRa / sqrtPr);
+  c->treat = make_treat(cfg->implicitx, cfg->implicity);
+  // eigenvalues of the periodic second difference (src/fluid/compute_potential.c:112-115)
+  std::vector<double> eig(jtot / 2 + 1);
+  for (int J = 0; J <= jtot / 2; J++)
+    eig[J] = -4. / pow(dy, 2.) * pow(sin(M_PI * (J) / (1. * jtot)), 2.);
+  std::vector<double2> tw(jtot);
+  for (int m = 0; m < jtot; m++) {
+    const double a = 2. * M_PI * (double)m / (double)jtot;
+    tw[m] = make_double2(cos(a), -sin(a));
+  }
+  if (hipMemcpy(c->d_eigen, eig.data(), sizeof(double) * eig.size(), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(c->d_tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice) !=
+          hipSuccess)
+    return fail(set_err(SNSG_E_ARG, "upload tables"));
+  *out = c;
+  return 0;
+}
+
+extern "C" int snsg_destroy(snsg_ctx *c) {
+  if (!c) return 0;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  void *bufs[] = {c->d_dxf, c->d_dxc, c->d_dt, c->d_eigen, c->d_tw, c->d_psihat, c->d_q,
+                  c->d_qy, c->d_red, c->d_tf};
+  for (void *b : bufs)
+    if (b) hipFree(b);
+  for (int k = 0; k < 3; k++) {
+    if (c->d_srcb[k]) hipFree(c->d_srcb[k]);
+    if (c->d_srcg[k]) hipFree(c->d_srcg[k]);
+  }
+  if (c->h_red) hipHostFree(c->h_red);
+  for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+extern "C" int snsg_set_treatment(snsg_ctx *c, int implicitx, int implicity) {
+  if (!c) return set_err(SNSG_E_ARG, "null ctx");
+  c->cfg.implicitx = implicitx ? 1 : 0;
+  c->cfg.implicity = implicity ? 1 : 0;
+  c->treat = make_treat(c->cfg.implicitx, c->cfg.implicity);
+  return 0;
+}
+
+__global__ void k_set_f64(double *p, double v) { *p = v; }
+
+extern "C" int snsg_set_dt(snsg_ctx *c, double dt) {
+  if (!c) return set_err(SNSG_E_ARG, "null ctx");
+  HIPCHECK(hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_set_f64, dim3(1), dim3(1), 0, c->stream, c->d_dt, dt);
+  return check_launch("snsg_set_dt");
+}
+
+extern "C" int snsg_synchronize(snsg_ctx *c) {
+  if (!c) return set_err(SNSG_E_ARG, "null ctx");
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// --------------------------------------------------------------- memory
+
+extern "C" int snsg_malloc(snsg_ctx *c, size_t bytes, void **dptr) {
+  if (!c || !dptr) return set_err(SNSG_E_ARG, "snsg_malloc: null argument");
+  HIPCHECK(hipSetDevice(c->device));
+  return ctx_alloc(dptr, bytes);
+}
+
+extern "C" int snsg_free(snsg_ctx *c, void *dptr) {
+  if (!c) return set_err(SNSG_E_ARG, "null ctx");
+  if (!dptr) return 0;
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipFree(dptr));
+  return 0;
+}
+
+extern "C" int snsg_upload(snsg_ctx *c, void *dst, const void *src, size_t bytes) {
+  if (!c || !dst || !src) return set_err(SNSG_E_ARG, "snsg_upload: null argument");
+  HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int snsg_download(snsg_ctx *c, void *dst, const void *src, size_t bytes) {
+  if (!c || !dst || !src) return set_err(SNSG_E_ARG, "snsg_download: null argument");
+  HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// ------------------------------------------------------ boundary values
+
+#define CHECK_CTX(c, p) \
+  if (!(c) || !(p)) return set_err(SNSG_E_ARG, __func__)
+
+extern "C" int snsg_fluid_update_boundaries_ux(snsg_ctx *c, double *ux) {
+  CHECK_CTX(c, ux);
+  return run(c, "bc_ux", [&] { launch_bc_ux(c->stream, c->geo, ux); });
+}
+extern "C" int snsg_fluid_update_boundaries_uy(snsg_ctx *c, double *uy) {
+  CHECK_CTX(c, uy);
+  return run(c, "bc_uy", [&] { launch_bc_uy(c->stream, c->geo, uy); });
+}
+extern "C" int snsg_fluid_update_boundaries_p(snsg_ctx *c, double *p) {
+  CHECK_CTX(c, p);
+  return run(c, "bc_p", [&] { launch_bc_p(c->stream, c->geo, p); });
+}
+extern "C" int snsg_temperature_update_boundaries_temp(snsg_ctx *c, double *temp) {
+  CHECK_CTX(c, temp);
+  return run(c, "bc_temp", [&] { launch_bc_temp(c->stream, c->geo, temp); });
+}
+
+// ------------------------------------------------- predictor internals
+
+static RK rk_of(const snsg_ctx *c, int rkstep) {
+  RK r;
+  r.alpha = c->cfg.alpha[rkstep];
+  r.beta = c->cfg.beta[rkstep];
+  r.gamma = c->cfg.gamma[rkstep];
+  return r;
+}
+
+// Apply the RK increment of one field with the configured diffusive
+// treatment: src/temperature/update.c:89-231, src/fluid/update_velocity.c:194-488
+static int update_field(snsg_ctx *c, int kind, int rkstep, const double *srca, const double *srcb,
+                        const double *srcg, double *field) {
+  const Geo &g = c->geo;
+  const RK rk = rk_of(c, rkstep);
+  const int n = kind == KIND_UX ? g.itot - 1 : g.itot;
+  int rc;
+  if (!c->treat.implicity) {
+    if (!c->treat.implicitx)
+      return run(c, "incr", [&] {
+        launch_incr(c->stream, g, kind, rk, c->d_dt, srca, srcb, srcg, field, nullptr);
+      });
+    return run(c, "cnx", [&] {
+      launch_cnx(c->stream, g, kind, rk, c->d_dt, srca, srcb, srcg, nullptr, field, nullptr);
+    });
+  }
+  if (c->treat.implicitx)
+    rc = run(c, "cnx", [&] {
+      launch_cnx(c->stream, g, kind, rk, c->d_dt, srca, srcb, srcg, nullptr, nullptr, c->d_q);
+    });
+  else
+    rc = run(c, "incr", [&] {
+      launch_incr(c->stream, g, kind, rk, c->d_dt, srca, srcb, srcg, nullptr, c->d_q);
+    });
+  if (rc) return rc;
+  if ((rc = run(c, "transpose", [&] { launch_transpose(c->stream, n, g.jtot, 8, c->d_q, c->d_qy); })))
+    return rc;
+  if ((rc = run(c, "cny", [&] { launch_cny_rows(c->stream, g, kind, rk, c->d_dt, n, c->d_qy); })))
+    return rc;
+  if ((rc = run(c, "transpose", [&] { launch_transpose(c->stream, g.jtot, n, 8, c->d_qy, c->d_q); })))
+    return rc;
+  return run(c, "addq", [&] { launch_addq(c->stream, g, kind, c->d_q, field); });
+}
+
+extern "C" int snsg_temperature_compute_force(snsg_ctx *c, const double *temp, double *tf) {
+  CHECK_CTX(c, temp);
+  if (!tf) return set_err(SNSG_E_ARG, __func__);
+  return run(c, "force", [&] {
+    launch_force(c->stream, c->geo, c->cfg.with_thermal_forcing, temp, tf);
+  });
+}
+
+extern "C" int snsg_temperature_update_temp(snsg_ctx *c, int rkstep, const double *ux,
+                                            const double *uy, double *temp, double *srca,
+                                            const double *srcb, double *srcg) {
+  CHECK_CTX(c, temp);
+  if (!ux || !uy || !srca || !srcb || !srcg || rkstep < 0 || rkstep > 2)
+    return set_err(SNSG_E_ARG, __func__);
+  int rc = run(c, "src_temp", [&] {
+    launch_src_temp(c->stream, c->geo, c->treat, ux, uy, temp, srca, srcg);
+  });
+  if (rc) return rc;
+  return update_field(c, KIND_TEMP, rkstep, srca, srcb, srcg, temp);
+}
+
+extern "C" int snsg_fluid_update_velocity(snsg_ctx *c, int rkstep, double *ux, double *uy,
+                                          const double *p, const double *tf, double *srcuxa,
+                                          const double *srcuxb, double *srcuxg, double *srcuya,
+                                          const double *srcuyb, double *srcuyg) {
+  CHECK_CTX(c, ux);
+  if (!uy || !p || !tf || !srcuxa || !srcuxb || !srcuxg || !srcuya || !srcuyb || !srcuyg ||
+      rkstep < 0 || rkstep > 2)
+    return set_err(SNSG_E_ARG, __func__);
+  int rc = run(c, "src_vel", [&] {
+    launch_src_vel(c->stream, c->geo, c->treat, ux, uy, p, tf, srcuxa, srcuxg, srcuya, srcuyg);
+  });
+  if (rc) return rc;
+  if ((rc = update_field(c, KIND_UX, rkstep, srcuxa, srcuxb, srcuxg, ux))) return rc;
+  return update_field(c, KIND_UY, rkstep, srcuya, srcuyb, srcuyg, uy);
+}
+
+extern "C" int snsg_fluid_compute_potential(snsg_ctx *c, int rkstep, const double *ux,
+                                            const double *uy, double *psi) {
+  CHECK_CTX(c, psi);
+  if (!ux || !uy || rkstep < 0 || rkstep > 2) return set_err(SNSG_E_ARG, __func__);
+  const double gamma = c->cfg.gamma[rkstep];
+  int rc = run(c, "poisson_fwd", [&] {
+    launch_poisson_fwd(c->stream, c->geo, c->plan, gamma, c->d_dt, ux, uy, c->d_tw, c->d_psihat);
+  });
+  if (rc) return rc;
+  if ((rc = run(c, "poisson_x", [&] {
+         launch_poisson_x(c->stream, c->geo, c->d_eigen, c->d_psihat);
+       })))
+    return rc;
+  return run(c, "poisson_inv", [&] {
+    launch_poisson_inv(c->stream, c->geo, c->plan, c->d_tw, c->d_psihat, psi);
+  });
+}
+
+extern "C" int snsg_fluid_correct_velocity(snsg_ctx *c, int rkstep, const double *psi, double *ux,
+                                           double *uy) {
+  CHECK_CTX(c, psi);
+  if (!ux || !uy || rkstep < 0 || rkstep > 2) return set_err(SNSG_E_ARG, __func__);
+  return run(c, "correct", [&] {
+    launch_correct(c->stream, c->geo, c->cfg.gamma[rkstep], c->d_dt, psi, ux, uy);
+  });
+}
+
+extern "C" int snsg_fluid_update_pressure(snsg_ctx *c, int rkstep, const double *psi, double *p) {
+  CHECK_CTX(c, psi);
+  if (!p || rkstep < 0 || rkstep > 2) return set_err(SNSG_E_ARG, __func__);
+  return run(c, "pressure", [&] {
+    launch_pressure(c->stream, c->geo, c->treat, c->cfg.gamma[rkstep], c->d_dt, psi, p);
+  });
+}
+
+extern "C" int snsg_param_decide_dt(snsg_ctx *c, const double *ux, const double *uy,
+                                    double *dt_adv) {
+  CHECK_CTX(c, ux);
+  if (!uy || !dt_adv) return set_err(SNSG_E_ARG, __func__);
+  const unsigned long long inf = 0x7ff0000000000000ull;
+  int rc = run(c, "dt_reduce", [&] {
+    launch_fill_u64(c->stream, c->d_red, inf, 1);
+    launch_dt_reduce(c->stream, c->geo, ux, uy, c->d_red);
+  });
+  if (rc) return rc;
+  HIPCHECK(hipMemcpyAsync(c->h_red, c->d_red, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  double v;
+  memcpy(&v, c->h_red, sizeof v);
+  *dt_adv = v;
+  return 0;
+}
+
+extern "C" int snsg_transpose(snsg_ctx *c, int g_isize, int g_jsize, size_t elemsize,
+                              const void *send, void *recv) {
+  if (!c || !send || !recv || g_isize < 0 || g_jsize < 0 ||
+      (elemsize != 4 && elemsize != 8 && elemsize != 16))
+    return set_err(SNSG_E_ARG, __func__);
+  if (g_isize == 0 || g_jsize == 0) return 0;
+  return run(c, "transpose", [&] {
+    launch_transpose(c->stream, g_isize, g_jsize, elemsize, send, recv);
+  });
+}
+
+// ---------------------------------------------------------- fused step
+
+static size_t bytes_ux(const snsg_ctx *c) {
+  return sizeof(double) * (size_t)(c->geo.itot + 1) * (c->geo.jtot + 2);
+}
+static size_t bytes_c(const snsg_ctx *c) {
+  return sizeof(double) * (size_t)(c->geo.itot + 2) * (c->geo.jtot + 2);
+}
+
+extern "C" int snsg_state_alloc(snsg_ctx *c, snsg_state *st) {
+  if (!c || !st) return set_err(SNSG_E_ARG, __func__);
+  HIPCHECK(hipSetDevice(c->device));
+  memset(st, 0, sizeof *st);
+  const int itot = c->geo.itot, jtot = c->geo.jtot;
+  int rc;
+  if ((rc = ctx_alloc((void **)&st->ux, bytes_ux(c)))) return rc;
+  if ((rc = ctx_alloc((void **)&st->uy, bytes_c(c)))) return rc;
+  if ((rc = ctx_alloc((void **)&st->p, bytes_c(c)))) return rc;
+  if ((rc = ctx_alloc((void **)&st->psi, bytes_c(c)))) return rc;
+  if ((rc = ctx_alloc((void **)&st->temp, bytes_c(c)))) return rc;
+  if ((rc = ctx_alloc((void **)&st->temp_alt, bytes_c(c)))) return rc;
+  if ((rc = ctx_alloc((void **)&st->ux_star, bytes_ux(c)))) return rc;
+  if ((rc = ctx_alloc((void **)&st->uy_star, bytes_c(c)))) return rc;
+  if ((rc = ctx_alloc((void **)&st->srcuxa, sizeof(double) * (size_t)(itot - 1) * jtot))) return rc;
+  if ((rc = ctx_alloc((void **)&st->srcuya, sizeof(double) * (size_t)itot * jtot))) return rc;
+  if ((rc = ctx_alloc((void **)&st->srctempa, sizeof(double) * (size_t)itot * jtot))) return rc;
+  return 0;
+}
+
+extern "C" int snsg_state_free(snsg_ctx *c, snsg_state *st) {
+  if (!c || !st) return set_err(SNSG_E_ARG, __func__);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  double *ptrs[] = {st->ux, st->uy, st->p, st->psi, st->temp, st->temp_alt, st->ux_star,
+                    st->uy_star, st->srcuxa, st->srcuya, st->srctempa};
+  for (double *p : ptrs)
+    if (p) hipFree(p);
+  memset(st, 0, sizeof *st);
+  return 0;
+}
+
+// One RK3 step (src/main.c:79-101), composed from the per-function kernels
+// with the source histories kept on the device.
+extern "C" int snsg_integrate(snsg_ctx *c, snsg_state *st) {
+  if (!c || !st || !st->ux) return set_err(SNSG_E_ARG, __func__);
+  const int itot = c->geo.itot, jtot = c->geo.jtot;
+  int rc;
+  if (!c->d_tf) {
+    if ((rc = ctx_alloc((void **)&c->d_tf, sizeof(double) * (size_t)(itot - 1) * jtot))) return rc;
+    const size_t nb[3] = {(size_t)itot * jtot, (size_t)(itot - 1) * jtot, (size_t)itot * jtot};
+    for (int k = 0; k < 3; k++) {
+      if ((rc = ctx_alloc((void **)&c->d_srcb[k], sizeof(double) * nb[k]))) return rc;
+      if ((rc = ctx_alloc((void **)&c->d_srcg[k], sizeof(double) * nb[k]))) return rc;
+    }
+  }
+  for (int rk = 0; rk < 3; rk++) {
+    if (c->cfg.with_temperature) {
+      if ((rc = snsg_temperature_compute_force(c, st->temp, c->d_tf))) return rc;
+      std::swap(st->srctempa, c->d_srcb[0]);
+      if ((rc = snsg_temperature_update_temp(c, rk, st->ux, st->uy, st->temp, st->srctempa,
+                                             c->d_srcb[0], c->d_srcg[0])))
+        return rc;
+    }
+    std::swap(st->srcuxa, c->d_srcb[1]);
+    std::swap(st->srcuya, c->d_srcb[2]);
+    if ((rc = snsg_fluid_update_velocity(c, rk, st->ux, st->uy, st->p, c->d_tf, st->srcuxa,
+                                         c->d_srcb[1], c->d_srcg[1], st->srcuya, c->d_srcb[2],
+                                         c->d_srcg[2])))
+      return rc;
+    if ((rc = snsg_fluid_compute_potential(c, rk, st->ux, st->uy, st->psi))) return rc;
+    if ((rc = snsg_fluid_correct_velocity(c, rk, st->psi, st->ux, st->uy))) return rc;
+    if ((rc = snsg_fluid_update_pressure(c, rk, st->psi, st->p))) return rc;
+  }
+  return 0;
+}
+
+// --------------------------------------------------------------- timing
+
+extern "C" int snsg_timing_enable(snsg_ctx *c, const char *filter) {
+  if (!c) return set_err(SNSG_E_ARG, __func__);
+  c->timing = true;
+  c->filter = filter ? filter : "";
+  c->ev_used = 0;
+  return 0;
+}
+
+extern "C" int snsg_timing_disable(snsg_ctx *c) {
+  if (!c) return set_err(SNSG_E_ARG, __func__);
+  c->timing = false;
+  return 0;
+}
+
+extern "C" int snsg_timing_read(snsg_ctx *c, int *launches, double *total_ms) {
+  if (!c || !launches || !total_ms) return set_err(SNSG_E_ARG, __func__);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  double tot = 0.;
+  for (size_t k = 0; k + 1 < c->ev_used; k += 2) {
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, c->ev_pool[k], c->ev_pool[k + 1]));
+    tot += ms;
+  }
+  *launches = (int)(c->ev_used / 2);
+  *total_ms = tot;
+  c->ev_used = 0;
+  return 0;
+}

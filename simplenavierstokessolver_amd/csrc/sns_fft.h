@@ -1,0 +1,173 @@
+// sns_fft.h -- batched complex FFT staged in LDS (gfx950).
+//
+// Stockham auto-sort passes (no bit reversal): the pass with radix R and
+// running sub-transform length Ns reads x_r = buf[j + r*n/R], applies the
+// twiddles w^(r*k), k = j mod Ns, w = exp(sign*2*pi*i/(Ns*R)), performs an
+// R-point DFT and writes buf[(j-k)*R + k + r*Ns].  Every thread loads all
+// its butterfly inputs before a barrier and stores after it, so one LDS
+// buffer is reused in place.  Radices 2, 3, 4, 5 and 8 are supported; the
+// pass schedule is packed four bits per pass into FftPlan::code.
+//
+// Twiddles come from a table tw[m] = exp(-2*pi*i*m/n) evaluated on the host;
+// the inverse transform uses their conjugates.  Forward: sign = -1,
+// unnormalised, i.e. the FFTW r2c/c2r convention of the reference
+// (src/fluid/compute_potential.c:53-67).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace sns {
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+// multiply by sign*i
+template <int SIGN>
+__device__ __forceinline__ double2 cmul_si(double2 a) {
+  return SIGN < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
+}
+
+template <int R, int SIGN>
+struct Dft;
+
+template <int SIGN>
+struct Dft<2, SIGN> {
+  __device__ __forceinline__ static void run(double2 *x) {
+    const double2 a = x[0], b = x[1];
+    x[0] = cadd(a, b);
+    x[1] = csub(a, b);
+  }
+};
+
+template <int SIGN>
+struct Dft<4, SIGN> {
+  __device__ __forceinline__ static void run(double2 *x) {
+    const double2 t0 = cadd(x[0], x[2]), t1 = csub(x[0], x[2]);
+    const double2 t2 = cadd(x[1], x[3]), t3 = cmul_si<SIGN>(csub(x[1], x[3]));
+    x[0] = cadd(t0, t2);
+    x[2] = csub(t0, t2);
+    x[1] = cadd(t1, t3);
+    x[3] = csub(t1, t3);
+  }
+};
+
+template <int SIGN>
+struct Dft<8, SIGN> {
+  __device__ __forceinline__ static void run(double2 *x) {
+    constexpr double c = 0.70710678118654752440;  // sqrt(2)/2
+    double2 e[4] = {x[0], x[2], x[4], x[6]};
+    double2 o[4] = {x[1], x[3], x[5], x[7]};
+    Dft<4, SIGN>::run(e);
+    Dft<4, SIGN>::run(o);
+    // o_k *= exp(sign*2*pi*i*k/8)
+    const double2 o1 = make_double2(c * (o[1].x - SIGN * o[1].y), c * (o[1].y + SIGN * o[1].x));
+    const double2 o2 = cmul_si<SIGN>(o[2]);
+    const double2 o3 = make_double2(-c * (o[3].x + SIGN * o[3].y), c * (SIGN * o[3].x - o[3].y));
+    x[0] = cadd(e[0], o[0]);
+    x[4] = csub(e[0], o[0]);
+    x[1] = cadd(e[1], o1);
+    x[5] = csub(e[1], o1);
+    x[2] = cadd(e[2], o2);
+    x[6] = csub(e[2], o2);
+    x[3] = cadd(e[3], o3);
+    x[7] = csub(e[3], o3);
+  }
+};
+
+template <int SIGN>
+struct Dft<3, SIGN> {
+  __device__ __forceinline__ static void run(double2 *x) {
+    constexpr double h = 0.86602540378443864676;  // sqrt(3)/2
+    const double2 t1 = cadd(x[1], x[2]);
+    const double2 t2 = csub(x[0], cscale(t1, 0.5));
+    const double2 t3 = cscale(cmul_si<SIGN>(csub(x[1], x[2])), h);
+    x[0] = cadd(x[0], t1);
+    x[1] = cadd(t2, t3);
+    x[2] = csub(t2, t3);
+  }
+};
+
+template <int SIGN>
+struct Dft<5, SIGN> {
+  __device__ __forceinline__ static void run(double2 *x) {
+    constexpr double c1 = 0.30901699437494742410;   // cos(2 pi/5)
+    constexpr double c2 = -0.80901699437494742410;  // cos(4 pi/5)
+    constexpr double s1 = 0.95105651629515357212;   // sin(2 pi/5)
+    constexpr double s2 = 0.58778525229247312917;   // sin(4 pi/5)
+    const double2 t1 = cadd(x[1], x[4]), t2 = cadd(x[2], x[3]);
+    const double2 t3 = csub(x[1], x[4]), t4 = csub(x[2], x[3]);
+    const double2 a1 = cadd(x[0], cadd(cscale(t1, c1), cscale(t2, c2)));
+    const double2 a2 = cadd(x[0], cadd(cscale(t1, c2), cscale(t2, c1)));
+    const double2 b1 = cmul_si<SIGN>(cadd(cscale(t3, s1), cscale(t4, s2)));
+    const double2 b2 = cmul_si<SIGN>(csub(cscale(t3, s2), cscale(t4, s1)));
+    x[0] = cadd(x[0], cadd(t1, t2));
+    x[1] = cadd(a1, b1);
+    x[4] = csub(a1, b1);
+    x[2] = cadd(a2, b2);
+    x[3] = csub(a2, b2);
+  }
+};
+
+template <int NT, int EMAX, int R, int SIGN>
+__device__ __forceinline__ void fft_pass(double2 *buf, int S, int n, int Ns,
+                                         const double2 *__restrict__ tw) {
+  constexpr int BMAX = (EMAX + R - 1) / R;
+  const int nb = n / R;
+  const int total = S * nb;
+  const int tstep = n / (Ns * R);
+  double2 x[BMAX][R];
+#pragma unroll
+  for (int k = 0; k < BMAX; k++) {
+    const int bidx = threadIdx.x + k * NT;
+    if (bidx < total) {
+      const int s = bidx / nb, j = bidx - s * nb;
+#pragma unroll
+      for (int r = 0; r < R; r++) x[k][r] = buf[s * n + j + r * nb];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < BMAX; k++) {
+    const int bidx = threadIdx.x + k * NT;
+    if (bidx < total) {
+      const int s = bidx / nb, j = bidx - s * nb;
+      const int kk = j % Ns;
+      if (kk != 0) {
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+          double2 w = tw[r * kk * tstep];
+          if (SIGN > 0) w.y = -w.y;
+          x[k][r] = cmul(x[k][r], w);
+        }
+      }
+      Dft<R, SIGN>::run(x[k]);
+      const int base = s * n + (j - kk) * R + kk;
+#pragma unroll
+      for (int r = 0; r < R; r++) buf[base + r * Ns] = x[k][r];
+    }
+  }
+  __syncthreads();
+}
+
+// In-place transform of S sequences of length n stored back to back in buf.
+// EMAX bounds S*n/NT (complex values per thread).
+template <int NT, int EMAX, int SIGN>
+__device__ void fft_lds(double2 *buf, int S, int n, int npass, unsigned long long code,
+                        const double2 *__restrict__ tw) {
+  int Ns = 1;
+  for (int p = 0; p < npass; p++) {
+    const int R = (int)((code >> (4 * p)) & 15ull);
+    switch (R) {
+      case 2: fft_pass<NT, EMAX, 2, SIGN>(buf, S, n, Ns, tw); break;
+      case 3: fft_pass<NT, EMAX, 3, SIGN>(buf, S, n, Ns, tw); break;
+      case 4: fft_pass<NT, EMAX, 4, SIGN>(buf, S, n, Ns, tw); break;
+      case 5: fft_pass<NT, EMAX, 5, SIGN>(buf, S, n, Ns, tw); break;
+      default: fft_pass<NT, EMAX, 8, SIGN>(buf, S, n, Ns, tw); break;
+    }
+    Ns *= R;
+  }
+}
+
+}  // namespace sns

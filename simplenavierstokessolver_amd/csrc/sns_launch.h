@@ -1,0 +1,97 @@
+// sns_launch.h -- host-side launchers of the gfx950 kernels (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "sns_device.h"
+
+namespace sns {
+
+// Weights of the explicit / implicit split of the diffusive terms, exactly
+// (1.-1.*implicit) and (+1.*implicit) as in the reference.
+struct Treat {
+  double w_ex_x, w_ex_y, w_im_x, w_im_y;
+  int implicitx, implicity;
+};
+
+inline Treat make_treat(int implicitx, int implicity) {
+  Treat t;
+  t.implicitx = implicitx;
+  t.implicity = implicity;
+  t.w_ex_x = 1. - 1. * implicitx;
+  t.w_ex_y = 1. - 1. * implicity;
+  t.w_im_x = +1. * implicitx;
+  t.w_im_y = +1. * implicity;
+  return t;
+}
+
+struct RK {
+  double alpha, beta, gamma;
+};
+
+// ---- boundary values (src/fluid/boundary_conditions.c, src/temperature/boundary_conditions.c)
+void launch_bc_ux(hipStream_t s, const Geo &g, double *ux);
+void launch_bc_uy(hipStream_t s, const Geo &g, double *uy);
+void launch_bc_p(hipStream_t s, const Geo &g, double *p);
+void launch_bc_temp(hipStream_t s, const Geo &g, double *temp);
+
+// ---- temperature
+void launch_force(hipStream_t s, const Geo &g, int with_thermal_forcing, const double *temp,
+                  double *tempforcex);
+void launch_src_temp(hipStream_t s, const Geo &g, const Treat &tr, const double *ux,
+                     const double *uy, const double *temp, double *srca, double *srcg);
+// ---- fluid
+void launch_src_vel(hipStream_t s, const Geo &g, const Treat &tr, const double *ux,
+                    const double *uy, const double *p, const double *tempforcex, double *srcuxa,
+                    double *srcuxg, double *srcuya, double *srcuyg);
+
+// ---- RK increment of one field (kind): either added to the field with its
+// boundary values refreshed (q == nullptr) or stored x-aligned in q.
+void launch_incr(hipStream_t s, const Geo &g, int kind, RK rk, const double *dtp,
+                 const double *srca, const double *srcb, const double *srcg, double *field,
+                 double *q);
+// field += q (x-aligned), then boundary values
+void launch_addq(hipStream_t s, const Geo &g, int kind, const double *q, double *field);
+
+// ---- Crank-Nicolson in x: one workgroup per row.  Input increment from the
+// sources (qin == nullptr) or from qin; output to the field (qout == nullptr)
+// or to qout (x-aligned).
+void launch_cnx(hipStream_t s, const Geo &g, int kind, RK rk, const double *dtp,
+                const double *srca, const double *srcb, const double *srcg, const double *qin,
+                double *field, double *qout);
+// ---- Crank-Nicolson in y on y-aligned rows (after a transpose): periodic
+// solve of each of nrows rows of length jtot, in place.
+void launch_cny_rows(hipStream_t s, const Geo &g, int kind, RK rk, const double *dtp, int nrows,
+                     double *qy);
+
+// ---- transposes: recv[i*g_jsize + j] = send[j*g_isize + i]
+void launch_transpose(hipStream_t s, int g_isize, int g_jsize, size_t elemsize, const void *send,
+                      void *recv);
+
+// ---- Poisson (src/fluid/compute_potential.c)
+struct FftPlan {
+  int n;                    // transform length (jtot)
+  int npass;                // number of radix passes
+  int S;                    // complex sequences per workgroup (= real columns / 2)
+  unsigned long long code;  // radix of pass p in bits [4p, 4p+4)
+};
+void launch_poisson_fwd(hipStream_t s, const Geo &g, const FftPlan &pl, double gamma,
+                        const double *dtp, const double *ux, const double *uy,
+                        const double2 *tw, double2 *psihat);
+void launch_poisson_x(hipStream_t s, const Geo &g, const double *eigen, double2 *psihat);
+void launch_poisson_inv(hipStream_t s, const Geo &g, const FftPlan &pl, const double2 *tw,
+                        const double2 *psihat, double *psi);
+
+// ---- correction and pressure (src/fluid/correct_velocity.c, update_pressure.c)
+void launch_correct(hipStream_t s, const Geo &g, double gamma, const double *dtp,
+                    const double *psi, double *ux, double *uy);
+void launch_pressure(hipStream_t s, const Geo &g, const Treat &tr, double gamma,
+                     const double *dtp, const double *psi, double *p);
+
+// ---- advective time-step limit (src/param/decide_dt.c:25-45): writes the
+// minimum of DXC(i)/max(|UX|,1e-8), dy/max(|UY|,1e-8) and 5e-2 (before the
+// safety factor) as its bit pattern to *out (which must hold 0x7ff0... on entry)
+void launch_dt_reduce(hipStream_t s, const Geo &g, const double *ux, const double *uy,
+                      unsigned long long *out);
+void launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v, int n);
+
+}  // namespace sns

@@ -1,0 +1,441 @@
+// sns_solve.hip -- line solvers, transposes and the Poisson chain (gfx950).
+#include "sns_launch.h"
+#include "sns_tridiag.h"
+#include "sns_fft.h"
+
+namespace sns {
+
+// ------------------------------------------------------------ helpers
+
+template <int KIND>
+__device__ __forceinline__ double cn_prefactor(const Geo &g, double gamma, double dt) {
+  // T:     (gamma*dt)/(2.*sqrt(Pr)*sqrt(Ra))   src/temperature/update.c:142
+  // ux,uy: (gamma*dt*sqrt(Pr))/(2.*sqrt(Ra))   src/fluid/update_velocity.c:254,399
+  return KIND == KIND_TEMP ? (gamma * dt) / g.two_sqrtPr_sqrtRa
+                           : (gamma * dt * g.sqrtPr) / g.two_sqrtRa;
+}
+
+template <int KIND>
+__device__ __forceinline__ void cnx_coef(const Geo &g, double pf, int i, double &l, double &c,
+                                         double &u) {
+  if (KIND == KIND_UX) cnx_coef_ux(g, pf, i, l, c, u);
+  else if (KIND == KIND_UY) cnx_coef_uy(g, pf, i, l, c, u);
+  else cnx_coef_temp(g, pf, i, l, c, u);
+}
+
+template <int KIND>
+__device__ __forceinline__ void store_field_bc(const Geo &g, double *f, int i, int j, double v);
+
+// boundary-complete stores (same rules as sns_stencil.hip)
+template <>
+__device__ __forceinline__ void store_field_bc<KIND_UX>(const Geo &g, double *f, int i, int j,
+                                                        double v) {
+  f[ux_idx(g, i, j)] = v;
+  if (j == 1) f[ux_idx(g, i, g.jtot + 1)] = v;
+  if (j == g.jtot) f[ux_idx(g, i, 0)] = v;
+  if (i == 2) {
+    f[ux_idx(g, 1, j)] = 0.;
+    if (j == 1) f[ux_idx(g, 1, g.jtot + 1)] = 0.;
+    if (j == g.jtot) f[ux_idx(g, 1, 0)] = 0.;
+  }
+  if (i == g.itot) {
+    f[ux_idx(g, g.itot + 1, j)] = 0.;
+    if (j == 1) f[ux_idx(g, g.itot + 1, g.jtot + 1)] = 0.;
+    if (j == g.jtot) f[ux_idx(g, g.itot + 1, 0)] = 0.;
+  }
+}
+template <int WALLKIND>
+__device__ __forceinline__ void store_c_bc(const Geo &g, double *f, int i, int j, double v) {
+  f[c_idx(g, i, j)] = v;
+  if (j == 1) f[c_idx(g, i, g.jtot + 1)] = v;
+  if (j == g.jtot) f[c_idx(g, i, 0)] = v;
+  for (int side = 0; side < 2; side++) {
+    const bool hit = side == 0 ? (i == 1) : (i == g.itot);
+    if (!hit) continue;
+    const int iw = side == 0 ? 0 : g.itot + 1;
+    const double w = WALLKIND == 0 ? 0. : (WALLKIND == 1 ? v : (side == 0 ? +0.5 : -0.5));
+    f[c_idx(g, iw, j)] = w;
+    if (WALLKIND != 2) {
+      if (j == 1) f[c_idx(g, iw, g.jtot + 1)] = w;
+      if (j == g.jtot) f[c_idx(g, iw, 0)] = w;
+    }
+  }
+}
+template <>
+__device__ __forceinline__ void store_field_bc<KIND_UY>(const Geo &g, double *f, int i, int j,
+                                                        double v) {
+  store_c_bc<0>(g, f, i, j, v);
+}
+template <>
+__device__ __forceinline__ void store_field_bc<KIND_TEMP>(const Geo &g, double *f, int i, int j,
+                                                          double v) {
+  store_c_bc<2>(g, f, i, j, v);
+}
+
+template <int KIND>
+__device__ __forceinline__ double field_at(const Geo &g, const double *f, int i, int j) {
+  return KIND == KIND_UX ? f[ux_idx(g, i, j)] : f[c_idx(g, i, j)];
+}
+
+// ------------------------------------------------- Crank-Nicolson in x
+
+// One workgroup per row j: assemble the increment (from the RK sources or
+// from qin), solve (1 - pf*Dxx) q = increment along x, then either add to the
+// field with boundary values or store to qout.
+// src/temperature/update.c:123-167; src/fluid/update_velocity.c:230-280,380-424
+template <int KIND, int NT, int M>
+__global__ void __launch_bounds__(NT)
+    k_cnx(Geo g, RK rk, const double *__restrict__ dtp, const double *__restrict__ srca,
+          const double *__restrict__ srcb, const double *__restrict__ srcg,
+          const double *__restrict__ qin, double *__restrict__ field, double *__restrict__ qout) {
+  __shared__ TridiagLds<NT, 1> lds;
+  const int j = blockIdx.x + 1;
+  const int n = KIND == KIND_UX ? g.itot - 1 : g.itot;
+  const int i0 = KIND == KIND_UX ? 2 : 1;
+  const double dt = *dtp;
+  const double pf = cn_prefactor<KIND>(g, rk.gamma, dt);
+  double a[M], b[M], c[M], d[1][M];
+  const size_t rowoff = (size_t)(j - 1) * n;
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const int r = threadIdx.x * M + k;
+    if (r < n) {
+      double l, cc, u;
+      cnx_coef<KIND>(g, pf, r + i0, l, cc, u);
+      a[k] = (r == 0) ? 0. : l;
+      b[k] = cc;
+      c[k] = (r == n - 1) ? 0. : u;
+      const size_t s = rowoff + r;
+      d[0][k] = qin ? qin[s] : rk_incr(rk.alpha, rk.beta, rk.gamma, dt, srca[s], srcb[s], srcg[s]);
+    } else {
+      a[k] = 0.; b[k] = 1.; c[k] = 0.; d[0][k] = 0.;
+    }
+  }
+  block_tridiag<NT, M, 1>(a, b, c, d, lds);
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const int r = threadIdx.x * M + k;
+    if (r < n) {
+      if (qout) {
+        qout[rowoff + r] = d[0][k];
+      } else {
+        const int i = r + i0;
+        store_field_bc<KIND>(g, field, i, j, field_at<KIND>(g, field, i, j) + d[0][k]);
+      }
+    }
+  }
+}
+
+template <int KIND, int NT, int M>
+static void cnx_launch(hipStream_t s, const Geo &g, RK rk, const double *dtp, const double *srca,
+                       const double *srcb, const double *srcg, const double *qin, double *field,
+                       double *qout) {
+  hipLaunchKernelGGL((k_cnx<KIND, NT, M>), dim3(g.jtot), dim3(NT), 0, s, g, rk, dtp, srca, srcb,
+                     srcg, qin, field, qout);
+}
+
+template <int KIND>
+static void cnx_dispatch(hipStream_t s, const Geo &g, RK rk, const double *dtp, const double *srca,
+                         const double *srcb, const double *srcg, const double *qin, double *field,
+                         double *qout) {
+  const int n = KIND == KIND_UX ? g.itot - 1 : g.itot;
+  if (n <= 64 * 4) cnx_launch<KIND, 64, 4>(s, g, rk, dtp, srca, srcb, srcg, qin, field, qout);
+  else if (n <= 256 * 4) cnx_launch<KIND, 256, 4>(s, g, rk, dtp, srca, srcb, srcg, qin, field, qout);
+  else if (n <= 256 * 8) cnx_launch<KIND, 256, 8>(s, g, rk, dtp, srca, srcb, srcg, qin, field, qout);
+  else if (n <= 256 * 16) cnx_launch<KIND, 256, 16>(s, g, rk, dtp, srca, srcb, srcg, qin, field, qout);
+  else cnx_launch<KIND, 256, 32>(s, g, rk, dtp, srca, srcb, srcg, qin, field, qout);
+}
+
+void launch_cnx(hipStream_t s, const Geo &g, int kind, RK rk, const double *dtp,
+                const double *srca, const double *srcb, const double *srcg, const double *qin,
+                double *field, double *qout) {
+  if (kind == KIND_UX) cnx_dispatch<KIND_UX>(s, g, rk, dtp, srca, srcb, srcg, qin, field, qout);
+  else if (kind == KIND_UY) cnx_dispatch<KIND_UY>(s, g, rk, dtp, srca, srcb, srcg, qin, field, qout);
+  else cnx_dispatch<KIND_TEMP>(s, g, rk, dtp, srca, srcb, srcg, qin, field, qout);
+}
+
+// ------------------------------------------------- Crank-Nicolson in y
+
+// Periodic solve of one y-aligned row of length n = jtot in place, by the
+// reference's superposition of two (n-1) systems (src/linalg.c:52-108): the
+// row's own right-hand side and the matrix-only right-hand side
+// [-l, 0, .., 0, -u] share one elimination (NR = 2).
+// Coefficients: src/temperature/update.c:188-197, src/fluid/update_velocity.c:302-310
+template <int KIND, int NT, int M>
+__global__ void __launch_bounds__(NT)
+    k_cny_rows(Geo g, RK rk, const double *__restrict__ dtp, double *__restrict__ qy) {
+  __shared__ TridiagLds<NT, 2> lds;
+  __shared__ double ends[4];
+  const int n = g.jtot;
+  double *q = qy + (size_t)blockIdx.x * n;
+  const double dt = *dtp;
+  const double pf = cn_prefactor<KIND>(g, rk.gamma, dt);
+  const double coef_ym = 1. / g.dy / g.dy;
+  const double coef_yp = 1. / g.dy / g.dy;
+  const double coef_y0 = -coef_ym - coef_yp;
+  const double l = -pf * coef_ym;
+  const double u = -pf * coef_yp;
+  const double cc = 1. - pf * coef_y0;
+  const double qlast = q[n - 1];
+  const int m = n - 1;  // unknowns of the reduced (non-periodic) systems
+  double a[M], b[M], c[M], d[2][M];
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const int r = threadIdx.x * M + k;
+    if (r < m) {
+      a[k] = (r == 0) ? 0. : l;
+      b[k] = cc;
+      c[k] = (r == m - 1) ? 0. : u;
+      d[0][k] = q[r];
+      d[1][k] = (r == 0) ? -l : (r == m - 1 ? -u : 0.);
+      if (m == 1) d[1][k] = -l;  // single unknown: both couplings on row 0
+    } else {
+      a[k] = 0.; b[k] = 1.; c[k] = 0.; d[0][k] = 0.; d[1][k] = 0.;
+    }
+  }
+  block_tridiag<NT, M, 2>(a, b, c, d, lds);
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const int r = threadIdx.x * M + k;
+    if (r == 0) { ends[0] = d[0][k]; ends[2] = d[1][k]; }
+    if (r == m - 1) { ends[1] = d[0][k]; ends[3] = d[1][k]; }
+  }
+  __syncthreads();
+  const double xl = (qlast - u * ends[0] - l * ends[1]) / (cc + u * ends[2] + l * ends[3]);
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const int r = threadIdx.x * M + k;
+    if (r < m) q[r] = d[0][k] + xl * d[1][k];
+  }
+  if (threadIdx.x == 0) q[n - 1] = xl;
+}
+
+template <int KIND, int NT, int M>
+static void cny_launch(hipStream_t s, const Geo &g, RK rk, const double *dtp, int nrows,
+                       double *qy) {
+  hipLaunchKernelGGL((k_cny_rows<KIND, NT, M>), dim3(nrows), dim3(NT), 0, s, g, rk, dtp, qy);
+}
+
+template <int KIND>
+static void cny_dispatch(hipStream_t s, const Geo &g, RK rk, const double *dtp, int nrows,
+                         double *qy) {
+  const int m = g.jtot - 1;
+  if (m <= 64 * 4) cny_launch<KIND, 64, 4>(s, g, rk, dtp, nrows, qy);
+  else if (m <= 256 * 4) cny_launch<KIND, 256, 4>(s, g, rk, dtp, nrows, qy);
+  else if (m <= 256 * 8) cny_launch<KIND, 256, 8>(s, g, rk, dtp, nrows, qy);
+  else if (m <= 256 * 16) cny_launch<KIND, 256, 16>(s, g, rk, dtp, nrows, qy);
+  else cny_launch<KIND, 256, 32>(s, g, rk, dtp, nrows, qy);
+}
+
+void launch_cny_rows(hipStream_t s, const Geo &g, int kind, RK rk, const double *dtp, int nrows,
+                     double *qy) {
+  if (kind == KIND_UX) cny_dispatch<KIND_UX>(s, g, rk, dtp, nrows, qy);
+  else if (kind == KIND_UY) cny_dispatch<KIND_UY>(s, g, rk, dtp, nrows, qy);
+  else cny_dispatch<KIND_TEMP>(s, g, rk, dtp, nrows, qy);
+}
+
+// ------------------------------------------------------------ transpose
+
+// recv[i*g_jsize + j] = send[j*g_isize + i] through a padded LDS tile, so
+// both the read and the write are row-contiguous (the permutation of
+// src/parallel/transpose.c:6-85 on one rank).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_transpose(int g_isize, int g_jsize, const T *__restrict__ send, T *__restrict__ recv) {
+  __shared__ T tile[32][33];
+  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + tx, j = j0 + r;
+    if (i < g_isize && j < g_jsize) tile[r][tx] = send[(size_t)j * g_isize + i];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int j = j0 + tx, i = i0 + r;
+    if (i < g_isize && j < g_jsize) recv[(size_t)i * g_jsize + j] = tile[tx][r];
+  }
+}
+
+void launch_transpose(hipStream_t s, int g_isize, int g_jsize, size_t elemsize, const void *send,
+                      void *recv) {
+  const dim3 gr((g_isize + 31) / 32, (g_jsize + 31) / 32);
+  if (elemsize == 4)
+    hipLaunchKernelGGL(k_transpose<uint32_t>, gr, dim3(256), 0, s, g_isize, g_jsize,
+                       (const uint32_t *)send, (uint32_t *)recv);
+  else if (elemsize == 8)
+    hipLaunchKernelGGL(k_transpose<uint2>, gr, dim3(256), 0, s, g_isize, g_jsize,
+                       (const uint2 *)send, (uint2 *)recv);
+  else
+    hipLaunchKernelGGL(k_transpose<uint4>, gr, dim3(256), 0, s, g_isize, g_jsize,
+                       (const uint4 *)send, (uint4 *)recv);
+}
+
+// --------------------------------------------------------------- Poisson
+
+static constexpr int kFftThreads = 256;
+static constexpr int kFftMaxElems = 8192;  // complex values per workgroup (128 KB LDS)
+
+// Right-hand side of the Poisson equation at cell (i, j):
+// 1./(gamma*dt)*((ux_xp-ux_xm)/DXF(i) + (uy_yp-uy_ym)/dy)
+// (src/fluid/compute_potential.c:74-86)
+__device__ __forceinline__ double poisson_rhs(const Geo &g, const double *ux, const double *uy,
+                                              double inv_gdt, int i, int j) {
+  const double ux_xm = ux[ux_idx(g, i, j)];
+  const double ux_xp = ux[ux_idx(g, i + 1, j)];
+  const double uy_ym = uy[c_idx(g, i, j)];
+  const double uy_yp = uy[c_idx(g, i, j + 1)];
+  return inv_gdt * (+(ux_xp - ux_xm) / DXF(g, i) + (uy_yp - uy_ym) / g.dy);
+}
+
+// Forward half of the Poisson solve for a strip of 2*S columns: RHS, real
+// FFT along y of each column (two real columns packed as one complex
+// sequence), spectrum stored x-aligned in psihat[J*itot + (i-1)],
+// J = 0..jtot/2 (src/fluid/compute_potential.c:69-92).
+__global__ void __launch_bounds__(kFftThreads)
+    k_poisson_fwd(Geo g, FftPlan pl, double gamma, const double *__restrict__ dtp,
+                  const double *__restrict__ ux, const double *__restrict__ uy,
+                  const double2 *__restrict__ tw, double2 *__restrict__ psihat) {
+  extern __shared__ double2 buf[];
+  const int n = pl.n, S = pl.S;
+  const int ibase = 1 + 2 * S * blockIdx.x;
+  const double inv_gdt = 1. / (gamma * (*dtp));
+  for (int idx = threadIdx.x; idx < S * n; idx += kFftThreads) {
+    const int s = idx % S, j = idx / S;
+    const int ia = ibase + 2 * s, ib = ia + 1;
+    double2 z;
+    z.x = ia <= g.itot ? poisson_rhs(g, ux, uy, inv_gdt, ia, j + 1) : 0.;
+    z.y = ib <= g.itot ? poisson_rhs(g, ux, uy, inv_gdt, ib, j + 1) : 0.;
+    buf[s * n + j] = z;
+  }
+  __syncthreads();
+  fft_lds<kFftThreads, kFftMaxElems / kFftThreads, -1>(buf, S, pl.n, pl.npass, pl.code, tw);
+  const int nh = n / 2 + 1;
+  for (int idx = threadIdx.x; idx < S * nh; idx += kFftThreads) {
+    const int s = idx % S, k = idx / S;
+    const double2 zk = buf[s * n + k];
+    const double2 zn = buf[s * n + (k == 0 ? 0 : n - k)];
+    const int ia = ibase + 2 * s, ib = ia + 1;
+    double2 xa, xb;
+    xa.x = 0.5 * (zk.x + zn.x);
+    xa.y = 0.5 * (zk.y - zn.y);
+    xb.x = 0.5 * (zk.y + zn.y);
+    xb.y = 0.5 * (zn.x - zk.x);
+    if (ia <= g.itot) psihat[(size_t)k * g.itot + (ia - 1)] = xa;
+    if (ib <= g.itot) psihat[(size_t)k * g.itot + (ib - 1)] = xb;
+  }
+}
+
+// Inverse half: Hermitian-extended spectra of two columns packed as one
+// complex sequence, inverse FFT, psi = result/jtot with Neumann walls and
+// periodic halos (src/fluid/compute_potential.c:146-159).
+__global__ void __launch_bounds__(kFftThreads)
+    k_poisson_inv(Geo g, FftPlan pl, const double2 *__restrict__ tw,
+                  const double2 *__restrict__ psihat, double *__restrict__ psi) {
+  extern __shared__ double2 buf[];
+  const int n = pl.n, S = pl.S;
+  const int ibase = 1 + 2 * S * blockIdx.x;
+  const int nh = n / 2 + 1;
+  for (int idx = threadIdx.x; idx < S * nh; idx += kFftThreads) {
+    const int s = idx % S, k = idx / S;
+    const int ia = ibase + 2 * s, ib = ia + 1;
+    double2 xa = make_double2(0., 0.), xb = make_double2(0., 0.);
+    if (ia <= g.itot) xa = psihat[(size_t)k * g.itot + (ia - 1)];
+    if (ib <= g.itot) xb = psihat[(size_t)k * g.itot + (ib - 1)];
+    const bool selfconj = (k == 0) || (2 * k == n);
+    if (selfconj) { xa.y = 0.; xb.y = 0.; }
+    buf[s * n + k] = make_double2(xa.x - xb.y, xa.y + xb.x);
+    if (!selfconj) buf[s * n + (n - k)] = make_double2(xa.x + xb.y, -xa.y + xb.x);
+  }
+  __syncthreads();
+  fft_lds<kFftThreads, kFftMaxElems / kFftThreads, +1>(buf, S, pl.n, pl.npass, pl.code, tw);
+  const double njtot = 1. * g.jtot;
+  for (int idx = threadIdx.x; idx < S * n; idx += kFftThreads) {
+    const int s = idx % S, j = idx / S;
+    const int ia = ibase + 2 * s, ib = ia + 1;
+    const double2 z = buf[s * n + j];
+    if (ia <= g.itot) store_c_bc<1>(g, psi, ia, j + 1, z.x / njtot);
+    if (ib <= g.itot) store_c_bc<1>(g, psi, ib, j + 1, z.y / njtot);
+  }
+}
+
+static int fft_strips(const Geo &g, const FftPlan &pl) {
+  const int cols = 2 * pl.S;
+  return (g.itot + cols - 1) / cols;
+}
+
+// dynamic LDS above 64 KB must be enabled per kernel
+static void allow_lds(const void *fn, size_t bytes) {
+  if (bytes > 65536) hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+void launch_poisson_fwd(hipStream_t s, const Geo &g, const FftPlan &pl, double gamma,
+                        const double *dtp, const double *ux, const double *uy, const double2 *tw,
+                        double2 *psihat) {
+  const size_t lds = sizeof(double2) * pl.S * pl.n;
+  allow_lds((const void *)k_poisson_fwd, lds);
+  hipLaunchKernelGGL(k_poisson_fwd, dim3(fft_strips(g, pl)), dim3(kFftThreads), lds, s, g, pl,
+                     gamma, dtp, ux, uy, tw, psihat);
+}
+
+void launch_poisson_inv(hipStream_t s, const Geo &g, const FftPlan &pl, const double2 *tw,
+                        const double2 *psihat, double *psi) {
+  const size_t lds = sizeof(double2) * pl.S * pl.n;
+  allow_lds((const void *)k_poisson_inv, lds);
+  hipLaunchKernelGGL(k_poisson_inv, dim3(fft_strips(g, pl)), dim3(kFftThreads), lds, s, g, pl,
+                     tw, psihat, psi);
+}
+
+// Per-wavenumber tri-diagonal solve along x (src/fluid/compute_potential.c:109-136):
+// l_i = 1/DXC(i)/DXF(i), u_i = 1/DXC(i+1)/DXF(i), c_i = -l_i-u_i+lambda_J,
+// Neumann rows c_1 += l_1, c_itot += u_itot.  The J = 0 system is singular
+// (the potential is defined up to a constant); its last equation is replaced
+// by psi_hat(itot) = 0, which fixes that constant.
+template <int NT, int M>
+__global__ void __launch_bounds__(NT)
+    k_poisson_x(Geo g, const double *__restrict__ eigen, double2 *__restrict__ psihat) {
+  __shared__ TridiagLds<NT, 2> lds;
+  const int J = blockIdx.x;
+  const int n = g.itot;
+  const double lam = eigen[J];
+  double2 *row = psihat + (size_t)J * n;
+  double a[M], b[M], c[M], d[2][M];
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const int r = threadIdx.x * M + k;
+    if (r < n) {
+      const int i = r + 1;
+      const double l = 1. / DXC(g, i) / DXF(g, i);
+      const double u = 1. / DXC(g, i + 1) / DXF(g, i);
+      double cc = -l - u + lam;
+      if (r == 0) cc += l;
+      if (r == n - 1) cc += u;
+      a[k] = (r == 0) ? 0. : l;
+      b[k] = cc;
+      c[k] = (r == n - 1) ? 0. : u;
+      const double2 v = row[r];
+      d[0][k] = v.x;
+      d[1][k] = v.y;
+      if (J == 0 && r == n - 1) { a[k] = 0.; b[k] = 1.; d[0][k] = 0.; d[1][k] = 0.; }
+    } else {
+      a[k] = 0.; b[k] = 1.; c[k] = 0.; d[0][k] = 0.; d[1][k] = 0.;
+    }
+  }
+  block_tridiag<NT, M, 2>(a, b, c, d, lds);
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const int r = threadIdx.x * M + k;
+    if (r < n) row[r] = make_double2(d[0][k], d[1][k]);
+  }
+}
+
+void launch_poisson_x(hipStream_t s, const Geo &g, const double *eigen, double2 *psihat) {
+  const dim3 gr(g.jtot / 2 + 1);
+  const int n = g.itot;
+  if (n <= 64 * 4) hipLaunchKernelGGL((k_poisson_x<64, 4>), gr, dim3(64), 0, s, g, eigen, psihat);
+  else if (n <= 256 * 4) hipLaunchKernelGGL((k_poisson_x<256, 4>), gr, dim3(256), 0, s, g, eigen, psihat);
+  else if (n <= 256 * 8) hipLaunchKernelGGL((k_poisson_x<256, 8>), gr, dim3(256), 0, s, g, eigen, psihat);
+  else if (n <= 256 * 16) hipLaunchKernelGGL((k_poisson_x<256, 16>), gr, dim3(256), 0, s, g, eigen, psihat);
+  else hipLaunchKernelGGL((k_poisson_x<256, 32>), gr, dim3(256), 0, s, g, eigen, psihat);
+}
+
+}  // namespace sns
