@@ -114,6 +114,23 @@ int logging_compute(const param_t *param, const double *ux, const double *uy, co
   return 0;
 }
 
+int sns_logging_compute_raw(const int itot, const int jtot, const double lx, const double ly,
+                            const double Ra, const double Pr, const double *dxf, const double *dxc,
+                            const double dy, const double *ux, const double *uy, const double *temp,
+                            double out[11]){
+  param_t param;
+  param.itot = itot;
+  param.jtot = jtot;
+  param.lx = lx;
+  param.ly = ly;
+  param.Ra = Ra;
+  param.Pr = Pr;
+  param.dxf = (double *)dxf;
+  param.dxc = (double *)dxc;
+  param.dy = dy;
+  return logging_compute(&param, ux, uy, temp, out);
+}
+
 static FILE *open_log(const param_t *param, const char *name){
   return fileio_fopen(name, param->step == 0 ? "w" : "a");
 }
