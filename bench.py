@@ -35,19 +35,16 @@ B_ALG_STEP = 600.0  # bytes per cell-step, explicit-y treatments
 # Algorithmic (compulsory) HBM bytes per interior cell for one launch of each
 # kernel of the step: each array the kernel must read or write, once.
 KERNEL_BYTES_PER_CELL = {
-    "force": 8 * 2,            # T -> tempforcex
-    "src_temp": 8 * 5,         # ux, uy, T -> A, G
-    "src_vel": 8 * 8,          # ux, uy, p, tf -> A, G for ux and uy
-    "cnx": 8 * 5,              # A, B, G, field -> field (one field)
-    "incr": 8 * 5,
-    "poisson_fwd": 8 * 3,      # ux, uy -> spectrum (jtot/2+1 complex per column)
+    # fused predictor: reads ux, uy, p, T and (substages 1, 2) the three RK
+    # histories, writes T', ux*, uy* and (substages 0, 1) the histories:
+    # 10, 13, 10 words over the three substages, 11 on average
+    "pred": 8 * 11,
+    "poisson_fwd": 8 * 3,      # ux*, uy* -> spectrum (jtot/2+1 complex per column)
     "poisson_x": 8 * 2,        # spectrum in place
     "poisson_inv": 8 * 2,      # spectrum -> psi
-    "correct": 8 * 5,          # ux, uy, psi -> ux, uy
-    "pressure": 8 * 3,         # p, psi -> p
+    "corr": 8 * 7,             # ux*, uy*, psi, p -> ux, uy, p
     "dt_reduce": 8 * 2,        # ux, uy
-    "pred": 8 * 10,            # fused predictor: ux, uy, p, T, A(3) -> T', ux*, uy*, A(3)
-    "corr": 8 * 7,             # fused correction: ux*, uy*, psi, p -> ux, uy, p
+    "dt": 0,
 }
 KERNEL_NAMES = list(KERNEL_BYTES_PER_CELL.keys())
 
@@ -83,13 +80,10 @@ def make_instance(sns, itot, jtot):
     return inst
 
 
-def step(sns, inst):
-    w = WORKLOAD
-    raw = inst.state_dt_adv_raw()
-    dt = sns.decide_dt(inst.g, w["Ra"], w["Pr"], raw, w["implicitx"], w["implicity"])
-    inst.set_dt(dt)
-    inst.integrate()
-    return dt
+def step(sns, inst, n=1):
+    """n iterations of the reference's time loop body: device-side time-step
+    decision (src/param/decide_dt.c for the fixed treatment) + RK3 step."""
+    inst.advance(n)
 
 
 def per_kernel_times(sns, inst, steps):
@@ -97,8 +91,7 @@ def per_kernel_times(sns, inst, steps):
     out = {}
     for name in KERNEL_NAMES:
         inst.timing_enable(name)
-        for _ in range(steps):
-            step(sns, inst)
+        step(sns, inst, steps)
         n, ms = inst.timing_read()
         inst.timing_disable()
         if n:
@@ -135,8 +128,7 @@ def main():
     itot, jtot = a.itot, a.jtot
     cells = itot * jtot
     inst = make_instance(sns, itot, jtot)
-    for _ in range(a.warmup):
-        step(sns, inst)
+    step(sns, inst, a.warmup)
     inst.sync()
     # timed region, with the dominant-kernel events recorded live on the stream
     kt = per_kernel_times(sns, inst, 1)
@@ -145,8 +137,7 @@ def main():
     if dominant:
         inst.timing_enable(dominant)
     t0 = time.perf_counter()
-    for _ in range(a.steps):
-        step(sns, inst)
+    step(sns, inst, a.steps)
     inst.sync()
     el = time.perf_counter() - t0
     nlaunch, kms = inst.timing_read() if dominant else (0, 0.0)
@@ -174,7 +165,7 @@ def main():
         kt = per_kernel_times(sns, inst, 2)
         tot = sum(v[1] for v in kt.values())
         for k, (n, ms) in sorted(kt.items(), key=lambda kv: -kv[1][1]):
-            gbs = KERNEL_BYTES_PER_CELL[k] * cells / (ms / n * 1e-3) / 1e9
+            gbs = KERNEL_BYTES_PER_CELL[k] * cells / max(ms / n * 1e-3, 1e-12) / 1e9
             print("%-12s launches %4d  avg %8.3f ms  share %5.1f%%  %7.0f GB/s" %
                   (k, n, ms / n, 100 * ms / tot, gbs), file=sys.stderr)
     inst.close()

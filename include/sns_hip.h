@@ -26,6 +26,7 @@
  *   snsg_param_decide_dt                     include/param.h:59 (device reduction part)
  *   snsg_transpose                           include/parallel.h:21 (one rank)
  *   snsg_integrate                           src/main.c:79-101 (static integrate())
+ *   snsg_advance                             src/main.c:28-35 (time-loop body)
  */
 #ifndef SNS_HIP_H
 #define SNS_HIP_H
@@ -113,13 +114,29 @@ int snsg_param_decide_dt(snsg_ctx *ctx, const double *ux, const double *uy, doub
 int snsg_transpose(snsg_ctx *ctx, int g_isize, int g_jsize, size_t elemsize, const void *send,
                    void *recv);
 
-/* ---- one RK3 step with fused kernels, using the dt set by snsg_set_dt */
+/* ---- fused time integration on instance-owned state.
+ * snsg_integrate: one RK3 step (src/main.c:79-101) with the dt set by snsg_set_dt.
+ * snsg_advance: nsteps iterations of the reference's time loop body
+ * (src/main.c:28-35): the time-step decision of src/param/decide_dt.c for
+ * the configured treatment, evaluated on the device, then one RK3 step; the
+ * run time accumulates on the device (snsg_get_time / snsg_set_time).  No
+ * host synchronisation happens inside. */
 int snsg_state_alloc(snsg_ctx *ctx, snsg_state *st);
 int snsg_state_free(snsg_ctx *ctx, snsg_state *st);
 int snsg_integrate(snsg_ctx *ctx, snsg_state *st);
+int snsg_advance(snsg_ctx *ctx, snsg_state *st, int nsteps);
+int snsg_get_time(snsg_ctx *ctx, double *time, double *dt);
+int snsg_set_time(snsg_ctx *ctx, double time);
+
+/* ---- self-check of the fused kernels' division scheme (no counterpart in the
+ * reference): q_cr[k] = a[k]/b[k] formed from r[k] = 1/b[k] by the FMA
+ * correction sequence, q_ieee[k] = a[k]/b[k] by the IEEE division; device
+ * arrays of n entries */
+int snsg_check_division(snsg_ctx *ctx, int n, const double *a, const double *b, const double *r,
+                        double *q_cr, double *q_ieee);
 
 /* ---- per-kernel timing with HIP events on the instance's stream.  While
- * enabled, every launch of a kernel whose name contains `filter` (NULL: all)
+ * enabled, every launch of the kernel group named `filter` (NULL: all)
  * is bracketed by two events; snsg_timing_read returns the number of such
  * launches and their summed duration in milliseconds. */
 int snsg_timing_enable(snsg_ctx *ctx, const char *filter);

@@ -80,6 +80,10 @@ def _load_hip():
         L.snsg_state_alloc.argtypes = [_vp, C.POINTER(SnsgState)]
         L.snsg_state_free.argtypes = [_vp, C.POINTER(SnsgState)]
         L.snsg_integrate.argtypes = [_vp, C.POINTER(SnsgState)]
+        L.snsg_advance.argtypes = [_vp, C.POINTER(SnsgState), C.c_int]
+        L.snsg_get_time.argtypes = [_vp, _dp, _dp]
+        L.snsg_set_time.argtypes = [_vp, C.c_double]
+        L.snsg_check_division.argtypes = [_vp, C.c_int] + [_vp] * 5
         L.snsg_timing_enable.argtypes = [_vp, C.c_char_p]
         L.snsg_timing_disable.argtypes = [_vp]
         L.snsg_timing_read.argtypes = [_vp, C.POINTER(C.c_int), _dp]
@@ -339,6 +343,18 @@ class Instance:
 
     def integrate(self):
         _check(self.L.snsg_integrate(self.ctx, C.byref(self.state)), "snsg_integrate")
+
+    def advance(self, nsteps):
+        """nsteps time-loop iterations with the time step decided on the device."""
+        _check(self.L.snsg_advance(self.ctx, C.byref(self.state), int(nsteps)), "snsg_advance")
+
+    def get_time(self):
+        t, dt = C.c_double(0.0), C.c_double(0.0)
+        _check(self.L.snsg_get_time(self.ctx, C.byref(t), C.byref(dt)), "snsg_get_time")
+        return t.value, dt.value
+
+    def set_time(self, t):
+        _check(self.L.snsg_set_time(self.ctx, float(t)), "snsg_set_time")
 
     def state_dt_adv_raw(self):
         v = C.c_double(0.0)

@@ -8,6 +8,8 @@
 
 #include "sns_hip.h"
 #include "sns_launch.h"
+#include "sns_fused.h"
+#include "sns_poisson.h"
 
 using namespace sns;
 
@@ -29,11 +31,15 @@ struct snsg_ctx {
   double *d_q = nullptr, *d_qy = nullptr;
   unsigned long long *d_red = nullptr;
   unsigned long long *h_red = nullptr;  // pinned
-  FftPlan plan{};
-  // fused-step scratch (RK history B/G kept on device for the composed path)
-  double *d_srcb[3] = {nullptr, nullptr, nullptr};
-  double *d_srcg[3] = {nullptr, nullptr, nullptr};
-  double *d_tf = nullptr;
+  Fft2Plan plan{};
+  double2 *d_Y = nullptr;  // four-step FFT intermediate, jtot rows of (itot+1)/2 pairs
+  // per-column tables of the fused kernels (one allocation)
+  double *d_tab = nullptr;
+  Tables tab{};
+  // fused-step scratch: x-aligned increments of T, ux, uy when a y solve follows
+  double *d_qf[3] = {nullptr, nullptr, nullptr};
+  double *d_time = nullptr;
+  double dt_dif_x = 0., dt_dif_y = 0.;
   // timing
   bool timing = false;
   std::string filter;
@@ -67,7 +73,7 @@ static int check_launch(const char *what) {
 // and the name matches, brackets it with two events.
 template <typename F>
 static int run(snsg_ctx *ctx, const char *name, F fn) {
-  const bool t = ctx->timing && (ctx->filter.empty() || strstr(name, ctx->filter.c_str()));
+  const bool t = ctx->timing && (ctx->filter.empty() || ctx->filter == name);
   if (t) {
     while (ctx->ev_pool.size() < ctx->ev_used + 2) {
       hipEvent_t e;
@@ -87,33 +93,6 @@ static int run(snsg_ctx *ctx, const char *name, F fn) {
 }
 
 // ------------------------------------------------------------- creation
-
-static bool fft_plan(int n, int itot, FftPlan &pl) {
-  if (n < 2 || n > 8192) return false;
-  int rest = n, np = 0;
-  unsigned long long code = 0;
-  auto push = [&](int r) {
-    code |= (unsigned long long)r << (4 * np);
-    np++;
-    rest /= r;
-  };
-  while (rest % 8 == 0 && np < 16) push(8);
-  while (rest % 4 == 0 && np < 16) push(4);
-  while (rest % 2 == 0 && np < 16) push(2);
-  while (rest % 3 == 0 && np < 16) push(3);
-  while (rest % 5 == 0 && np < 16) push(5);
-  if (rest != 1) return false;
-  pl.n = n;
-  pl.npass = np;
-  pl.code = code;
-  const int maxS = 8192 / n;
-  const int half = (itot + 1) / 2;
-  int S = (half + 255) / 256;
-  if (S < 1) S = 1;
-  if (S > maxS) S = maxS;
-  pl.S = S;
-  return true;
-}
 
 static int ctx_alloc(void **p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes ? bytes : 8);
@@ -136,10 +115,11 @@ extern "C" int snsg_create(int device, const snsg_config *cfg, const double *dxf
   const int itot = cfg->itot, jtot = cfg->jtot;
   if (itot < 4 || jtot < 4) return set_err(SNSG_E_SIZE, "snsg_create: itot and jtot must be >= 4");
   if (itot > 8192) return set_err(SNSG_E_SIZE, "snsg_create: itot > 8192 is not supported");
-  FftPlan pl;
-  if (!fft_plan(jtot, itot, pl))
+  Fft2Plan pl;
+  if (jtot > 16384 || !fft2_plan(jtot, itot, pl))
     return set_err(SNSG_E_SIZE,
-                   "snsg_create: jtot must be <= 8192 with prime factors 2, 3 and 5 only");
+                   "snsg_create: jtot must be <= 16384, not prime, with prime factors 2, 3 "
+                   "and 5 only");
   HIPCHECK(hipSetDevice(device));
   snsg_ctx *c = new snsg_ctx();
   c->device = device;
@@ -159,6 +139,7 @@ extern "C" int snsg_create(int device, const snsg_config *cfg, const double *dxf
   if ((rc = ctx_alloc((void **)&c->d_tw, sizeof(double2) * jtot))) return fail(rc);
   if ((rc = ctx_alloc((void **)&c->d_psihat, sizeof(double2) * (size_t)itot * (jtot / 2 + 1))))
     return fail(rc);
+  if ((rc = ctx_alloc((void **)&c->d_Y, sizeof(double2) * (size_t)pl.npair * jtot))) return fail(rc);
   if ((rc = ctx_alloc((void **)&c->d_q, sizeof(double) * (size_t)(itot + 1) * jtot))) return fail(rc);
   if ((rc = ctx_alloc((void **)&c->d_qy, sizeof(double) * (size_t)(itot + 1) * jtot))) return fail(rc);
   if ((rc = ctx_alloc((void **)&c->d_red, sizeof(unsigned long long)))) return fail(rc);
@@ -194,6 +175,56 @@ extern "C" int snsg_create(int device, const snsg_config *cfg, const double *dxf
     const double a = 2. * M_PI * (double)m / (double)jtot;
     tw[m] = make_double2(cos(a), -sin(a));
   }
+  // per-column tables (see sns_fast.h), each evaluated as in the reference
+  {
+    const int nt = 14;
+    const size_t stride = (size_t)itot + 2;
+    if ((rc = ctx_alloc((void **)&c->d_tab, sizeof(double) * nt * stride))) return fail(rc);
+    std::vector<double> h(nt * stride, 0.);
+    auto DXF = [&](int i) { return dxf[i - 1]; };
+    auto DXC = [&](int i) { return dxc[i - 1]; };
+    double *t[nt];
+    for (int k = 0; k < nt; k++) t[k] = h.data() + k * stride;
+    for (int i = 1; i <= itot; i++) t[0][i - 1] = DXF(i);
+    for (int i = 1; i <= itot + 1; i++) t[1][i - 1] = DXC(i);
+    for (int i = 1; i <= itot; i++) t[2][i - 1] = 1. / DXF(i);
+    for (int i = 1; i <= itot + 1; i++) t[3][i - 1] = 1. / DXC(i);
+    for (int i = 1; i <= itot; i++) {
+      t[4][i - 1] = DXC(i) / (2. * DXF(i));
+      t[5][i - 1] = DXC(i + 1) / (2. * DXF(i));
+      t[8][i - 1] = 1. / DXF(i) / DXC(i);
+      t[9][i - 1] = 1. / DXF(i) / DXC(i + 1);
+      t[12][i - 1] = 1. / DXC(i) / DXF(i);
+      t[13][i - 1] = 1. / DXC(i + 1) / DXF(i);
+    }
+    for (int i = 2; i <= itot; i++) {
+      t[6][i - 1] = DXF(i - 1) / (2. * DXC(i));
+      t[7][i - 1] = DXF(i) / (2. * DXC(i));
+      t[10][i - 1] = 1. / DXC(i) / DXF(i - 1);
+      t[11][i - 1] = 1. / DXC(i) / DXF(i);
+    }
+    if (hipMemcpy(c->d_tab, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice) !=
+        hipSuccess)
+      return fail(set_err(SNSG_E_ARG, "upload column tables"));
+    const double *d = c->d_tab;
+    Tables &tb = c->tab;
+    tb.dxf = d + 0 * stride; tb.dxc = d + 1 * stride;
+    tb.rdxf = d + 2 * stride; tb.rdxc = d + 3 * stride;
+    tb.cT_xm = d + 4 * stride; tb.cT_xp = d + 5 * stride;
+    tb.cU_xm = d + 6 * stride; tb.cU_xp = d + 7 * stride;
+    tb.nT_xm = d + 8 * stride; tb.nT_xp = d + 9 * stride;
+    tb.nU_xm = d + 10 * stride; tb.nU_xp = d + 11 * stride;
+    tb.pl = d + 12 * stride; tb.pu = d + 13 * stride;
+    tb.dy = dy;
+    tb.rdy = 1. / dy;
+    // fixed diffusive time-step bounds (src/param/decide_dt.c:48-66)
+    const double diffusivity = fmax(sqrtPr / sqrtRa, 1. / sqrtPr / sqrtRa);
+    double dx = 1.;
+    for (int i = 2; i <= itot; i++) dx = fmin(dx, DXC(i));
+    c->dt_dif_x = 0.25 / diffusivity * pow(dx, 2.) * 0.9;
+    c->dt_dif_y = 0.25 / diffusivity * pow(dy, 2.) * 0.9;
+  }
+  if ((rc = ctx_alloc((void **)&c->d_time, sizeof(double)))) return fail(rc);
   if (hipMemcpy(c->d_eigen, eig.data(), sizeof(double) * eig.size(), hipMemcpyHostToDevice) !=
           hipSuccess ||
       hipMemcpy(c->d_tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice) !=
@@ -207,14 +238,11 @@ extern "C" int snsg_destroy(snsg_ctx *c) {
   if (!c) return 0;
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
-  void *bufs[] = {c->d_dxf, c->d_dxc, c->d_dt, c->d_eigen, c->d_tw, c->d_psihat, c->d_q,
-                  c->d_qy, c->d_red, c->d_tf};
+  void *bufs[] = {c->d_dxf, c->d_dxc, c->d_dt, c->d_eigen, c->d_tw, c->d_psihat, c->d_Y, c->d_q,
+                  c->d_qy, c->d_red, c->d_tab, c->d_time, c->d_qf[0], c->d_qf[1],
+                  c->d_qf[2]};
   for (void *b : bufs)
-    if (b) hipFree(b);
-  for (int k = 0; k < 3; k++) {
-    if (c->d_srcb[k]) hipFree(c->d_srcb[k]);
-    if (c->d_srcg[k]) hipFree(c->d_srcg[k]);
-  }
+    if (b) (void)hipFree(b);
   if (c->h_red) hipHostFree(c->h_red);
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -309,6 +337,25 @@ static RK rk_of(const snsg_ctx *c, int rkstep) {
 
 // Apply the RK increment of one field with the configured diffusive
 // treatment: src/temperature/update.c:89-231, src/fluid/update_velocity.c:194-488
+// src/fluid/compute_potential.c:15-168 (three kernel groups: forward
+// transform, x solves, inverse transform)
+static int potential(snsg_ctx *c, double gamma, const double *ux, const double *uy, double *psi) {
+  int rc = run(c, "poisson_fwd", [&] {
+    launch_poisson2(c->stream, c->geo, c->tab, c->plan, gamma, c->d_dt, ux, uy, c->d_tw,
+                    c->d_eigen, c->d_Y, c->d_psihat, psi, 0);
+  });
+  if (rc) return rc;
+  if ((rc = run(c, "poisson_x", [&] {
+         launch_poisson2(c->stream, c->geo, c->tab, c->plan, gamma, c->d_dt, ux, uy, c->d_tw,
+                         c->d_eigen, c->d_Y, c->d_psihat, psi, 1);
+       })))
+    return rc;
+  return run(c, "poisson_inv", [&] {
+    launch_poisson2(c->stream, c->geo, c->tab, c->plan, gamma, c->d_dt, ux, uy, c->d_tw,
+                    c->d_eigen, c->d_Y, c->d_psihat, psi, 2);
+  });
+}
+
 static int update_field(snsg_ctx *c, int kind, int rkstep, const double *srca, const double *srcb,
                         const double *srcg, double *field) {
   const Geo &g = c->geo;
@@ -383,18 +430,7 @@ extern "C" int snsg_fluid_compute_potential(snsg_ctx *c, int rkstep, const doubl
                                             const double *uy, double *psi) {
   CHECK_CTX(c, psi);
   if (!ux || !uy || rkstep < 0 || rkstep > 2) return set_err(SNSG_E_ARG, __func__);
-  const double gamma = c->cfg.gamma[rkstep];
-  int rc = run(c, "poisson_fwd", [&] {
-    launch_poisson_fwd(c->stream, c->geo, c->plan, gamma, c->d_dt, ux, uy, c->d_tw, c->d_psihat);
-  });
-  if (rc) return rc;
-  if ((rc = run(c, "poisson_x", [&] {
-         launch_poisson_x(c->stream, c->geo, c->d_eigen, c->d_psihat);
-       })))
-    return rc;
-  return run(c, "poisson_inv", [&] {
-    launch_poisson_inv(c->stream, c->geo, c->plan, c->d_tw, c->d_psihat, psi);
-  });
+  return potential(c, c->cfg.gamma[rkstep], ux, uy, psi);
 }
 
 extern "C" int snsg_fluid_correct_velocity(snsg_ctx *c, int rkstep, const double *psi, double *ux,
@@ -484,39 +520,117 @@ extern "C" int snsg_state_free(snsg_ctx *c, snsg_state *st) {
   return 0;
 }
 
-// One RK3 step (src/main.c:79-101), composed from the per-function kernels
-// with the source histories kept on the device.
-extern "C" int snsg_integrate(snsg_ctx *c, snsg_state *st) {
-  if (!c || !st || !st->ux) return set_err(SNSG_E_ARG, __func__);
-  const int itot = c->geo.itot, jtot = c->geo.jtot;
+// One RK3 step (src/main.c:79-101) with the fused kernels; dt is read from
+// the device slot (snsg_set_dt or the device time-step kernel).  With
+// `reduce`, the last correction also folds the advective bound for the next
+// step into d_red.
+static int integrate_fused(snsg_ctx *c, snsg_state *st, bool reduce) {
+  const Geo &g = c->geo;
+  const int itot = g.itot, jtot = g.jtot;
   int rc;
-  if (!c->d_tf) {
-    if ((rc = ctx_alloc((void **)&c->d_tf, sizeof(double) * (size_t)(itot - 1) * jtot))) return rc;
+  const bool impx = c->treat.implicitx, impy = c->treat.implicity;
+  if (impx && !pred_impx_supported(itot))
+    return set_err(SNSG_E_SIZE, "fused step: implicit x needs itot <= 4096");
+  if (impy && !c->d_qf[0]) {
     const size_t nb[3] = {(size_t)itot * jtot, (size_t)(itot - 1) * jtot, (size_t)itot * jtot};
-    for (int k = 0; k < 3; k++) {
-      if ((rc = ctx_alloc((void **)&c->d_srcb[k], sizeof(double) * nb[k]))) return rc;
-      if ((rc = ctx_alloc((void **)&c->d_srcg[k], sizeof(double) * nb[k]))) return rc;
-    }
+    for (int k = 0; k < 3; k++)
+      if ((rc = ctx_alloc((void **)&c->d_qf[k], sizeof(double) * nb[k]))) return rc;
   }
+  const bool with_temp = c->cfg.with_temperature != 0;
   for (int rk = 0; rk < 3; rk++) {
-    if (c->cfg.with_temperature) {
-      if ((rc = snsg_temperature_compute_force(c, st->temp, c->d_tf))) return rc;
-      std::swap(st->srctempa, c->d_srcb[0]);
-      if ((rc = snsg_temperature_update_temp(c, rk, st->ux, st->uy, st->temp, st->srctempa,
-                                             c->d_srcb[0], c->d_srcg[0])))
-        return rc;
+    PredArgs pa;
+    pa.g = g;
+    pa.tb = c->tab;
+    pa.tr = c->treat;
+    pa.ux = st->ux; pa.uy = st->uy; pa.p = st->p; pa.temp = st->temp;
+    pa.hist_ux = st->srcuxa; pa.hist_uy = st->srcuya; pa.hist_t = st->srctempa;
+    pa.out_u = st->ux_star; pa.out_v = st->uy_star; pa.out_t = st->temp_alt;
+    pa.q_u = impy ? c->d_qf[1] : nullptr;
+    pa.q_v = impy ? c->d_qf[2] : nullptr;
+    pa.q_t = impy ? c->d_qf[0] : nullptr;
+    pa.dtp = c->d_dt;
+    pa.alpha = c->cfg.alpha[rk]; pa.beta = c->cfg.beta[rk]; pa.gamma = c->cfg.gamma[rk];
+    pa.read_hist = c->cfg.beta[rk] != 0.;
+    pa.write_hist = rk < 2;
+    pa.with_temp = with_temp;
+    pa.forcing = with_temp && c->cfg.with_thermal_forcing;
+    if ((rc = run(c, "pred", [&] { launch_pred(c->stream, pa, impx); }))) return rc;
+    if (impy) {
+      const RK rkc = rk_of(c, rk);
+      const int kinds[3] = {KIND_TEMP, KIND_UX, KIND_UY};
+      const double *ins[3] = {st->temp, st->ux, st->uy};
+      double *outs[3] = {st->temp_alt, st->ux_star, st->uy_star};
+      for (int f = 0; f < 3; f++) {
+        if (f == 0 && !with_temp) continue;
+        const int n = kinds[f] == KIND_UX ? itot - 1 : itot;
+        rc = run(c, "ysolve", [&] {
+          launch_transpose(c->stream, n, jtot, 8, c->d_qf[f], c->d_qy);
+          launch_cny_rows(c->stream, g, kinds[f], rkc, c->d_dt, n, c->d_qy);
+          launch_transpose(c->stream, jtot, n, 8, c->d_qy, c->d_qf[f]);
+          launch_add_out(c->stream, g, kinds[f], ins[f], c->d_qf[f], outs[f]);
+        });
+        if (rc) return rc;
+      }
     }
-    std::swap(st->srcuxa, c->d_srcb[1]);
-    std::swap(st->srcuya, c->d_srcb[2]);
-    if ((rc = snsg_fluid_update_velocity(c, rk, st->ux, st->uy, st->p, c->d_tf, st->srcuxa,
-                                         c->d_srcb[1], c->d_srcg[1], st->srcuya, c->d_srcb[2],
-                                         c->d_srcg[2])))
-      return rc;
-    if ((rc = snsg_fluid_compute_potential(c, rk, st->ux, st->uy, st->psi))) return rc;
-    if ((rc = snsg_fluid_correct_velocity(c, rk, st->psi, st->ux, st->uy))) return rc;
-    if ((rc = snsg_fluid_update_pressure(c, rk, st->psi, st->p))) return rc;
+    const double gamma = c->cfg.gamma[rk];
+    if ((rc = potential(c, gamma, st->ux_star, st->uy_star, st->psi))) return rc;
+    CorrArgs ca;
+    ca.g = g;
+    ca.tb = c->tab;
+    ca.tr = c->treat;
+    ca.ux_star = st->ux_star; ca.uy_star = st->uy_star; ca.psi = st->psi;
+    ca.ux = st->ux; ca.uy = st->uy; ca.p = st->p;
+    ca.dtp = c->d_dt;
+    ca.gamma = gamma;
+    ca.red = (reduce && rk == 2) ? c->d_red : nullptr;
+    if ((rc = run(c, "corr", [&] { launch_corr(c->stream, ca); }))) return rc;
+    if (with_temp) std::swap(st->temp, st->temp_alt);
   }
   return 0;
+}
+
+extern "C" int snsg_integrate(snsg_ctx *c, snsg_state *st) {
+  if (!c || !st || !st->ux) return set_err(SNSG_E_ARG, __func__);
+  return integrate_fused(c, st, false);
+}
+
+extern "C" int snsg_advance(snsg_ctx *c, snsg_state *st, int nsteps) {
+  if (!c || !st || !st->ux || nsteps < 0) return set_err(SNSG_E_ARG, __func__);
+  int rc = run(c, "dt_reduce", [&] {
+    launch_fill_u64(c->stream, c->d_red, 0x7ff0000000000000ull, 1);
+    launch_dt_reduce(c->stream, c->geo, st->ux, st->uy, c->d_red);
+  });
+  if (rc) return rc;
+  for (int n = 0; n < nsteps; n++) {
+    rc = run(c, "dt", [&] {
+      launch_dt(c->stream, c->d_red, c->d_dt, c->d_time, c->dt_dif_x, c->dt_dif_y,
+                c->treat.implicitx, c->treat.implicity);
+    });
+    if (rc) return rc;
+    if ((rc = integrate_fused(c, st, true))) return rc;
+  }
+  return 0;
+}
+
+extern "C" int snsg_get_time(snsg_ctx *c, double *time, double *dt) {
+  if (!c) return set_err(SNSG_E_ARG, __func__);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  if (time) HIPCHECK(hipMemcpy(time, c->d_time, sizeof(double), hipMemcpyDeviceToHost));
+  if (dt) HIPCHECK(hipMemcpy(dt, c->d_dt, sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int snsg_check_division(snsg_ctx *c, int n, const double *a, const double *b,
+                                   const double *r, double *q_cr, double *q_ieee) {
+  if (!c || n < 0 || !a || !b || !r || !q_cr || !q_ieee) return set_err(SNSG_E_ARG, __func__);
+  if (n == 0) return 0;
+  return run(c, "check_div", [&] { launch_check_div(c->stream, n, a, b, r, q_cr, q_ieee); });
+}
+
+extern "C" int snsg_set_time(snsg_ctx *c, double time) {
+  if (!c) return set_err(SNSG_E_ARG, __func__);
+  hipLaunchKernelGGL(k_set_f64, dim3(1), dim3(1), 0, c->stream, c->d_time, time);
+  return check_launch("snsg_set_time");
 }
 
 // --------------------------------------------------------------- timing

@@ -1,0 +1,44 @@
+// sns_fused.h -- arguments and launchers of the fused substage kernels.
+#pragma once
+#include "sns_launch.h"
+#include "sns_fast.h"
+
+namespace sns {
+
+struct PredArgs {
+  Geo g;
+  Tables tb;
+  Treat tr;
+  const double *ux, *uy, *p, *temp;       // substage-start fields
+  double *hist_ux, *hist_uy, *hist_t;     // RK history of the explicit terms (in place)
+  double *out_u, *out_v, *out_t;          // ux*, uy*, T' with boundary values
+  double *q_u, *q_v, *q_t;                // x-aligned increments (set when a y solve follows)
+  const double *dtp;
+  double alpha, beta, gamma;
+  int read_hist, write_hist;
+  int with_temp;                          // temperature equation integrated
+  int forcing;                            // buoyancy term from T
+};
+
+struct CorrArgs {
+  Geo g;
+  Tables tb;
+  Treat tr;
+  const double *ux_star, *uy_star, *psi;
+  double *ux, *uy, *p;
+  const double *dtp;
+  double gamma;
+  unsigned long long *red;  // advective-bound reduction slot, or nullptr
+};
+
+bool pred_impx_supported(int itot);
+void launch_pred(hipStream_t s, const PredArgs &a, int implicitx);
+void launch_add_out(hipStream_t s, const Geo &g, int kind, const double *in, const double *q,
+                    double *out);
+void launch_corr(hipStream_t s, const CorrArgs &a);
+void launch_dt(hipStream_t s, unsigned long long *red, double *dt, double *time, double dt_dif_x,
+               double dt_dif_y, int implicitx, int implicity);
+void launch_check_div(hipStream_t s, int n, const double *a, const double *b, const double *r,
+                      double *q_cr, double *q_ieee);
+
+}  // namespace sns

@@ -1,0 +1,28 @@
+// sns_poisson.h -- four-step FFT plan and launcher of the Poisson solve.
+#pragma once
+#include "sns_launch.h"
+#include "sns_fast.h"
+
+namespace sns {
+
+struct Fft2Plan {
+  int n;                         // transform length (jtot) = n1*n2
+  int n1, n2;
+  int np1, np2;                  // radix passes of the two factors
+  unsigned long long code1, code2;
+  int npair;                     // complex column pairs, (itot+1)/2
+  int G;                         // rows j2 per workgroup in passes A and A'
+};
+
+bool fft2_plan(int n, int itot, Fft2Plan &pl);
+
+// stage 0: RHS + forward transform -> psihat (x-aligned spectrum)
+// stage 1: tri-diagonal solves along x, one per wavenumber
+// stage 2: inverse transform -> psi with boundary values
+// stage < 0: all three
+void launch_poisson2(hipStream_t s, const Geo &g, const Tables &tb, const Fft2Plan &pl,
+                     double gamma, const double *dtp, const double *ux, const double *uy,
+                     const double2 *tw, const double *eigen, double2 *Y, double2 *psihat,
+                     double *psi, int stage);
+
+}  // namespace sns

@@ -1,0 +1,358 @@
+// sns_poisson.hip -- the pressure-potential Poisson solve (gfx950).
+//
+// Reference: src/fluid/compute_potential.c:15-168: RHS = div(u*)/(gamma dt),
+// real FFT along y (FFTW r2c, unnormalised), one tri-diagonal system along x
+// per wavenumber, inverse c2r FFT, psi = result / jtot with Neumann walls.
+//
+// The y transforms run as four-step FFTs, n = n1*n2, j = n2*j1 + j2,
+// k = k1 + n1*k2:
+//   forward  pass A : Y[k1*n2+j2] = W_n^(j2 k1) * DFT_n1 over j1 of x[n2 j1 + j2]
+//            pass B : X[k1 + n1 k2] = DFT_n2 over j2 of Y[k1*n2+j2]
+//   inverse  pass B': Y[k1*n2+j2] = W_n^(-j2 k1) * IDFT_n2 over k2 of Z[k1 + n1 k2]
+//            pass A': x[n2 j1 + j2] = IDFT_n1 over k1 of Y[k1*n2+j2]
+// Every pass works on a tile of 16 real columns (8 complex pairs = 128-byte
+// row segments), so all global accesses are whole cache lines; two real
+// columns are packed as one complex sequence, and the spectra of the pair are
+// separated (forward) or combined (inverse) in pass B / B', which therefore
+// handles the wavenumber rows k1 and n1-k1 together.
+#include "sns_launch.h"
+#include "sns_fast.h"
+#include "sns_fft.h"
+#include "sns_poisson.h"
+#include "sns_tridiag.h"
+
+namespace sns {
+
+static constexpr int TILE = 8;   // complex pairs per tile (16 real columns)
+static constexpr int NTF = 256;  // threads of the FFT passes
+static constexpr int EMAXF = 8;  // complex values per thread and pass
+
+__device__ __forceinline__ double poisson_rhs2(const Geo &g, const Tables &tb, const double *ux,
+                                               const double *uy, double inv_gdt, int i, int j) {
+  // 1./(gamma*dt)*((ux_xp-ux_xm)/DXF(i) + (uy_yp-uy_ym)/dy)
+  const double ax = div_cr(ux[ux_idx(g, i + 1, j)] - ux[ux_idx(g, i, j)], tb.dxf[i - 1],
+                           tb.rdxf[i - 1]);
+  const double ay = div_cr(uy[c_idx(g, i, j + 1)] - uy[c_idx(g, i, j)], tb.dy, tb.rdy);
+  return inv_gdt * (+ax + ay);
+}
+
+__device__ __forceinline__ void put_psi(const Geo &g, double *f, int i, int j, double v) {
+  // Neumann walls and periodic halo rows (src/fluid/boundary_conditions.c:39-55)
+  f[c_idx(g, i, j)] = v;
+  if (j == 1) f[c_idx(g, i, g.jtot + 1)] = v;
+  if (j == g.jtot) f[c_idx(g, i, 0)] = v;
+  if (i == 1 || i == g.itot) {
+    const int iw = i == 1 ? 0 : g.itot + 1;
+    f[c_idx(g, iw, j)] = v;
+    if (j == 1) f[c_idx(g, iw, g.jtot + 1)] = v;
+    if (j == g.jtot) f[c_idx(g, iw, 0)] = v;
+  }
+}
+
+// ------------------------------------------------------------ forward A
+
+__global__ void __launch_bounds__(NTF)
+    k_pfwd_a(Geo g, Tables tb, Fft2Plan pl, double gamma, const double *__restrict__ dtp,
+             const double *__restrict__ ux, const double *__restrict__ uy,
+             const double2 *__restrict__ tw, double2 *__restrict__ Y) {
+  __shared__ double2 buf[EMAXF * NTF];
+  const int tile = blockIdx.x, j20 = blockIdx.y * pl.G;
+  const int S = pl.G * TILE, n1 = pl.n1, n2 = pl.n2;
+  const double inv_gdt = 1. / (gamma * (*dtp));
+  for (int e = threadIdx.x; e < S * n1; e += NTF) {
+    const int j1 = e / S, s = e - j1 * S;
+    const int gg = s / TILE, q = s - gg * TILE;
+    const int j = n2 * j1 + j20 + gg;  // 0-based row
+    const int ia = 2 * (tile * TILE + q) + 1;
+    double2 z;
+    z.x = ia <= g.itot ? poisson_rhs2(g, tb, ux, uy, inv_gdt, ia, j + 1) : 0.;
+    z.y = ia + 1 <= g.itot ? poisson_rhs2(g, tb, ux, uy, inv_gdt, ia + 1, j + 1) : 0.;
+    buf[e] = z;
+  }
+  __syncthreads();
+  fft_il<NTF, EMAXF, -1>(buf, S, n1, pl.np1, pl.code1, tw, pl.n / n1);
+  for (int e = threadIdx.x; e < S * n1; e += NTF) {
+    const int k1 = e / S, s = e - k1 * S;
+    const int gg = s / TILE, q = s - gg * TILE;
+    const int j2 = j20 + gg;
+    const int pair = tile * TILE + q;
+    if (pair < pl.npair) {
+      double2 z = buf[e];
+      if (k1 && j2) z = cmul(z, tw[j2 * k1]);
+      Y[(size_t)(k1 * n2 + j2) * pl.npair + pair] = z;
+    }
+  }
+}
+
+// ------------------------------------------------------------ forward B
+
+__global__ void __launch_bounds__(NTF)
+    k_pfwd_b(Geo g, Fft2Plan pl, const double2 *__restrict__ tw, const double2 *__restrict__ Y,
+             double2 *__restrict__ psihat) {
+  __shared__ double2 buf[EMAXF * NTF];
+  const int tile = blockIdx.x;
+  const int n1 = pl.n1, n2 = pl.n2, n = pl.n;
+  const int k1a = blockIdx.y, k1b = (n1 - k1a) % n1;
+  const int nh = (k1a == k1b) ? 1 : 2;
+  const int S = nh * TILE;
+  for (int e = threadIdx.x; e < S * n2; e += NTF) {
+    const int j2 = e / S, s = e - j2 * S;
+    const int h = s / TILE, q = s - h * TILE;
+    const int pair = tile * TILE + q;
+    const int k1 = h ? k1b : k1a;
+    buf[e] = pair < pl.npair ? Y[(size_t)(k1 * n2 + j2) * pl.npair + pair] : make_double2(0., 0.);
+  }
+  __syncthreads();
+  fft_il<NTF, EMAXF, -1>(buf, S, n2, pl.np2, pl.code2, tw, n / n2);
+  // separate the two real columns of each pair: X_a = (Z_k + conj Z_{n-k})/2,
+  // X_b = (Z_k - conj Z_{n-k})/(2i), for the wavenumbers k <= n/2 of this group
+  for (int e = threadIdx.x; e < S * n2; e += NTF) {
+    const int k2 = e / S, s = e - k2 * S;
+    const int h = s / TILE, q = s - h * TILE;
+    const int k1 = h ? k1b : k1a;
+    const int k = k1 + n1 * k2;
+    if (2 * k > n) continue;
+    // partner n-k = k1' + n1*k2'
+    int k2p, hp;
+    if (k1 == 0) {
+      k2p = (n2 - k2) % n2;
+      hp = h;
+    } else {
+      k2p = n2 - 1 - k2;
+      hp = nh == 1 ? h : 1 - h;
+    }
+    const double2 zk = buf[e];
+    const double2 zn = buf[k2p * S + hp * TILE + q];
+    const int pair = tile * TILE + q;
+    const int ia = 2 * pair + 1;
+    double2 xa, xb;
+    xa.x = 0.5 * (zk.x + zn.x);
+    xa.y = 0.5 * (zk.y - zn.y);
+    xb.x = 0.5 * (zk.y + zn.y);
+    xb.y = 0.5 * (zn.x - zk.x);
+    if (ia <= g.itot) psihat[(size_t)k * g.itot + (ia - 1)] = xa;
+    if (ia + 1 <= g.itot) psihat[(size_t)k * g.itot + ia] = xb;
+  }
+}
+
+// ------------------------------------------------------------ inverse B'
+
+__global__ void __launch_bounds__(NTF)
+    k_pinv_b(Geo g, Fft2Plan pl, const double2 *__restrict__ tw,
+             const double2 *__restrict__ psihat, double2 *__restrict__ Y) {
+  __shared__ double2 buf[EMAXF * NTF];
+  const int tile = blockIdx.x;
+  const int n1 = pl.n1, n2 = pl.n2, n = pl.n;
+  const int k1a = blockIdx.y, k1b = (n1 - k1a) % n1;
+  const int nh = (k1a == k1b) ? 1 : 2;
+  const int S = nh * TILE;
+  // Hermitian extension of the two real columns' spectra packed as
+  // Z_k = X_a,k + i X_b,k (src/fluid/compute_potential.c:147-149: c2r input)
+  for (int e = threadIdx.x; e < S * n2; e += NTF) {
+    const int k2 = e / S, s = e - k2 * S;
+    const int h = s / TILE, q = s - h * TILE;
+    const int k1 = h ? k1b : k1a;
+    const int k = k1 + n1 * k2;
+    const int pair = tile * TILE + q;
+    const int ia = 2 * pair + 1;
+    const bool upper = 2 * k > n;
+    const int m = upper ? n - k : k;
+    double2 xa = make_double2(0., 0.), xb = make_double2(0., 0.);
+    if (ia <= g.itot) xa = psihat[(size_t)m * g.itot + (ia - 1)];
+    if (ia + 1 <= g.itot) xb = psihat[(size_t)m * g.itot + ia];
+    if (m == 0 || 2 * m == n) { xa.y = 0.; xb.y = 0.; }
+    buf[e] = upper ? make_double2(xa.x + xb.y, -xa.y + xb.x) : make_double2(xa.x - xb.y, xa.y + xb.x);
+  }
+  __syncthreads();
+  fft_il<NTF, EMAXF, +1>(buf, S, n2, pl.np2, pl.code2, tw, n / n2);
+  for (int e = threadIdx.x; e < S * n2; e += NTF) {
+    const int j2 = e / S, s = e - j2 * S;
+    const int h = s / TILE, q = s - h * TILE;
+    const int k1 = h ? k1b : k1a;
+    const int pair = tile * TILE + q;
+    if (pair < pl.npair) {
+      double2 z = buf[e];
+      if (k1 && j2) {
+        double2 w = tw[j2 * k1];
+        w.y = -w.y;
+        z = cmul(z, w);
+      }
+      Y[(size_t)(k1 * n2 + j2) * pl.npair + pair] = z;
+    }
+  }
+}
+
+// ------------------------------------------------------------ inverse A'
+
+__global__ void __launch_bounds__(NTF)
+    k_pinv_a(Geo g, Fft2Plan pl, const double2 *__restrict__ tw, const double2 *__restrict__ Y,
+             double *__restrict__ psi) {
+  __shared__ double2 buf[EMAXF * NTF];
+  const int tile = blockIdx.x, j20 = blockIdx.y * pl.G;
+  const int S = pl.G * TILE, n1 = pl.n1, n2 = pl.n2;
+  for (int e = threadIdx.x; e < S * n1; e += NTF) {
+    const int k1 = e / S, s = e - k1 * S;
+    const int gg = s / TILE, q = s - gg * TILE;
+    const int pair = tile * TILE + q;
+    buf[e] = pair < pl.npair ? Y[(size_t)(k1 * n2 + j20 + gg) * pl.npair + pair]
+                             : make_double2(0., 0.);
+  }
+  __syncthreads();
+  fft_il<NTF, EMAXF, +1>(buf, S, n1, pl.np1, pl.code1, tw, pl.n / n1);
+  const double njtot = 1. * g.jtot;
+  for (int e = threadIdx.x; e < S * n1; e += NTF) {
+    const int j1 = e / S, s = e - j1 * S;
+    const int gg = s / TILE, q = s - gg * TILE;
+    const int j = n2 * j1 + j20 + gg + 1;  // 1-based row
+    const int ia = 2 * (tile * TILE + q) + 1;
+    const double2 z = buf[e];
+    if (ia <= g.itot) put_psi(g, psi, ia, j, z.x / njtot);
+    if (ia + 1 <= g.itot) put_psi(g, psi, ia + 1, j, z.y / njtot);
+  }
+}
+
+// --------------------------------------------------- x solve per wavenumber
+
+// One workgroup per wavenumber row; the row is staged through LDS (coalesced
+// global access, chunk order for the solver) and the same LDS then holds the
+// reduced system.  Matrix: src/fluid/compute_potential.c:117-127.  The J = 0
+// system is singular; its last equation is replaced by psi_hat(itot) = 0.
+template <int NT, int M>
+struct PxLds {
+  union {
+    double2 row[NT * (M + 1)];
+    TridiagLds<NT, 2> s;
+  };
+};
+
+template <int NT, int M>
+__global__ void __launch_bounds__(NT)
+    k_poisson_x2(Geo g, Tables tb, const double *__restrict__ eigen, double2 *__restrict__ psihat) {
+  extern __shared__ double lds_raw[];
+  PxLds<NT, M> &L = *reinterpret_cast<PxLds<NT, M> *>(lds_raw);
+  const int J = blockIdx.x;
+  const int n = g.itot;
+  double2 *row = psihat + (size_t)J * n;
+  for (int r = threadIdx.x; r < n; r += NT) L.row[r + r / M] = row[r];
+  __syncthreads();
+  const double lam = eigen[J];
+  double a[M], b[M], c[M], d[2][M];
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const int r = threadIdx.x * M + k;
+    if (r < n) {
+      const double l = tb.pl[r], u = tb.pu[r];
+      double cc = -l - u + lam;
+      if (r == 0) cc += l;
+      if (r == n - 1) cc += u;
+      a[k] = (r == 0) ? 0. : l;
+      b[k] = cc;
+      c[k] = (r == n - 1) ? 0. : u;
+      const double2 v = L.row[r + r / M];
+      d[0][k] = v.x;
+      d[1][k] = v.y;
+      if (J == 0 && r == n - 1) { a[k] = 0.; b[k] = 1.; d[0][k] = 0.; d[1][k] = 0.; }
+    } else {
+      a[k] = 0.; b[k] = 1.; c[k] = 0.; d[0][k] = 0.; d[1][k] = 0.;
+    }
+  }
+  __syncthreads();  // the row buffer becomes the reduced-system storage
+  block_tridiag<NT, M, 2>(a, b, c, d, L.s);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const int r = threadIdx.x * M + k;
+    if (r < n) L.row[r + r / M] = make_double2(d[0][k], d[1][k]);
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < n; r += NT) row[r] = L.row[r + r / M];
+}
+
+template <int NT, int M>
+static void px_launch(hipStream_t s, const Geo &g, const Tables &tb, const double *eigen,
+                      double2 *psihat) {
+  const size_t lds = sizeof(PxLds<NT, M>);
+  static bool attr = false;
+  if (!attr) {
+    if (lds > 65536)
+      hipFuncSetAttribute((const void *)k_poisson_x2<NT, M>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_poisson_x2<NT, M>), dim3(g.jtot / 2 + 1), dim3(NT), lds, s, g, tb, eigen,
+                     psihat);
+}
+
+// ------------------------------------------------------------- plan, launch
+
+static bool radix_code(int m, int &np, unsigned long long &code) {
+  int rest = m;
+  np = 0;
+  code = 0;
+  auto push = [&](int r) {
+    code |= (unsigned long long)r << (4 * np);
+    np++;
+    rest /= r;
+  };
+  while (rest % 8 == 0 && np < 16) push(8);
+  while (rest % 4 == 0 && np < 16) push(4);
+  while (rest % 2 == 0 && np < 16) push(2);
+  while (rest % 3 == 0 && np < 16) push(3);
+  while (rest % 5 == 0 && np < 16) push(5);
+  return rest == 1;
+}
+
+bool fft2_plan(int n, int itot, Fft2Plan &pl) {
+  if (n < 4) return false;
+  // n1 = the largest divisor of n not above sqrt(n) that the radices cover
+  int best = -1;
+  for (int d = 1; d * d <= n; d++) {
+    if (n % d) continue;
+    int np1, np2;
+    unsigned long long c1, c2;
+    if (radix_code(d, np1, c1) && radix_code(n / d, np2, c2)) best = d;
+  }
+  if (best < 1) return false;
+  pl.n = n;
+  pl.n1 = best;
+  pl.n2 = n / best;
+  if (pl.n1 == 1) {  // keep both passes non-trivial only when possible
+    return false;
+  }
+  radix_code(pl.n1, pl.np1, pl.code1);
+  radix_code(pl.n2, pl.np2, pl.code2);
+  if (2 * TILE * pl.n2 > EMAXF * NTF) return false;  // pass B must fit its LDS tile
+  int G = 1;
+  for (int c = 1; c <= pl.n2; c++)
+    if (pl.n2 % c == 0 && c * TILE * pl.n1 <= EMAXF * NTF) G = c;
+  pl.G = G;
+  pl.npair = (itot + 1) / 2;
+  return TILE * pl.n1 <= EMAXF * NTF;
+}
+
+static dim3 grid_a(const Fft2Plan &pl) { return dim3((pl.npair + TILE - 1) / TILE, pl.n2 / pl.G); }
+static dim3 grid_b(const Fft2Plan &pl) { return dim3((pl.npair + TILE - 1) / TILE, pl.n1 / 2 + 1); }
+
+void launch_poisson2(hipStream_t s, const Geo &g, const Tables &tb, const Fft2Plan &pl,
+                     double gamma, const double *dtp, const double *ux, const double *uy,
+                     const double2 *tw, const double *eigen, double2 *Y, double2 *psihat,
+                     double *psi, int stage) {
+  if (stage == 0 || stage < 0) {
+    hipLaunchKernelGGL(k_pfwd_a, grid_a(pl), dim3(NTF), 0, s, g, tb, pl, gamma, dtp, ux, uy, tw, Y);
+    hipLaunchKernelGGL(k_pfwd_b, grid_b(pl), dim3(NTF), 0, s, g, pl, tw, Y, psihat);
+  }
+  if (stage == 1 || stage < 0) {
+    const int n = g.itot;
+    if (n <= 64 * 4) px_launch<64, 4>(s, g, tb, eigen, psihat);
+    else if (n <= 256 * 4) px_launch<256, 4>(s, g, tb, eigen, psihat);
+    else if (n <= 512 * 4) px_launch<512, 4>(s, g, tb, eigen, psihat);
+    else if (n <= 512 * 8) px_launch<512, 8>(s, g, tb, eigen, psihat);
+    else px_launch<512, 16>(s, g, tb, eigen, psihat);
+  }
+  if (stage == 2 || stage < 0) {
+    hipLaunchKernelGGL(k_pinv_b, grid_b(pl), dim3(NTF), 0, s, g, pl, tw, psihat, Y);
+    hipLaunchKernelGGL(k_pinv_a, grid_a(pl), dim3(NTF), 0, s, g, pl, tw, Y, psi);
+  }
+}
+
+}  // namespace sns

@@ -253,3 +253,93 @@ def test_steps_match_oracle(sns, case, fused):
         assert_close(interior(get(n)), interior(o.f[n]), 1e-12, n)
     assert_close(interior(get("p")), interior(o.f["p"]), 1e-10, "p", demean=True)
     inst.close()
+
+
+# ------------------------------------------------------------ fused path
+
+def test_division_scheme_is_ieee(sns):
+    """The fused kernels divide by tabulated reciprocals with two FMA
+    corrections; every quotient must equal the IEEE one bit for bit."""
+    import ctypes as C
+    rng = np.random.default_rng(123)
+    n = 1 << 22
+    g = sns.grid(4096, 8, 2.0, 3.0)
+    divs = np.concatenate([g["dxf"], g["dxc"], [2.0 / 8192, 2.0 / 256, 1.0 / 3.0, 0.7]])
+    b = divs[rng.integers(0, divs.size, n)]
+    mant = rng.uniform(-1, 1, n)
+    expo = rng.integers(-60, 60, n).astype(float)
+    a = np.ldexp(mant, expo.astype(int))
+    a[:1000] = 0.0
+    a[1000:2000] = -0.0
+    # quotients close to rounding boundaries: a = b*(q + half ulp) rounded
+    q = rng.uniform(1, 2, 100000)
+    a[2000:102000] = b[2000:102000] * (q + np.spacing(q) * 0.5)
+    r = 1.0 / b
+    inst = sns.Instance(8, 8)
+    L = inst.L
+    bufs = []
+    for arr in (a, b, r, np.zeros(n), np.zeros(n)):
+        p = C.c_void_p()
+        assert L.snsg_malloc(inst.ctx, arr.nbytes, C.byref(p)) == 0
+        assert L.snsg_upload(inst.ctx, p, arr.ctypes.data, arr.nbytes) == 0
+        bufs.append(p)
+    assert L.snsg_check_division(inst.ctx, n, *bufs) == 0
+    qcr, qie = np.zeros(n), np.zeros(n)
+    assert L.snsg_download(inst.ctx, qcr.ctypes.data, bufs[3], qcr.nbytes) == 0
+    assert L.snsg_download(inst.ctx, qie.ctypes.data, bufs[4], qie.nbytes) == 0
+    for p in bufs:
+        L.snsg_free(inst.ctx, p)
+    inst.close()
+    assert np.array_equal(qcr.view(np.uint64), qie.view(np.uint64))
+    assert np.array_equal(qie.view(np.uint64), (a / b).view(np.uint64))
+
+
+FUSED_CASES = [
+    # (itot, jtot, stretch, Ra, Pr, imx, imy)
+    (64, 128, 100.0, 1e8, 1.0, 0, 0),
+    (64, 128, 3.0, 1e10, 1.0, 1, 1),
+    (96, 192, 100.0, 1e11, 0.7, 1, 0),
+    (48, 96, 3.0, 1e6, 2.0, 0, 1),
+    (300, 64, 3.0, 1e9, 1.0, 1, 0),
+]
+
+
+@pytest.mark.parametrize("case", FUSED_CASES)
+def test_fused_step_matches_reference_functions(sns, case):
+    """The fused step evaluates the same expressions as the reference-named
+    kernels (the division scheme is exact and the line solvers partition the
+    same way at these sizes), so the fields agree bit for bit."""
+    itot, jtot, stretch, Ra, Pr, imx, imy = case
+    o, inst = seeded_pair(sns, itot, jtot, stretch, Ra, Pr, imx, imy, 9, 1e-2)
+    inst.state_alloc()
+    for n in ["ux", "uy", "p", "psi", "temp"]:
+        inst.state_upload(n, o.f[n])
+    for _ in range(5):
+        dt = sns.decide_dt(inst.g, Ra, Pr, inst.dt_adv_raw(), imx, imy)
+        inst.set_dt(dt)
+        inst.integrate_reference_order()
+        inst.integrate()
+    for n in ["ux", "uy", "p", "temp"]:
+        a, b = inst.state_download(n), inst.download(n)
+        assert np.array_equal(a, b), (n, np.abs(a - b).max())
+    inst.close()
+
+
+@pytest.mark.parametrize("case", STEP_CASES)
+def test_advance_matches_oracle_time_loop(sns, case):
+    """snsg_advance (time step decided on the device) against the oracle's
+    time loop: fields within 1e-12, accumulated time within 1e-13."""
+    itot, jtot, stretch, Ra, Pr, imx, imy, seed, amp, steps = case
+    o, inst = seeded_pair(sns, itot, jtot, stretch, Ra, Pr, imx, imy, seed, amp)
+    inst.state_alloc()
+    for n in ["ux", "uy", "p", "psi", "temp"]:
+        inst.state_upload(n, o.f[n])
+    inst.set_time(0.0)
+    inst.advance(steps)
+    o.run(steps)
+    t, dt = inst.get_time()
+    assert abs(t - o.param.time) <= 1e-13 * o.param.time
+    assert abs(dt - o.dt) <= 1e-12 * o.dt
+    for n in ["ux", "uy", "temp"]:
+        assert_close(interior(inst.state_download(n)), interior(o.f[n]), 1e-12, n)
+    inst.close()
