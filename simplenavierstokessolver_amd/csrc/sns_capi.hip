@@ -38,6 +38,9 @@ struct snsg_ctx {
   Tables tab{};
   // fused-step scratch: x-aligned increments of T, ux, uy when a y solve follows
   double *d_qf[3] = {nullptr, nullptr, nullptr};
+  // LU tables of the CN-x operators (3 substages x 3 fields x 5 tables)
+  double *d_lu = nullptr;
+  int lu_ntm = 0;
   double *d_time = nullptr;
   double dt_dif_x = 0., dt_dif_y = 0.;
   // timing
@@ -236,11 +239,11 @@ extern "C" int snsg_create(int device, const snsg_config *cfg, const double *dxf
 
 extern "C" int snsg_destroy(snsg_ctx *c) {
   if (!c) return 0;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *bufs[] = {c->d_dxf, c->d_dxc, c->d_dt, c->d_eigen, c->d_tw, c->d_psihat, c->d_Y, c->d_q,
                   c->d_qy, c->d_red, c->d_tab, c->d_time, c->d_qf[0], c->d_qf[1],
-                  c->d_qf[2]};
+                  c->d_qf[2], c->d_lu};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (c->h_red) hipHostFree(c->h_red);
@@ -537,6 +540,21 @@ static int integrate_fused(snsg_ctx *c, snsg_state *st, bool reduce) {
       if ((rc = ctx_alloc((void **)&c->d_qf[k], sizeof(double) * nb[k]))) return rc;
   }
   const bool with_temp = c->cfg.with_temperature != 0;
+  if (impx) {
+    if (!c->d_lu) {
+      c->lu_ntm = lu_table_entries(itot);
+      if ((rc = ctx_alloc((void **)&c->d_lu, sizeof(double) * 45 * (size_t)c->lu_ntm))) return rc;
+    }
+    // the CN-x operators of all three substages depend on dt only
+    LuFactorArgs la;
+    la.g = g;
+    la.tb = c->tab;
+    la.dtp = c->d_dt;
+    for (int k = 0; k < 3; k++) la.gamma[k] = c->cfg.gamma[k];
+    la.out = c->d_lu;
+    la.ntm = c->lu_ntm;
+    if ((rc = run(c, "lu_factor", [&] { launch_lu_factor(c->stream, la); }))) return rc;
+  }
   for (int rk = 0; rk < 3; rk++) {
     PredArgs pa;
     pa.g = g;
@@ -554,7 +572,9 @@ static int integrate_fused(snsg_ctx *c, snsg_state *st, bool reduce) {
     pa.write_hist = rk < 2;
     pa.with_temp = with_temp;
     pa.forcing = with_temp && c->cfg.with_thermal_forcing;
-    if ((rc = run(c, "pred", [&] { launch_pred(c->stream, pa, impx); }))) return rc;
+    const double *lu = c->d_lu ? c->d_lu + (size_t)(rk * 3 * 5) * c->lu_ntm : nullptr;
+    if ((rc = run(c, "pred", [&] { launch_pred(c->stream, pa, impx, lu, c->lu_ntm); })))
+      return rc;
     if (impy) {
       const RK rkc = rk_of(c, rk);
       const int kinds[3] = {KIND_TEMP, KIND_UX, KIND_UY};

@@ -31,8 +31,22 @@ struct CorrArgs {
   unsigned long long *red;  // advective-bound reduction slot, or nullptr
 };
 
+// Thomas factorisation tables of the three CN-x operators for the three
+// substages: out holds 9 groups (substage s, field f at (s*3+f)) of 5 tables
+// of ntm entries each (see sns_lu.h)
+struct LuFactorArgs {
+  Geo g;
+  Tables tb;
+  const double *dtp;
+  double gamma[3];
+  double *out;
+  int ntm;
+};
+
 bool pred_impx_supported(int itot);
-void launch_pred(hipStream_t s, const PredArgs &a, int implicitx);
+int lu_table_entries(int itot);
+void launch_lu_factor(hipStream_t s, const LuFactorArgs &a);
+void launch_pred(hipStream_t s, const PredArgs &a, int implicitx, const double *lu, int ntm);
 void launch_add_out(hipStream_t s, const Geo &g, int kind, const double *in, const double *q,
                     double *out);
 void launch_corr(hipStream_t s, const CorrArgs &a);

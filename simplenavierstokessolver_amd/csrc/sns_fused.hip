@@ -16,7 +16,7 @@
 #include "sns_launch.h"
 #include "sns_fast.h"
 #include "sns_fused.h"
-#include "sns_tridiag.h"
+#include "sns_lu.h"
 
 namespace sns {
 
@@ -127,12 +127,142 @@ __global__ void __launch_bounds__(256) k_pred_exp(PredArgs a) {
   pred_emit(a, i, j, r);
 }
 
-// Crank-Nicolson in x: one workgroup per row; increments staged in LDS
-// (lane order for the stencils, chunk order for the solver).
+// ------------------------------------------ shared LU factorisation of CN-x
+
+// Coefficients of row r (0-based unknown index) of the CN-x operator of
+// field f (0: T, 1: ux, 2: uy): src/temperature/update.c:141-149,
+// src/fluid/update_velocity.c:253-261,398-406.
+__device__ __forceinline__ void cnx_row(const Geo &g, const Tables &tb, int f, double pf, int r,
+                                        double &l, double &c, double &u) {
+  const int i = r + (f == 1 ? 2 : 1);
+  const double coef_xm = f == 1 ? tb.nU_xm[i - 1] : tb.nT_xm[i - 1];
+  const double coef_xp = f == 1 ? tb.nU_xp[i - 1] : tb.nT_xp[i - 1];
+  const double coef_x0 = -coef_xm - coef_xp;
+  l = -pf * coef_xm;
+  u = -pf * coef_xp;
+  c = 1. - pf * coef_x0;
+}
+
+__device__ __forceinline__ double cn_pf(const Geo &g, int f, double gdt) {
+  return f == 0 ? gdt / g.two_sqrtPr_sqrtRa : (gdt * g.sqrtPr) / g.two_sqrtRa;
+}
+
+struct M22 {
+  double a, b, c, d;
+};
+__device__ __forceinline__ M22 mmul(const M22 &x, const M22 &y) {  // x*y
+  return {x.a * y.a + x.b * y.c, x.a * y.b + x.b * y.d, x.c * y.a + x.d * y.c,
+          x.c * y.b + x.d * y.d};
+}
+// exact power-of-two rescaling (ratios of entries are unchanged)
+__device__ __forceinline__ M22 mnorm(M22 m) {
+  const double mx = fmax(fmax(fabs(m.a), fabs(m.b)), fmax(fabs(m.c), fabs(m.d)));
+  if (mx == 0. || !isfinite(mx)) return m;
+  const int e = ilogb(mx);
+  return {ldexp(m.a, -e), ldexp(m.b, -e), ldexp(m.c, -e), ldexp(m.d, -e)};
+}
+__device__ __forceinline__ M22 mshfl_up(const M22 &m, int off) {
+  return {__shfl_up(m.a, off), __shfl_up(m.b, off), __shfl_up(m.c, off), __shfl_up(m.d, off)};
+}
+
+// Thomas factorisation of the CN-x operator of field blockIdx.x for the
+// substage blockIdx.y, written as the interleaved tables of sns_lu.h.  The
+// pivot recurrence u'_i = u_i/(c_i - l_i u'_{i-1}) is the Moebius map of the
+// vector v_i = [[0, u_i], [-l_i, c_i]] v_{i-1}, v_0 = (u_0, c_0), with
+// u'_i = p_i/q_i and 1/pivot_i = q_{i-1}/q_i; the products of these 2x2
+// matrices are scanned across the workgroup.
 template <int NT, int M>
-struct PredLds {
+__global__ void __launch_bounds__(NT) k_lu_factor(LuFactorArgs a) {
+  constexpr int NW = NT / 64;
+  __shared__ double wtot[NW][4];
+  const Geo &g = a.g;
+  const int f = blockIdx.x, s = blockIdx.y;
+  const int n = f == 1 ? g.itot - 1 : g.itot;
+  const double pf = cn_pf(g, f, a.gamma[s] * (*a.dtp));
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  // chunk product of the row matrices (rows >= 1)
+  M22 T = {1., 0., 0., 1.};
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const int r = t * M + k;
+    if (r >= 1 && r < n) {
+      double l, c, u;
+      cnx_row(g, a.tb, f, pf, r, l, c, u);
+      T = mnorm(mmul(M22{0., u, -l, c}, T));
+    }
+  }
+  // inclusive scan within the wave (later chunks multiply from the left)
+  M22 X = T;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const M22 o = mshfl_up(X, off);
+    if (lane >= off) X = mnorm(mmul(X, o));
+  }
+  if (lane == 63) {
+    wtot[wave][0] = X.a; wtot[wave][1] = X.b; wtot[wave][2] = X.c; wtot[wave][3] = X.d;
+  }
+  __syncthreads();
+  M22 cin = {1., 0., 0., 1.};
+  for (int w = 0; w < wave; w++)
+    cin = mnorm(mmul(M22{wtot[w][0], wtot[w][1], wtot[w][2], wtot[w][3]}, cin));
+  const M22 incl = mnorm(mmul(X, cin));
+  M22 excl = mshfl_up(incl, 1);
+  if (lane == 0) excl = cin;
+  // v before this chunk's first row
+  double l0, c0, u0;
+  cnx_row(g, a.tb, f, pf, 0, l0, c0, u0);
+  double p = excl.a * u0 + excl.b * c0, q = excl.c * u0 + excl.d * c0;
+  double rr[M], al[M], be[M];
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const int r = t * M + k;
+    rr[k] = al[k] = be[k] = 0.;
+    if (r >= n) continue;
+    double l, c, u;
+    cnx_row(g, a.tb, f, pf, r, l, c, u);
+    double up;
+    if (r == 0) {
+      rr[k] = 1. / c;
+      up = u / c;
+      p = u0;
+      q = c0;
+    } else {
+      const double pn = u * q, qn = -l * p + c * q;
+      rr[k] = q / qn;
+      up = pn / qn;
+      al[k] = -rr[k] * l;
+      const double mx = fmax(fabs(pn), fabs(qn));
+      const int e = ilogb(mx);
+      p = ldexp(pn, -e);
+      q = ldexp(qn, -e);
+    }
+    be[k] = (r == n - 1) ? 0. : -up;
+  }
+  double *base = a.out + (size_t)((s * 3 + f) * 5) * a.ntm;
+  double P = 1., Q = 1.;
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    P *= al[k];
+    base[0 * a.ntm + k * NT + t] = rr[k];
+    base[1 * a.ntm + k * NT + t] = al[k];
+    base[2 * a.ntm + k * NT + t] = be[k];
+    base[3 * a.ntm + k * NT + t] = P;
+  }
+#pragma unroll
+  for (int k = M - 1; k >= 0; k--) {
+    Q *= be[k];
+    base[4 * a.ntm + k * NT + t] = Q;
+  }
+}
+
+// ----------------------------------------------- row-wise fused predictor
+
+// Crank-Nicolson in x: one workgroup per row; increments staged in LDS (lane
+// order for the stencils, chunk order for the line solves).
+template <int NT, int M>
+struct PredRowLds {
   double d[3][NT * (M + 1)];  // padded: chunk t starts at t*(M+1)
-  TridiagLds<NT, 1> s;
+  double scr[2 * 3 * (NT / 64)];
 };
 
 template <int M>
@@ -140,43 +270,10 @@ __device__ __forceinline__ int pad_idx(int r) {
   return r + r / M;
 }
 
-template <int KIND, int NT, int M>
-__device__ __forceinline__ void cnx_solve_row(const PredArgs &a, double pf, double *sd,
-                                              TridiagLds<NT, 1> &s) {
-  const Geo &g = a.g;
-  const int n = KIND == KIND_UX ? g.itot - 1 : g.itot;
-  const int i0 = KIND == KIND_UX ? 2 : 1;
-  const double *cxm = KIND == KIND_UX ? a.tb.nU_xm : a.tb.nT_xm;
-  const double *cxp = KIND == KIND_UX ? a.tb.nU_xp : a.tb.nT_xp;
-  double va[M], vb[M], vc[M], vd[1][M];
-#pragma unroll
-  for (int k = 0; k < M; k++) {
-    const int r = threadIdx.x * M + k;
-    if (r < n) {
-      // src/temperature/update.c:141-149, src/fluid/update_velocity.c:253-261,398-406
-      const double coef_xm = cxm[r + i0 - 1];
-      const double coef_xp = cxp[r + i0 - 1];
-      const double coef_x0 = -coef_xm - coef_xp;
-      va[k] = (r == 0) ? 0. : -pf * coef_xm;
-      vc[k] = (r == n - 1) ? 0. : -pf * coef_xp;
-      vb[k] = 1. - pf * coef_x0;
-      vd[0][k] = sd[pad_idx<M>(r)];
-    } else {
-      va[k] = 0.; vb[k] = 1.; vc[k] = 0.; vd[0][k] = 0.;
-    }
-  }
-  block_tridiag<NT, M, 1>(va, vb, vc, vd, s);
-#pragma unroll
-  for (int k = 0; k < M; k++) {
-    const int r = threadIdx.x * M + k;
-    if (r < n) sd[pad_idx<M>(r)] = vd[0][k];
-  }
-}
-
 template <int NT, int M>
-__global__ void __launch_bounds__(NT) k_pred_impx(PredArgs a) {
+__global__ void __launch_bounds__(NT) k_pred_row(PredArgs a, const double *lu, int ntm) {
   extern __shared__ double lds_raw[];
-  PredLds<NT, M> &L = *reinterpret_cast<PredLds<NT, M> *>(lds_raw);
+  PredRowLds<NT, M> &L = *reinterpret_cast<PredRowLds<NT, M> *>(lds_raw);
   const Geo &g = a.g;
   const int j = xcd_remap(blockIdx.x, gridDim.x) + 1;
   const double dt = *a.dtp;
@@ -191,11 +288,30 @@ __global__ void __launch_bounds__(NT) k_pred_impx(PredArgs a) {
     }
   }
   __syncthreads();
-  const double gdt = a.gamma * dt;
-  if (a.with_temp) cnx_solve_row<KIND_TEMP, NT, M>(a, gdt / g.two_sqrtPr_sqrtRa, L.d[0], L.s);
-  const double pfu = (gdt * g.sqrtPr) / g.two_sqrtRa;
-  cnx_solve_row<KIND_UX, NT, M>(a, pfu, L.d[1], L.s);
-  cnx_solve_row<KIND_UY, NT, M>(a, pfu, L.d[2], L.s);
+  const int nf[3] = {g.itot, g.itot - 1, g.itot};
+  double dd[3][M];
+#pragma unroll
+  for (int f = 0; f < 3; f++)
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+      const int r = threadIdx.x * M + k;
+      dd[f][k] = r < nf[f] ? L.d[f][pad_idx<M>(r)] : 0.;
+    }
+  LuTab tb[3];
+#pragma unroll
+  for (int f = 0; f < 3; f++) {
+    const double *b = lu + (size_t)(f * 5) * ntm;
+    tb[f] = {b, b + ntm, b + 2 * ntm, b + 3 * ntm, b + 4 * ntm};
+  }
+  if (!a.with_temp) tb[0].r = nullptr;
+  lu_solve<NT, M, 3>(dd, tb, L.scr);
+#pragma unroll
+  for (int f = 0; f < 3; f++)
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+      const int r = threadIdx.x * M + k;
+      if (r < nf[f]) L.d[f][pad_idx<M>(r)] = dd[f][k];
+    }
   __syncthreads();
   for (int k = 0; k < nk; k++) {
     const int i = 1 + threadIdx.x + k * NT;
@@ -213,29 +329,64 @@ __global__ void __launch_bounds__(NT) k_pred_impx(PredArgs a) {
 }
 
 template <int NT, int M>
-static void pred_impx_launch(hipStream_t s, const PredArgs &a) {
-  const size_t lds = sizeof(PredLds<NT, M>);
+static void pred_row_launch(hipStream_t s, const PredArgs &a, const double *lu, int ntm) {
+  const size_t lds = sizeof(PredRowLds<NT, M>);
   static bool attr = false;
   if (!attr) {
-    hipFuncSetAttribute((const void *)k_pred_impx<NT, M>,
+    hipFuncSetAttribute((const void *)k_pred_row<NT, M>,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr = true;
   }
-  hipLaunchKernelGGL((k_pred_impx<NT, M>), dim3(a.g.jtot), dim3(NT), lds, s, a);
+  hipLaunchKernelGGL((k_pred_row<NT, M>), dim3(a.g.jtot), dim3(NT), lds, s, a, lu, ntm);
 }
 
-bool pred_impx_supported(int itot) { return itot <= 4096; }
+template <int NT, int M>
+static void lu_factor_launch(hipStream_t s, const LuFactorArgs &a) {
+  hipLaunchKernelGGL((k_lu_factor<NT, M>), dim3(3, 3), dim3(NT), 0, s, a);
+}
 
-void launch_pred(hipStream_t s, const PredArgs &a, int implicitx) {
+// (NT, M) of the row kernels for a given itot; ntm = NT*M
+static int row_config(int itot, int &NT, int &M) {
+  M = 4;
+  if (itot <= 64 * 4) NT = 64;
+  else if (itot <= 256 * 4) NT = 256;
+  else if (itot <= 512 * 4) NT = 512;
+  else if (itot <= 1024 * 4) NT = 1024;
+  else return 0;
+  return NT * M;
+}
+
+bool pred_impx_supported(int itot) {
+  int NT, M;
+  return row_config(itot, NT, M) > 0;
+}
+
+int lu_table_entries(int itot) {
+  int NT, M;
+  return row_config(itot, NT, M);
+}
+
+void launch_lu_factor(hipStream_t s, const LuFactorArgs &a) {
+  int NT, M;
+  row_config(a.g.itot, NT, M);
+  if (NT == 64) lu_factor_launch<64, 4>(s, a);
+  else if (NT == 256) lu_factor_launch<256, 4>(s, a);
+  else if (NT == 512) lu_factor_launch<512, 4>(s, a);
+  else lu_factor_launch<1024, 4>(s, a);
+}
+
+void launch_pred(hipStream_t s, const PredArgs &a, int implicitx, const double *lu, int ntm) {
   const int itot = a.g.itot;
   if (!implicitx) {
     hipLaunchKernelGGL(k_pred_exp, dim3((itot + 255) / 256, a.g.jtot), dim3(256), 0, s, a);
     return;
   }
-  if (itot <= 64 * 4) pred_impx_launch<64, 4>(s, a);
-  else if (itot <= 256 * 4) pred_impx_launch<256, 4>(s, a);
-  else if (itot <= 512 * 4) pred_impx_launch<512, 4>(s, a);
-  else pred_impx_launch<512, 8>(s, a);
+  int NT, M;
+  row_config(itot, NT, M);
+  if (NT == 64) pred_row_launch<64, 4>(s, a, lu, ntm);
+  else if (NT == 256) pred_row_launch<256, 4>(s, a, lu, ntm);
+  else if (NT == 512) pred_row_launch<512, 4>(s, a, lu, ntm);
+  else pred_row_launch<1024, 4>(s, a, lu, ntm);
 }
 
 // ----------------------------------------------------- y-solve completion

@@ -1,0 +1,128 @@
+// sns_lu.h -- Crank-Nicolson line solves with a shared LU factorisation.
+//
+// The x-direction Crank-Nicolson operators (1 - pf*Dxx) of T, ux and uy
+// (src/temperature/update.c:141-154, src/fluid/update_velocity.c:248-267,
+// 398-411) depend on the column index and on gamma*dt only, so every row of a
+// substage solves the same matrix.  The Thomas factorisation of that matrix
+// (the reference's u'_i and 1/pivot, src/linalg.c:9-30) is computed once per
+// substage by k_lu_factor; each row then needs only the two first-order
+// recurrences
+//   forward  y_i = r_i d_i + al_i y_{i-1},     al_i = -r_i l_i
+//   backward x_i = y_i + be_i x_{i+1},         be_i = -u'_i
+// evaluated in parallel: every thread runs them over its chunk of M rows with
+// zero carries, the chunk-end values are combined by an affine scan across
+// the workgroup (wave shuffles, then one value per wave), and each element
+// is corrected with the precomputed products of al (P) and be (Q) across its
+// chunk.  Rounding differs from the sequential sweep at the ulp level.
+//
+// Tables are stored interleaved, entry (t, k) of row i = t*M + k at
+// [k*NT + t], so that the loads of a wave are contiguous.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace sns {
+
+struct LuTab {
+  const double *r, *al, *be, *P, *Q;  // each NT*M entries, interleaved
+};
+
+// composition of affine maps x -> A x + B: (A2,B2) after (A1,B1)
+__device__ __forceinline__ void aff_after(double &A, double &B, double A1, double B1) {
+  B = A * B1 + B;
+  A = A * A1;
+}
+
+// Exclusive affine scan over the threads of a workgroup (forward: thread t
+// receives the composition of maps 0..t-1 applied to 0; reverse: of maps
+// NT-1..t+1).  NF independent scans share the barriers.  scr holds
+// 2*NF*(NT/64) doubles.
+template <int NT, int NF, bool REVERSE>
+__device__ __forceinline__ void affine_scan_excl(double (&A)[NF], double (&B)[NF], double (&C)[NF],
+                                                 double *scr) {
+  constexpr int NW = NT / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // inclusive scan inside the wave
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      const double Ao = REVERSE ? __shfl_down(A[f], off) : __shfl_up(A[f], off);
+      const double Bo = REVERSE ? __shfl_down(B[f], off) : __shfl_up(B[f], off);
+      const bool ok = REVERSE ? (lane + off < 64) : (lane >= off);
+      if (ok) aff_after(A[f], B[f], Ao, Bo);
+    }
+  }
+  // wave totals
+  const int last = REVERSE ? 0 : 63;
+  if (lane == last) {
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      scr[(f * NW + wave) * 2 + 0] = A[f];
+      scr[(f * NW + wave) * 2 + 1] = B[f];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    double cin = 0.;
+    if (!REVERSE) {
+      for (int w = 0; w < wave; w++) cin = scr[(f * NW + w) * 2] * cin + scr[(f * NW + w) * 2 + 1];
+    } else {
+      for (int w = NW - 1; w > wave; w--) cin = scr[(f * NW + w) * 2] * cin + scr[(f * NW + w) * 2 + 1];
+    }
+    const double incl = A[f] * cin + B[f];  // value after this thread's map
+    const double prev = REVERSE ? __shfl_down(incl, 1) : __shfl_up(incl, 1);
+    const bool edge = REVERSE ? (lane == 63) : (lane == 0);
+    C[f] = edge ? cin : prev;
+  }
+  __syncthreads();
+}
+
+// Solve NF systems (one per field) for this thread's chunk d[f][0..M-1]
+// (rows t*M .. t*M+M-1); tab[f].r == nullptr skips field f.
+template <int NT, int M, int NF>
+__device__ __forceinline__ void lu_solve(double (&d)[NF][M], const LuTab (&tab)[NF], double *scr) {
+  const int t = threadIdx.x;
+  double A[NF], B[NF], C[NF];
+  // forward recurrences with zero carry in
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    A[f] = 0.;
+    B[f] = 0.;
+    if (!tab[f].r) continue;
+    double y = 0.;
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+      const double r = tab[f].r[k * NT + t], al = tab[f].al[k * NT + t];
+      y = r * d[f][k] + al * y;
+      d[f][k] = y;
+    }
+    A[f] = tab[f].P[(M - 1) * NT + t];
+    B[f] = y;
+  }
+  affine_scan_excl<NT, NF, false>(A, B, C, scr);
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    if (!tab[f].r) continue;
+#pragma unroll
+    for (int k = 0; k < M; k++) d[f][k] += tab[f].P[k * NT + t] * C[f];
+    // backward recurrences with zero carry in
+    double x = 0.;
+#pragma unroll
+    for (int k = M - 1; k >= 0; k--) {
+      x = d[f][k] + tab[f].be[k * NT + t] * x;
+      d[f][k] = x;
+    }
+    A[f] = tab[f].Q[t];
+    B[f] = x;
+  }
+  affine_scan_excl<NT, NF, true>(A, B, C, scr);
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    if (!tab[f].r) continue;
+#pragma unroll
+    for (int k = 0; k < M; k++) d[f][k] += tab[f].Q[k * NT + t] * C[f];
+  }
+}
+
+}  // namespace sns

@@ -307,8 +307,10 @@ FUSED_CASES = [
 @pytest.mark.parametrize("case", FUSED_CASES)
 def test_fused_step_matches_reference_functions(sns, case):
     """The fused step evaluates the same expressions as the reference-named
-    kernels (the division scheme is exact and the line solvers partition the
-    same way at these sizes), so the fields agree bit for bit."""
+    kernels (the division scheme is exact), so with explicit x diffusion the
+    fields agree bit for bit; with Crank-Nicolson in x the fused step uses the
+    shared-LU line solver, which rounds differently from the per-function
+    solver (1e-12, the tolerance both paths meet against the oracle)."""
     itot, jtot, stretch, Ra, Pr, imx, imy = case
     o, inst = seeded_pair(sns, itot, jtot, stretch, Ra, Pr, imx, imy, 9, 1e-2)
     inst.state_alloc()
@@ -321,7 +323,10 @@ def test_fused_step_matches_reference_functions(sns, case):
         inst.integrate()
     for n in ["ux", "uy", "p", "temp"]:
         a, b = inst.state_download(n), inst.download(n)
-        assert np.array_equal(a, b), (n, np.abs(a - b).max())
+        if imx == 0:
+            assert np.array_equal(a, b), (n, np.abs(a - b).max())
+        else:
+            assert_close(interior(a), interior(b), 1e-12, n, demean=(n == "p"))
     inst.close()
 
 
