@@ -31,6 +31,8 @@ struct snsg_ctx {
   double *d_q = nullptr, *d_qy = nullptr;
   unsigned long long *d_red = nullptr;
   unsigned long long *h_red = nullptr;  // pinned
+  double *d_part = nullptr;             // per-workgroup minima of the fused correction
+  int nparts = 0;
   Fft2Plan plan{};
   double2 *d_Y = nullptr;  // four-step FFT intermediate, jtot rows of (itot+1)/2 pairs
   // per-column tables of the fused kernels (one allocation)
@@ -243,7 +245,7 @@ extern "C" int snsg_destroy(snsg_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *bufs[] = {c->d_dxf, c->d_dxc, c->d_dt, c->d_eigen, c->d_tw, c->d_psihat, c->d_Y, c->d_q,
                   c->d_qy, c->d_red, c->d_tab, c->d_time, c->d_qf[0], c->d_qf[1],
-                  c->d_qf[2], c->d_lu};
+                  c->d_qf[2], c->d_lu, c->d_part};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (c->h_red) hipHostFree(c->h_red);
@@ -602,7 +604,7 @@ static int integrate_fused(snsg_ctx *c, snsg_state *st, bool reduce) {
     ca.ux = st->ux; ca.uy = st->uy; ca.p = st->p;
     ca.dtp = c->d_dt;
     ca.gamma = gamma;
-    ca.red = (reduce && rk == 2) ? c->d_red : nullptr;
+    ca.red = (reduce && rk == 2) ? c->d_part : nullptr;
     if ((rc = run(c, "corr", [&] { launch_corr(c->stream, ca); }))) return rc;
     if (with_temp) std::swap(st->temp, st->temp_alt);
   }
@@ -616,14 +618,23 @@ extern "C" int snsg_integrate(snsg_ctx *c, snsg_state *st) {
 
 extern "C" int snsg_advance(snsg_ctx *c, snsg_state *st, int nsteps) {
   if (!c || !st || !st->ux || nsteps < 0) return set_err(SNSG_E_ARG, __func__);
-  int rc = run(c, "dt_reduce", [&] {
+  int rc;
+  if (!c->d_part) {
+    c->nparts = corr_partials(c->geo);
+    if ((rc = ctx_alloc((void **)&c->d_part, sizeof(double) * (size_t)c->nparts))) return rc;
+  }
+  rc = run(c, "dt_reduce", [&] {
     launch_fill_u64(c->stream, c->d_red, 0x7ff0000000000000ull, 1);
     launch_dt_reduce(c->stream, c->geo, st->ux, st->uy, c->d_red);
   });
   if (rc) return rc;
   for (int n = 0; n < nsteps; n++) {
+    // step 0 starts from the full reduction (one positive double in d_red),
+    // later steps from the minima the previous step's last correction stored
+    const double *part = n == 0 ? reinterpret_cast<const double *>(c->d_red) : c->d_part;
+    const int np = n == 0 ? 1 : c->nparts;
     rc = run(c, "dt", [&] {
-      launch_dt(c->stream, c->d_red, c->d_dt, c->d_time, c->dt_dif_x, c->dt_dif_y,
+      launch_dt(c->stream, part, np, c->d_dt, c->d_time, c->dt_dif_x, c->dt_dif_y,
                 c->treat.implicitx, c->treat.implicity);
     });
     if (rc) return rc;

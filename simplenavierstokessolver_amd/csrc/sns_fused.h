@@ -28,7 +28,7 @@ struct CorrArgs {
   double *ux, *uy, *p;
   const double *dtp;
   double gamma;
-  unsigned long long *red;  // advective-bound reduction slot, or nullptr
+  double *red;  // per-workgroup advective-bound minima (corr_partials entries), or nullptr
 };
 
 // Thomas factorisation tables of the three CN-x operators for the three
@@ -49,9 +49,10 @@ void launch_lu_factor(hipStream_t s, const LuFactorArgs &a);
 void launch_pred(hipStream_t s, const PredArgs &a, int implicitx, const double *lu, int ntm);
 void launch_add_out(hipStream_t s, const Geo &g, int kind, const double *in, const double *q,
                     double *out);
+int corr_partials(const Geo &g);
 void launch_corr(hipStream_t s, const CorrArgs &a);
-void launch_dt(hipStream_t s, unsigned long long *red, double *dt, double *time, double dt_dif_x,
-               double dt_dif_y, int implicitx, int implicity);
+void launch_dt(hipStream_t s, const double *part, int n, double *dt, double *time,
+               double dt_dif_x, double dt_dif_y, int implicitx, int implicity);
 void launch_check_div(hipStream_t s, int n, const double *a, const double *b, const double *r,
                       double *q_cr, double *q_ieee);
 

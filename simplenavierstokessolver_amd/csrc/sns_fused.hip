@@ -257,12 +257,58 @@ __global__ void __launch_bounds__(NT) k_lu_factor(LuFactorArgs a) {
 
 // ----------------------------------------------- row-wise fused predictor
 
-// Crank-Nicolson in x: one workgroup per row; increments staged in LDS (lane
-// order for the stencils, chunk order for the line solves).
+// Increment of field f (0: T, 1: ux, 2: uy) at (i, j); the RK history of the
+// explicit terms is read (beta != 0) and rewritten in place.
+__device__ __forceinline__ double pred_field(const PredArgs &a, int f, int i, int j, double dt) {
+  const Geo &g = a.g;
+  double A, G, B = 0.;
+  if (f == 0) {
+    temp_src_t(g, a.tb, a.ux, a.uy, a.temp, i, j, a.tr.w_ex_x, a.tr.w_ex_y, a.tr.w_im_x,
+               a.tr.w_im_y, A, G);
+    const size_t s = sy_idx(g, i, j);
+    if (a.read_hist) B = a.hist_t[s];
+    if (a.write_hist) a.hist_t[s] = A;
+  } else if (f == 1) {
+    double tf = 0.;
+    if (a.forcing) tf = 0.5 * a.temp[c_idx(g, i - 1, j)] + 0.5 * a.temp[c_idx(g, i, j)];
+    ux_src_t(g, a.tb, a.ux, a.uy, a.p, tf, i, j, a.tr.w_ex_x, a.tr.w_ex_y, a.tr.w_im_x,
+             a.tr.w_im_y, A, G);
+    const size_t s = sx_idx(g, i, j);
+    if (a.read_hist) B = a.hist_ux[s];
+    if (a.write_hist) a.hist_ux[s] = A;
+  } else {
+    uy_src_t(g, a.tb, a.ux, a.uy, a.p, i, j, a.tr.w_ex_x, a.tr.w_ex_y, a.tr.w_im_x, a.tr.w_im_y,
+             A, G);
+    const size_t s = sy_idx(g, i, j);
+    if (a.read_hist) B = a.hist_uy[s];
+    if (a.write_hist) a.hist_uy[s] = A;
+  }
+  return rk_incr(a.alpha, a.beta, a.gamma, dt, A, B, G);
+}
+
+// Final value of field f at (i, j), or its x-aligned increment when a y
+// solve follows.
+__device__ __forceinline__ void emit_field(const PredArgs &a, int f, int i, int j, double d) {
+  const Geo &g = a.g;
+  if (f == 0) {
+    if (a.q_t) a.q_t[sy_idx(g, i, j)] = d;
+    else put_c<2>(g, a.out_t, i, j, a.temp[c_idx(g, i, j)] + d);
+  } else if (f == 1) {
+    if (a.q_u) a.q_u[sx_idx(g, i, j)] = d;
+    else put_ux(g, a.out_u, i, j, a.ux[ux_idx(g, i, j)] + d);
+  } else {
+    if (a.q_v) a.q_v[sy_idx(g, i, j)] = d;
+    else put_c<0>(g, a.out_v, i, j, a.uy[c_idx(g, i, j)] + d);
+  }
+}
+
+// Crank-Nicolson in x: one workgroup per (row, field).  The increments are
+// staged in LDS: lane order for the stencils, chunk order (M unknowns per
+// thread) for the line solve.
 template <int NT, int M>
 struct PredRowLds {
-  double d[3][NT * (M + 1)];  // padded: chunk t starts at t*(M+1)
-  double scr[2 * 3 * (NT / 64)];
+  double d[NT * (M + 1)];  // padded: chunk t starts at t*(M+1)
+  double scr[2 * (NT / 64)];
 };
 
 template <int M>
@@ -270,74 +316,46 @@ __device__ __forceinline__ int pad_idx(int r) {
   return r + r / M;
 }
 
+// Workgroup w covers row w / nfld + 1 and field w % nfld (+1 without the
+// temperature equation); consecutive w share an XCD, so the fields of a row
+// and neighbouring rows reuse one L2.
 template <int NT, int M>
-__global__ void __launch_bounds__(NT) k_pred_row(PredArgs a, const double *lu, int ntm) {
+__global__ void __launch_bounds__(NT) k_pred_row(PredArgs a, const double *lu, int ntm, int nfld) {
   extern __shared__ double lds_raw[];
   PredRowLds<NT, M> &L = *reinterpret_cast<PredRowLds<NT, M> *>(lds_raw);
   const Geo &g = a.g;
-  const int j = xcd_remap(blockIdx.x, gridDim.x) + 1;
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  const int j = w / nfld + 1;
+  const int f = w % nfld + (nfld == 3 ? 0 : 1);
+  const int n = f == 1 ? g.itot - 1 : g.itot;
+  const int i0 = f == 1 ? 2 : 1;
   const double dt = *a.dtp;
-  const int nk = (g.itot + NT - 1) / NT;
-  for (int k = 0; k < nk; k++) {
-    const int i = 1 + threadIdx.x + k * NT;
-    if (i <= g.itot) {
-      const Incr r = pred_point(a, i, j, dt);
-      if (r.has_t) L.d[0][pad_idx<M>(i - 1)] = r.t;
-      if (r.has_u) L.d[1][pad_idx<M>(i - 2)] = r.u;
-      if (r.has_v) L.d[2][pad_idx<M>(i - 1)] = r.v;
-    }
+  for (int ii = threadIdx.x; ii < n; ii += NT) L.d[pad_idx<M>(ii)] = pred_field(a, f, ii + i0, j, dt);
+  __syncthreads();
+  double dd[1][M];
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const int r = threadIdx.x * M + k;
+    dd[0][k] = r < n ? L.d[pad_idx<M>(r)] : 0.;
+  }
+  const double *b = lu + (size_t)(f * 5) * ntm;
+  const LuTab tb[1] = {{b, b + ntm, b + 2 * ntm, b + 3 * ntm, b + 4 * ntm}};
+  lu_solve<NT, M, 1>(dd, tb, L.scr);
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const int r = threadIdx.x * M + k;
+    if (r < n) L.d[pad_idx<M>(r)] = dd[0][k];
   }
   __syncthreads();
-  const int nf[3] = {g.itot, g.itot - 1, g.itot};
-  double dd[3][M];
-#pragma unroll
-  for (int f = 0; f < 3; f++)
-#pragma unroll
-    for (int k = 0; k < M; k++) {
-      const int r = threadIdx.x * M + k;
-      dd[f][k] = r < nf[f] ? L.d[f][pad_idx<M>(r)] : 0.;
-    }
-  LuTab tb[3];
-#pragma unroll
-  for (int f = 0; f < 3; f++) {
-    const double *b = lu + (size_t)(f * 5) * ntm;
-    tb[f] = {b, b + ntm, b + 2 * ntm, b + 3 * ntm, b + 4 * ntm};
-  }
-  if (!a.with_temp) tb[0].r = nullptr;
-  lu_solve<NT, M, 3>(dd, tb, L.scr);
-#pragma unroll
-  for (int f = 0; f < 3; f++)
-#pragma unroll
-    for (int k = 0; k < M; k++) {
-      const int r = threadIdx.x * M + k;
-      if (r < nf[f]) L.d[f][pad_idx<M>(r)] = dd[f][k];
-    }
-  __syncthreads();
-  for (int k = 0; k < nk; k++) {
-    const int i = 1 + threadIdx.x + k * NT;
-    if (i <= g.itot) {
-      Incr r;
-      r.has_t = a.with_temp;
-      r.has_u = i >= 2;
-      r.has_v = true;
-      r.t = r.has_t ? L.d[0][pad_idx<M>(i - 1)] : 0.;
-      r.u = r.has_u ? L.d[1][pad_idx<M>(i - 2)] : 0.;
-      r.v = L.d[2][pad_idx<M>(i - 1)];
-      pred_emit(a, i, j, r);
-    }
-  }
+  for (int ii = threadIdx.x; ii < n; ii += NT) emit_field(a, f, ii + i0, j, L.d[pad_idx<M>(ii)]);
 }
 
 template <int NT, int M>
 static void pred_row_launch(hipStream_t s, const PredArgs &a, const double *lu, int ntm) {
   const size_t lds = sizeof(PredRowLds<NT, M>);
-  static bool attr = false;
-  if (!attr) {
-    hipFuncSetAttribute((const void *)k_pred_row<NT, M>,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    attr = true;
-  }
-  hipLaunchKernelGGL((k_pred_row<NT, M>), dim3(a.g.jtot), dim3(NT), lds, s, a, lu, ntm);
+  const int nfld = a.with_temp ? 3 : 2;
+  hipLaunchKernelGGL((k_pred_row<NT, M>), dim3(a.g.jtot * nfld), dim3(NT), lds, s, a, lu, ntm,
+                     nfld);
 }
 
 template <int NT, int M>
@@ -345,13 +363,15 @@ static void lu_factor_launch(hipStream_t s, const LuFactorArgs &a) {
   hipLaunchKernelGGL((k_lu_factor<NT, M>), dim3(3, 3), dim3(NT), 0, s, a);
 }
 
-// (NT, M) of the row kernels for a given itot; ntm = NT*M
+// (NT, M) of the row kernels for a given itot; ntm = NT*M >= itot
+static constexpr int kRowM = 8;
 static int row_config(int itot, int &NT, int &M) {
-  M = 4;
-  if (itot <= 64 * 4) NT = 64;
-  else if (itot <= 256 * 4) NT = 256;
-  else if (itot <= 512 * 4) NT = 512;
-  else if (itot <= 1024 * 4) NT = 1024;
+  M = kRowM;
+  if (itot <= 64 * kRowM) NT = 64;
+  else if (itot <= 128 * kRowM) NT = 128;
+  else if (itot <= 256 * kRowM) NT = 256;
+  else if (itot <= 512 * kRowM) NT = 512;
+  else if (itot <= 1024 * kRowM) NT = 1024;
   else return 0;
   return NT * M;
 }
@@ -369,10 +389,13 @@ int lu_table_entries(int itot) {
 void launch_lu_factor(hipStream_t s, const LuFactorArgs &a) {
   int NT, M;
   row_config(a.g.itot, NT, M);
-  if (NT == 64) lu_factor_launch<64, 4>(s, a);
-  else if (NT == 256) lu_factor_launch<256, 4>(s, a);
-  else if (NT == 512) lu_factor_launch<512, 4>(s, a);
-  else lu_factor_launch<1024, 4>(s, a);
+  switch (NT) {
+    case 64: lu_factor_launch<64, kRowM>(s, a); break;
+    case 128: lu_factor_launch<128, kRowM>(s, a); break;
+    case 256: lu_factor_launch<256, kRowM>(s, a); break;
+    case 512: lu_factor_launch<512, kRowM>(s, a); break;
+    default: lu_factor_launch<1024, kRowM>(s, a); break;
+  }
 }
 
 void launch_pred(hipStream_t s, const PredArgs &a, int implicitx, const double *lu, int ntm) {
@@ -383,10 +406,13 @@ void launch_pred(hipStream_t s, const PredArgs &a, int implicitx, const double *
   }
   int NT, M;
   row_config(itot, NT, M);
-  if (NT == 64) pred_row_launch<64, 4>(s, a, lu, ntm);
-  else if (NT == 256) pred_row_launch<256, 4>(s, a, lu, ntm);
-  else if (NT == 512) pred_row_launch<512, 4>(s, a, lu, ntm);
-  else pred_row_launch<1024, 4>(s, a, lu, ntm);
+  switch (NT) {
+    case 64: pred_row_launch<64, kRowM>(s, a, lu, ntm); break;
+    case 128: pred_row_launch<128, kRowM>(s, a, lu, ntm); break;
+    case 256: pred_row_launch<256, kRowM>(s, a, lu, ntm); break;
+    case 512: pred_row_launch<512, kRowM>(s, a, lu, ntm); break;
+    default: pred_row_launch<1024, kRowM>(s, a, lu, ntm); break;
+  }
 }
 
 // ----------------------------------------------------- y-solve completion
@@ -418,59 +444,76 @@ void launch_add_out(hipStream_t s, const Geo &g, int kind, const double *in, con
 
 // ------------------------------------------------- correction + pressure
 
+// rows per workgroup of the correction kernel
+constexpr int kCorrRows = 4;
+
 // src/fluid/correct_velocity.c:6-42 and src/fluid/update_pressure.c:7-60 in
 // one pass: reads ux*, uy*, p and psi, writes ux, uy, p with boundary
 // values.  With red != nullptr it also folds the advective bound of
-// src/param/decide_dt.c:25-45 over the corrected velocities into *red.
+// src/param/decide_dt.c:25-45 over the corrected velocities: each workgroup
+// stores the minimum over its cells in red[workgroup] and k_dt reduces them
+// (min is exact, so the result does not depend on the order).
 __global__ void __launch_bounds__(256) k_corr(CorrArgs a) {
   const Geo &g = a.g;
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int j = xcd_remap(blockIdx.y, gridDim.y) + 1;
+  const int j0 = blockIdx.y * kCorrRows + 1;
   const double dt = *a.dtp;
   const double gdt = a.gamma * dt;
+  const double fac = -gdt / g.two_Re;
   double m = 5.e-2;
   if (i <= g.itot) {
-    const double psi_c = a.psi[c_idx(g, i, j)];
-    if (i >= 2) {
-      const double dxi = a.tb.rdxc[i - 1];  // 1./DXC(i)
-      const double v = a.ux_star[ux_idx(g, i, j)] - gdt * (psi_c - a.psi[c_idx(g, i - 1, j)]) * dxi;
-      put_ux(g, a.ux, i, j, v);
-      if (a.red) m = fmin(m, a.tb.dxc[i - 1] / fmax(fabs(v), 1.e-8));
+    double psi_s = a.psi[c_idx(g, i, j0 - 1)];  // psi(i, j-1)
+    double psi_c = a.psi[c_idx(g, i, j0)];
+#pragma unroll
+    for (int r = 0; r < kCorrRows; r++) {
+      const int j = j0 + r;
+      if (j > g.jtot) break;
+      const double psi_n = a.psi[c_idx(g, i, j + 1)];  // psi(i, j+1)
+      if (i >= 2) {
+        const double dxi = a.tb.rdxc[i - 1];  // 1./DXC(i)
+        const double v = a.ux_star[ux_idx(g, i, j)] - gdt * (psi_c - a.psi[c_idx(g, i - 1, j)]) * dxi;
+        put_ux(g, a.ux, i, j, v);
+        if (a.red) m = fmin(m, a.tb.dxc[i - 1] / fmax(fabs(v), 1.e-8));
+      }
+      {
+        const double v = a.uy_star[c_idx(g, i, j)] - div_cr(gdt * (psi_c - psi_s), a.tb.dy, a.tb.rdy);
+        put_c<0>(g, a.uy, i, j, v);
+        if (a.red) m = fmin(m, a.tb.dy / fmax(fabs(v), 1.e-8));
+      }
+      double pv = a.p[c_idx(g, i, j)] + psi_c;
+      if (a.tr.implicitx) {
+        const double dm = div_cr(-a.psi[c_idx(g, i - 1, j)] + psi_c, a.tb.dxc[i - 1], a.tb.rdxc[i - 1]);
+        const double dp = div_cr(-psi_c + a.psi[c_idx(g, i + 1, j)], a.tb.dxc[i], a.tb.rdxc[i]);
+        pv += fac * (+div_cr(dp - dm, a.tb.dxf[i - 1], a.tb.rdxf[i - 1]));
+      }
+      if (a.tr.implicity) {
+        const double dm = div_cr(-psi_s + psi_c, a.tb.dy, a.tb.rdy);
+        const double dp = div_cr(-psi_c + psi_n, a.tb.dy, a.tb.rdy);
+        pv += fac * (+div_cr(dp - dm, a.tb.dy, a.tb.rdy));
+      }
+      put_c<1>(g, a.p, i, j, pv);
+      psi_s = psi_c;
+      psi_c = psi_n;
     }
-    {
-      const double v = a.uy_star[c_idx(g, i, j)] -
-                       div_cr(gdt * (psi_c - a.psi[c_idx(g, i, j - 1)]), a.tb.dy, a.tb.rdy);
-      put_c<0>(g, a.uy, i, j, v);
-      if (a.red) m = fmin(m, a.tb.dy / fmax(fabs(v), 1.e-8));
-    }
-    double pv = a.p[c_idx(g, i, j)] + psi_c;
-    const double fac = -gdt / g.two_Re;
-    if (a.tr.implicitx) {
-      const double dm = div_cr(-a.psi[c_idx(g, i - 1, j)] + psi_c, a.tb.dxc[i - 1], a.tb.rdxc[i - 1]);
-      const double dp = div_cr(-psi_c + a.psi[c_idx(g, i + 1, j)], a.tb.dxc[i], a.tb.rdxc[i]);
-      pv += fac * (+div_cr(dp - dm, a.tb.dxf[i - 1], a.tb.rdxf[i - 1]));
-    }
-    if (a.tr.implicity) {
-      const double dm = div_cr(-a.psi[c_idx(g, i, j - 1)] + psi_c, a.tb.dy, a.tb.rdy);
-      const double dp = div_cr(-psi_c + a.psi[c_idx(g, i, j + 1)], a.tb.dy, a.tb.rdy);
-      pv += fac * (+div_cr(dp - dm, a.tb.dy, a.tb.rdy));
-    }
-    put_c<1>(g, a.p, i, j, pv);
   }
   if (a.red) {
-    __shared__ double red[256];
-    red[threadIdx.x] = m;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off));
+    __shared__ double wm[4];
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicMin(a.red, (unsigned long long)__double_as_longlong(red[0]));
+    if (threadIdx.x == 0)
+      a.red[blockIdx.y * gridDim.x + blockIdx.x] = fmin(fmin(wm[0], wm[1]), fmin(wm[2], wm[3]));
   }
 }
 
+int corr_partials(const Geo &g) {
+  return ((g.itot + 255) / 256) * ((g.jtot + kCorrRows - 1) / kCorrRows);
+}
+
 void launch_corr(hipStream_t s, const CorrArgs &a) {
-  hipLaunchKernelGGL(k_corr, dim3((a.g.itot + 255) / 256, a.g.jtot), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_corr, dim3((a.g.itot + 255) / 256, (a.g.jtot + kCorrRows - 1) / kCorrRows),
+                     dim3(256), 0, s, a);
 }
 
 // ------------------------------------------------ division self-check
@@ -492,24 +535,37 @@ void launch_check_div(hipStream_t s, int n, const double *a, const double *b, co
 // ------------------------------------------------------------- time step
 
 // src/param/decide_dt.c:45-97 on the device for a fixed treatment: the
-// advective bound times 0.7, the diffusive bounds (fixed per run, computed on
-// the host) and the treatment's dt; the run time advances by dt
-// (src/main.c:34-35).  The reduction slot is reset for the next step.
-__global__ void k_dt(unsigned long long *red, double *dt, double *time, double dt_dif_x,
-                     double dt_dif_y, int implicitx, int implicity) {
-  const double dt_adv = __longlong_as_double((long long)*red) * 0.7;
+// advective bound (minimum of the n partial minima in part) times 0.7, the
+// diffusive bounds (fixed per run, computed on the host) and the treatment's
+// dt; the run time advances by dt (src/main.c:34-35).  One workgroup of
+// kDtThreads threads.
+constexpr int kDtThreads = 1024;
+
+__global__ void __launch_bounds__(kDtThreads) k_dt(const double *part, int n, double *dt,
+                                                   double *time, double dt_dif_x, double dt_dif_y,
+                                                   int implicitx, int implicity) {
+  double m = part[0];
+  for (int k = threadIdx.x; k < n; k += kDtThreads) m = fmin(m, part[k]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off));
+  __shared__ double wm[kDtThreads / 64];
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int w = 1; w < kDtThreads / 64; w++) m = fmin(m, wm[w]);
+  m = fmin(m, wm[0]);
+  const double dt_adv = m * 0.7;
   const double d = implicitx ? implicity ? dt_adv : fmin(dt_adv, dt_dif_y)
                              : implicity ? fmin(dt_adv, dt_dif_x)
                                          : fmin(dt_adv, fmin(dt_dif_x, dt_dif_y));
   *dt = d;
   *time += d;
-  *red = 0x7ff0000000000000ull;
 }
 
-void launch_dt(hipStream_t s, unsigned long long *red, double *dt, double *time, double dt_dif_x,
-               double dt_dif_y, int implicitx, int implicity) {
-  hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, s, red, dt, time, dt_dif_x, dt_dif_y, implicitx,
-                     implicity);
+void launch_dt(hipStream_t s, const double *part, int n, double *dt, double *time,
+               double dt_dif_x, double dt_dif_y, int implicitx, int implicity) {
+  hipLaunchKernelGGL(k_dt, dim3(1), dim3(kDtThreads), 0, s, part, n, dt, time, dt_dif_x, dt_dif_y,
+                     implicitx, implicity);
 }
 
 }  // namespace sns
