@@ -5,9 +5,10 @@ Workload: the large single instance, 4096 x 8192 cells, uniform grid, lx=1,
 ly=2, Ra=1e11, Pr=0.7, FP64, Crank-Nicolson diffusion in x and explicit in y,
 seeded temperature noise of amplitude 1e-3 (u = 0 initially).  One step is
 what the reference's time loop does per iteration (src/main.c:28-35): the
-time-step decision (a device reduction over ux, uy read back to the host)
-and one three-substage RK3 integration.  Inputs are resident in HBM before
-the timed region starts.
+time-step decision (src/param/decide_dt.c, evaluated on the device from the
+advective bound that the previous step's last correction reduced) and one
+three-substage RK3 integration.  Inputs are resident in HBM before the timed
+region starts.
 
 Printed JSON (one line): value = cell-steps/s, plus
   roofline      -- the dominant kernel, timed with HIP events on the
@@ -16,7 +17,9 @@ Printed JSON (one line): value = cell-steps/s, plus
   step_roofline -- the whole step against 600 B/cell-step of compulsory
                    traffic (75 FP64 words per cell and step);
   cpu_baseline  -- the CPU oracle (a C port of the reference algorithm,
-                   one core) timed on a bounded sample of the same workload.
+                   OpenMP over rows and columns, OMP_NUM_THREADS threads)
+                   timed on a bounded sample of the same workload: --cpu-steps
+                   steps of the same grid from the same initial state.
 """
 import argparse
 import json
@@ -64,8 +67,11 @@ def parse():
     ap.add_argument("--itot", type=int, default=WORKLOAD["itot"])
     ap.add_argument("--jtot", type=int, default=WORKLOAD["jtot"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--per-kernel", action="store_true", help="print per-kernel timings to stderr")
+    ap.add_argument("--copy-gib", type=float, default=0.0,
+                    help="instead of the step benchmark: time a plain device copy of this "
+                         "many GiB (the known-size kernel that calibrates traffic counters)")
     return ap.parse_args()
 
 
@@ -109,12 +115,51 @@ def cpu_baseline(itot, jtot, nsteps):
     t = time.perf_counter()
     o.run(nsteps)
     el = time.perf_counter() - t
-    return {"value": itot * jtot * nsteps / el, "unit": "cell-steps/s", "cores": 1,
+    nthr = po.num_threads()
+    return {"value": itot * jtot * nsteps / el, "unit": "cell-steps/s", "cores": nthr,
             "kind": "port",
             "sample": "%d RK3 step(s) (time-step decision + 3 substages) of the same %dx%d "
-                      "workload from the same initial state, CPU oracle (C, one thread, "
-                      "gcc -O2)" % (nsteps, itot, jtot),
+                      "workload from the same initial state, CPU oracle (C, gcc -O2, OpenMP "
+                      "%d threads)" % (nsteps, itot, jtot, nthr),
             "seconds": el}
+
+
+def copy_calibration(sns, gib, reps=5):
+    """Device copy of `gib` GiB, `reps` times after one warm-up: prints the
+    copy's read+write bandwidth (one JSON line)."""
+    nbytes = int(gib * (1 << 30)) // 16 * 16
+    inst = sns.Instance(64, 64)
+    src, dst = inst.malloc(nbytes), inst.malloc(nbytes)
+    inst.copy(dst, src, nbytes)
+    inst.sync()
+    inst.timing_enable("copy")
+    for _ in range(reps):
+        inst.copy(dst, src, nbytes)
+    inst.sync()
+    n, ms = inst.timing_read()
+    inst.timing_disable()
+    inst.close()
+    avg = ms / max(n, 1)
+    print(json.dumps({"metric": "device copy bandwidth", "bytes_read": nbytes,
+                      "bytes_written": nbytes, "launches": n, "avg_launch_ms": avg,
+                      "GB/s": 2 * nbytes / (avg * 1e-3) / 1e9}))
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_4096x8192.json")
+
+
+def measured_traffic(group, cells):
+    """HBM bytes per launch of the kernels of `group` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("cells") != cells:
+        return None
+    rows = [r for r in t["kernels"] if r["group"] == group]
+    return sum(r["traffic_bytes"] for r in rows) if rows else None
 
 
 def main():
@@ -125,6 +170,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import simplenavierstokessolver_amd as sns
+    if a.copy_gib > 0:
+        copy_calibration(sns, a.copy_gib)
+        return
     itot, jtot = a.itot, a.jtot
     cells = itot * jtot
     inst = make_instance(sns, itot, jtot)
@@ -156,7 +204,8 @@ def main():
         ach = bytes_launch / avg_s / 1e9
         res["roofline"] = {"bound": "hbm", "kernel": dominant, "achieved": ach,
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                           "traffic": None, "avg_launch_ms": kms / nlaunch,
+                           "traffic": measured_traffic(dominant, cells),
+                           "avg_launch_ms": kms / nlaunch,
                            "bytes_per_launch": bytes_launch}
     step_gbs = value * B_ALG_STEP / 1e9
     res["step_roofline"] = {"bytes_per_cell_step": B_ALG_STEP, "achieved": step_gbs,

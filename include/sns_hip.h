@@ -77,6 +77,11 @@ int snsg_malloc(snsg_ctx *ctx, size_t bytes, void **dptr);
 int snsg_free(snsg_ctx *ctx, void *dptr);
 int snsg_upload(snsg_ctx *ctx, void *dst, const void *src, size_t bytes);
 int snsg_download(snsg_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* device-to-device copy on the instance's stream by a plain streaming
+ * kernel; bytes and both addresses multiples of 16.  No counterpart in the
+ * reference: state snapshots, and the known-size copy that calibrates
+ * hardware traffic counters. */
+int snsg_copy(snsg_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* ---- boundary values */
 int snsg_fluid_update_boundaries_ux(snsg_ctx *ctx, double *ux);

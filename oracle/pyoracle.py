@@ -291,5 +291,10 @@ def transpose_sim(nranks, g_isize, g_jsize, sendbufs, recvbufs):
     lib().orc_parallel_transpose_sim(nranks, g_isize, g_jsize, es, S, R)
 
 
+def num_threads():
+    """Threads the oracle's OpenMP loops use (OMP_NUM_THREADS)."""
+    return int(lib().orc_num_threads())
+
+
 def splitmix64(x):
     return lib().orc_splitmix64(x)

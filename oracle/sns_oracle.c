@@ -5,7 +5,9 @@
  * Arithmetic follows the reference expression by expression (same operand
  * order, IEEE division where the reference divides, no contraction: the
  * Makefile passes -ffp-contract=off) so that stencil results are bitwise
- * comparable.  Scratch is allocated per call, as in the reference.
+ * comparable.  Scratch is allocated per call, as in the reference.  Loops
+ * over independent rows, columns and wavenumbers run under OpenMP; no
+ * reduction is parallelised, so results do not depend on the thread count.
  *
  * The FFT (FFTW3 in the reference, src/fluid/compute_potential.c:53-67,90,149)
  * is not part of the reference sources; it is restated here from its
@@ -17,6 +19,9 @@
 #include <math.h>
 #include <float.h>
 #include "sns_oracle.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #ifndef M_PI
 #define M_PI 3.1415926535897932
@@ -310,21 +315,42 @@ int orc_dgtsv_p(int n, double *l, double *c, double *u, double *q){
   return 0;
 }
 
+/* threads the OpenMP loops use (1 without OpenMP) */
+int orc_num_threads(void){
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
 /* ------------------------------------------------------------------ FFT */
 
 /* Mixed-radix decimation-in-time complex DFT, recursive on the smallest
- * prime factor.  tw[m] = exp(-2 pi i m / ntop); a sub-transform of length n
- * uses every (ntop/n)-th entry.  sign = -1 forward, +1 backward. */
+ * prime factor (radix-2 levels in butterfly form).  tw[m] = exp(-2 pi i m /
+ * ntop); a sub-transform of length n uses every (ntop/n)-th entry.  sign =
+ * -1 forward, +1 backward.  scr holds 2n entries of scratch. */
 static void cdft_rec(int n, const double complex *x, ptrdiff_t xs, double complex *y,
-                     const double complex *tw, int ntop, int sign){
+                     const double complex *tw, int ntop, int sign, double complex *scr){
   if(n == 1){ y[0] = x[0]; return; }
   int p = 2;
   while(p*p <= n && n % p) p++;
   if(n % p) p = n;
   const int m = n/p;
   const int step = ntop/n;
-  double complex *sub = xcalloc((size_t)n, sizeof(double complex));
-  for(int q=0; q<p; q++) cdft_rec(m, x + q*xs, xs*p, sub + (size_t)q*m, tw, ntop, sign);
+  double complex *sub = scr;
+  for(int q=0; q<p; q++) cdft_rec(m, x + q*xs, xs*p, sub + (size_t)q*m, tw, ntop, sign, scr + n);
+  if(p == 2){
+    for(int k=0; k<m; k++){
+      double complex w = tw[(size_t)k*step];
+      if(sign > 0) w = conj(w);
+      const double complex a = sub[k];
+      const double complex b = w*sub[m+k];
+      y[k]   = a+b;
+      y[k+m] = a-b;
+    }
+    return;
+  }
   for(int k=0; k<n; k++){
     double complex acc = 0.;
     for(int q=0; q<p; q++){
@@ -335,7 +361,6 @@ static void cdft_rec(int n, const double complex *x, ptrdiff_t xs, double comple
     }
     y[k] = acc;
   }
-  free(sub);
 }
 
 static double complex *twiddles(int n){
@@ -347,17 +372,35 @@ static double complex *twiddles(int n){
   return tw;
 }
 
-/* r2c: out[J] = sum_j in[j] exp(-2 pi i jJ/n), J = 0..n/2 */
-int orc_rfft(int n, const double *in, double complex *out){
-  double complex *x = xcalloc((size_t)n, sizeof(double complex));
-  double complex *y = xcalloc((size_t)n, sizeof(double complex));
-  double complex *tw = twiddles(n);
+/* r2c with precomputed twiddles; work holds 4n entries */
+static void rfft_tw(int n, const double *in, double complex *out, const double complex *tw,
+                    double complex *work){
+  double complex *x = work, *y = work + n;
   for(int j=0; j<n; j++) x[j] = mkz(in[j], 0.);
-  cdft_rec(n, x, 1, y, tw, n, -1);
+  cdft_rec(n, x, 1, y, tw, n, -1, work + 2*(size_t)n);
   for(int k=0; k<=n/2; k++) out[k] = y[k];
   out[0] = mkz(creal(out[0]), 0.);
   if(n % 2 == 0) out[n/2] = mkz(creal(out[n/2]), 0.);
-  free(x); free(y); free(tw);
+}
+
+/* c2r with precomputed twiddles; work holds 4n entries */
+static void irfft_tw(int n, const double complex *in, double *out, const double complex *tw,
+                     double complex *work){
+  double complex *x = work, *y = work + n;
+  for(int k=0; k<=n/2; k++) x[k] = in[k];
+  for(int k=n/2+1; k<n; k++) x[k] = conj(in[n-k]);
+  x[0] = mkz(creal(x[0]), 0.);
+  if(n % 2 == 0) x[n/2] = mkz(creal(x[n/2]), 0.);
+  cdft_rec(n, x, 1, y, tw, n, +1, work + 2*(size_t)n);
+  for(int j=0; j<n; j++) out[j] = creal(y[j]);
+}
+
+/* r2c: out[J] = sum_j in[j] exp(-2 pi i jJ/n), J = 0..n/2 */
+int orc_rfft(int n, const double *in, double complex *out){
+  double complex *tw = twiddles(n);
+  double complex *work = xcalloc(4*(size_t)n, sizeof(double complex));
+  rfft_tw(n, in, out, tw, work);
+  free(work); free(tw);
   return 0;
 }
 
@@ -365,16 +408,10 @@ int orc_rfft(int n, const double *in, double complex *out){
  * extension Y_{n-J} = conj(Y_J); unnormalised.  Imaginary parts of the DC
  * (and even-n Nyquist) entries do not contribute, as with FFTW's c2r. */
 int orc_irfft(int n, const double complex *in, double *out){
-  double complex *x = xcalloc((size_t)n, sizeof(double complex));
-  double complex *y = xcalloc((size_t)n, sizeof(double complex));
   double complex *tw = twiddles(n);
-  for(int k=0; k<=n/2; k++) x[k] = in[k];
-  for(int k=n/2+1; k<n; k++) x[k] = conj(in[n-k]);
-  x[0] = mkz(creal(x[0]), 0.);
-  if(n % 2 == 0) x[n/2] = mkz(creal(x[n/2]), 0.);
-  cdft_rec(n, x, 1, y, tw, n, +1);
-  for(int j=0; j<n; j++) out[j] = creal(y[j]);
-  free(x); free(y); free(tw);
+  double complex *work = xcalloc(4*(size_t)n, sizeof(double complex));
+  irfft_tw(n, in, out, tw, work);
+  free(work); free(tw);
   return 0;
 }
 
@@ -422,12 +459,14 @@ int orc_parallel_transpose_sim(int nranks, int g_isize, int g_jsize, size_t elem
 
 /* single-rank transpose: recv[i*g_jsize + j] = send[j*g_isize + i] */
 static void transpose_d(int g_isize, int g_jsize, const double *send, double *recv){
+#pragma omp parallel for
   for(int i=0; i<g_isize; i++)
     for(int j=0; j<g_jsize; j++)
       recv[(size_t)i*g_jsize+j] = send[(size_t)j*g_isize+i];
 }
 
 static void transpose_z(int g_isize, int g_jsize, const double complex *send, double complex *recv){
+#pragma omp parallel for
   for(int i=0; i<g_isize; i++)
     for(int j=0; j<g_jsize; j++)
       recv[(size_t)i*g_jsize+j] = send[(size_t)j*g_isize+i];
@@ -473,6 +512,7 @@ static void temp_compute_src(const orc_param_t *param, orc_state_t *s){
   double *srctempb = s->srctempb;
   double *srctempg = s->srctempg;
   memcpy(srctempb, srctempa, sizeof(double)*itot*jtot);
+#pragma omp parallel for
   for(int j=1; j<=jtot; j++){
     for(int i=1; i<=itot; i++){
       double dtdx_xm = (-TEMP(i-1, j  )+TEMP(i  , j  ))/DXC(i  );
@@ -514,8 +554,11 @@ static void temp_compute_src(const orc_param_t *param, orc_state_t *s){
 static void solve_y_periodic(int ni, int jtot, double prefactor, double dy, double *qx){
   double *qy = xcalloc((size_t)ni*jtot, sizeof(double));
   transpose_d(ni, jtot, qx, qy);
+#pragma omp parallel
+  {
   double *l = xcalloc(jtot, sizeof(double)), *c = xcalloc(jtot, sizeof(double));
   double *u = xcalloc(jtot, sizeof(double)), *q = xcalloc(jtot, sizeof(double));
+#pragma omp for
   for(int i=0; i<ni; i++){
     for(int j=0; j<jtot; j++){
       double coef_ym = 1./dy/dy;
@@ -529,8 +572,10 @@ static void solve_y_periodic(int ni, int jtot, double prefactor, double dy, doub
     orc_dgtsv_p(jtot, l, c, u, q);
     for(int j=0; j<jtot; j++) qy[(size_t)i*jtot+j] = q[j];
   }
+  free(l); free(c); free(u); free(q);
+  }
   transpose_d(jtot, ni, qy, qx);
-  free(l); free(c); free(u); free(q); free(qy);
+  free(qy);
 }
 
 /* src/temperature/update.c:89-231 */
@@ -550,6 +595,7 @@ static void temp_update(const orc_param_t *param, int rkstep, orc_state_t *s){
   double *temp = s->temp;
   double *qx = xcalloc((size_t)itot*jtot, sizeof(double));
 #define QX(I, J) (qx[((J)-1)*(itot)+((I)-1)])
+#pragma omp parallel for
   for(int j=1; j<=jtot; j++){
     for(int i=1; i<=itot; i++){
       QX(i, j) =
@@ -559,8 +605,11 @@ static void temp_update(const orc_param_t *param, int rkstep, orc_state_t *s){
     }
   }
   if(param->implicitx){
+#pragma omp parallel
+    {
     double *l = xcalloc(itot, sizeof(double)), *c = xcalloc(itot, sizeof(double));
     double *u = xcalloc(itot, sizeof(double)), *q = xcalloc(itot, sizeof(double));
+#pragma omp for
     for(int j=1; j<=jtot; j++){
       for(int i=1; i<=itot; i++){
         double prefactor = (gamma*dt)/(2.*sqrt(Pr)*sqrt(Ra));
@@ -576,11 +625,13 @@ static void temp_update(const orc_param_t *param, int rkstep, orc_state_t *s){
       for(int i=1; i<=itot; i++) QX(i, j) = q[i-1];
     }
     free(l); free(c); free(u); free(q);
+    }
   }
   if(param->implicity){
     double prefactor = (gamma*dt)/(2.*sqrt(Pr)*sqrt(Ra));
     solve_y_periodic(itot, jtot, prefactor, dy, qx);
   }
+#pragma omp parallel for
   for(int j=1; j<=jtot; j++)
     for(int i=1; i<=itot; i++)
       TEMP(i, j) += QX(i, j);
@@ -615,6 +666,7 @@ static void compute_src_ux(const orc_param_t *param, orc_state_t *s){
   double *srcuxb = s->srcuxb;
   double *srcuxg = s->srcuxg;
   memcpy(srcuxb, srcuxa, sizeof(double)*(itot-1)*jtot);
+#pragma omp parallel for
   for(int j=1; j<=jtot; j++){
     for(int i=2; i<=itot; i++){
       double duxdx_xm = (-UX(i-1, j  )+UX(i  , j  ))/DXF(i-1);
@@ -677,6 +729,7 @@ static void compute_src_uy(const orc_param_t *param, orc_state_t *s){
   double *srcuyb = s->srcuyb;
   double *srcuyg = s->srcuyg;
   memcpy(srcuyb, srcuya, sizeof(double)*itot*jtot);
+#pragma omp parallel for
   for(int j=1; j<=jtot; j++){
     for(int i=1; i<=itot; i++){
       double duydx_xm = (-UY(i-1, j  )+UY(i  , j  ))/DXC(i  );
@@ -736,6 +789,7 @@ static void update_ux(const orc_param_t *param, int rkstep, orc_state_t *s){
   double *ux = s->ux;
   double *qx = xcalloc((size_t)(itot+1)*jtot, sizeof(double));
 #define QX(I, J) (qx[((J)-1)*(itot+1)+((I)-1)])
+#pragma omp parallel for
   for(int j=1; j<=jtot; j++){
     for(int i=1; i<=itot+1; i++){
       QX(i, j) =
@@ -748,8 +802,11 @@ static void update_ux(const orc_param_t *param, int rkstep, orc_state_t *s){
   }
   if(param->implicitx){
     const int n = itot+1;
+#pragma omp parallel
+    {
     double *l = xcalloc(n, sizeof(double)), *c = xcalloc(n, sizeof(double));
     double *u = xcalloc(n, sizeof(double)), *q = xcalloc(n, sizeof(double));
+#pragma omp for
     for(int j=1; j<=jtot; j++){
       for(int i=1; i<=itot+1; i++){
         if(i == 1 || i == itot+1){
@@ -771,11 +828,13 @@ static void update_ux(const orc_param_t *param, int rkstep, orc_state_t *s){
       for(int i=1; i<=itot+1; i++) QX(i, j) = q[i-1];
     }
     free(l); free(c); free(u); free(q);
+    }
   }
   if(param->implicity){
     double prefactor = (gamma*dt*sqrt(Pr))/(2.*sqrt(Ra));
     solve_y_periodic(itot+1, jtot, prefactor, dy, qx);
   }
+#pragma omp parallel for
   for(int j=1; j<=jtot; j++)
     for(int i=2; i<=itot; i++)
       UX(i, j) += QX(i, j);
@@ -801,6 +860,7 @@ static void update_uy(const orc_param_t *param, int rkstep, orc_state_t *s){
   double *uy = s->uy;
   double *qx = xcalloc((size_t)itot*jtot, sizeof(double));
 #define QX(I, J) (qx[((J)-1)*(itot)+((I)-1)])
+#pragma omp parallel for
   for(int j=1; j<=jtot; j++){
     for(int i=1; i<=itot; i++){
       QX(i, j) =
@@ -810,8 +870,11 @@ static void update_uy(const orc_param_t *param, int rkstep, orc_state_t *s){
     }
   }
   if(param->implicitx){
+#pragma omp parallel
+    {
     double *l = xcalloc(itot, sizeof(double)), *c = xcalloc(itot, sizeof(double));
     double *u = xcalloc(itot, sizeof(double)), *q = xcalloc(itot, sizeof(double));
+#pragma omp for
     for(int j=1; j<=jtot; j++){
       for(int i=1; i<=itot; i++){
         double prefactor = (gamma*dt*sqrt(Pr))/(2.*sqrt(Ra));
@@ -827,11 +890,13 @@ static void update_uy(const orc_param_t *param, int rkstep, orc_state_t *s){
       for(int i=1; i<=itot; i++) QX(i, j) = q[i-1];
     }
     free(l); free(c); free(u); free(q);
+    }
   }
   if(param->implicity){
     double prefactor = (gamma*dt*sqrt(Pr))/(2.*sqrt(Ra));
     solve_y_periodic(itot, jtot, prefactor, dy, qx);
   }
+#pragma omp parallel for
   for(int j=1; j<=jtot; j++)
     for(int i=1; i<=itot; i++)
       UY(i, j) += QX(i, j);
@@ -866,6 +931,7 @@ int orc_fluid_compute_potential(const orc_param_t *param, int rkstep, orc_state_
   const double dt = param->dt;
   const double *ux = s->ux;
   const double *uy = s->uy;
+#pragma omp parallel for
   for(int j=1; j<=jtot; j++){
     for(int i=1; i<=itot; i++){
       double ux_xm = UX(i  , j  );
@@ -880,12 +946,21 @@ int orc_fluid_compute_potential(const orc_param_t *param, int rkstep, orc_state_
     }
   }
   transpose_d(itot, jtot, qrx, qry);
-  for(int i=0; i<itot; i++) orc_rfft(jtot, qry + (size_t)i*jtot, qcy + (size_t)i*nwave);
+  double complex *tw = twiddles(jtot);
+#pragma omp parallel
+  {
+    double complex *work = xcalloc(4*(size_t)jtot, sizeof(double complex));
+#pragma omp for
+    for(int i=0; i<itot; i++) rfft_tw(jtot, qry + (size_t)i*jtot, qcy + (size_t)i*nwave, tw, work);
+    free(work);
+  }
   transpose_z(nwave, itot, qcy, qcx);
+#pragma omp parallel
   {
     double *l = xcalloc(itot, sizeof(double)), *c = xcalloc(itot, sizeof(double));
     double *u = xcalloc(itot, sizeof(double));
     double complex *q = xcalloc(itot, sizeof(double complex));
+#pragma omp for
     for(int j=1; j<=nwave; j++){
       double eigenvalue = -4./pow(dy, 2.)*pow(sin(M_PI*(j-1)/(1.*jtot)), 2.);
       for(int i=1; i<=itot; i++){
@@ -902,8 +977,16 @@ int orc_fluid_compute_potential(const orc_param_t *param, int rkstep, orc_state_
     free(l); free(c); free(u); free(q);
   }
   transpose_z(itot, nwave, qcx, qcy);
-  for(int i=0; i<itot; i++) orc_irfft(jtot, qcy + (size_t)i*nwave, qry + (size_t)i*jtot);
+#pragma omp parallel
+  {
+    double complex *work = xcalloc(4*(size_t)jtot, sizeof(double complex));
+#pragma omp for
+    for(int i=0; i<itot; i++) irfft_tw(jtot, qcy + (size_t)i*nwave, qry + (size_t)i*jtot, tw, work);
+    free(work);
+  }
+  free(tw);
   transpose_d(jtot, itot, qry, qrx);
+#pragma omp parallel for
   for(int j=1; j<=jtot; j++)
     for(int i=1; i<=itot; i++)
       PSI(i, j) = QRX(i, j)/(1.*jtot);
@@ -924,6 +1007,7 @@ int orc_fluid_correct_velocity(const orc_param_t *param, int rkstep, orc_state_t
   const double *psi = s->psi;
   double *ux = s->ux;
   double *uy = s->uy;
+#pragma omp parallel for
   for(int j=1; j<=jtot; j++){
     for(int i=2; i<=itot; i++){
       double dxi = 1./DXC(i);
@@ -933,6 +1017,7 @@ int orc_fluid_correct_velocity(const orc_param_t *param, int rkstep, orc_state_t
     }
   }
   orc_fluid_update_boundaries_ux(param, ux);
+#pragma omp parallel for
   for(int j=1; j<=jtot; j++){
     for(int i=1; i<=itot; i++){
       double psi_yp = PSI(i, j  );
@@ -956,6 +1041,7 @@ int orc_fluid_update_pressure(const orc_param_t *param, int rkstep, orc_state_t 
   const double dt = param->dt;
   const double *psi = s->psi;
   double *p = s->p;
+#pragma omp parallel for
   for(int j=1; j<=jtot; j++)
     for(int i=1; i<=itot; i++)
       P(i, j) += PSI(i, j);

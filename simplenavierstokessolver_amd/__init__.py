@@ -63,6 +63,7 @@ def _load_hip():
         L.snsg_synchronize.argtypes = [_vp]
         L.snsg_device_count.argtypes = [C.POINTER(C.c_int)]
         L.snsg_malloc.argtypes = [_vp, C.c_size_t, C.POINTER(_vp)]
+        L.snsg_copy.argtypes = [_vp, _vp, _vp, C.c_size_t]
         L.snsg_free.argtypes = [_vp, _vp]
         L.snsg_upload.argtypes = [_vp, _vp, _vp, C.c_size_t]
         L.snsg_download.argtypes = [_vp, _vp, _vp, C.c_size_t]
@@ -245,6 +246,21 @@ class Instance:
 
     def sync(self):
         _check(self.L.snsg_synchronize(self.ctx), "snsg_synchronize")
+
+    def malloc(self, nbytes):
+        """Raw zero-initialised device buffer (freed by free_raw or close)."""
+        ptr = _vp()
+        _check(self.L.snsg_malloc(self.ctx, int(nbytes), C.byref(ptr)), "snsg_malloc")
+        self.buf["_raw%d" % ptr.value] = ptr
+        return ptr
+
+    def free_raw(self, ptr):
+        self.buf.pop("_raw%d" % ptr.value, None)
+        _check(self.L.snsg_free(self.ctx, ptr), "snsg_free")
+
+    def copy(self, dst, src, nbytes):
+        """Device-to-device copy on the instance's stream (snsg_copy)."""
+        _check(self.L.snsg_copy(self.ctx, dst, src, int(nbytes)), "snsg_copy")
 
     # ---- parameters
     def set_dt(self, dt):

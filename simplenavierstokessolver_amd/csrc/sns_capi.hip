@@ -1,6 +1,7 @@
 // sns_capi.hip -- instance context and the extern "C" entry points of
 // libsns_hip.so (declared in include/sns_hip.h).
 #include <math.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <string.h>
 #include <string>
@@ -306,6 +307,13 @@ extern "C" int snsg_download(snsg_ctx *c, void *dst, const void *src, size_t byt
   HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   return 0;
+}
+
+extern "C" int snsg_copy(snsg_ctx *c, void *dst, const void *src, size_t bytes) {
+  if (!c || !dst || !src) return set_err(SNSG_E_ARG, "snsg_copy: null argument");
+  if (bytes % 16 || (uintptr_t)dst % 16 || (uintptr_t)src % 16)
+    return set_err(SNSG_E_ARG, "snsg_copy: size and addresses must be multiples of 16 bytes");
+  return run(c, "copy", [&] { launch_copy(c->stream, src, dst, bytes / 16); });
 }
 
 // ------------------------------------------------------ boundary values

@@ -93,5 +93,6 @@ void launch_pressure(hipStream_t s, const Geo &g, const Treat &tr, double gamma,
 void launch_dt_reduce(hipStream_t s, const Geo &g, const double *ux, const double *uy,
                       unsigned long long *out);
 void launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v, int n);
+void launch_copy(hipStream_t s, const void *src, void *dst, size_t n16);
 
 }  // namespace sns

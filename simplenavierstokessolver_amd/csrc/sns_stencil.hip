@@ -379,6 +379,23 @@ void launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v,
   hipLaunchKernelGGL(k_fill_u64, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p, v, n);
 }
 
+// Plain device copy of n 16-byte words (grid-stride, one load and one store
+// per word).
+__global__ void __launch_bounds__(kBlock) k_copy(const double2 *__restrict__ src,
+                                                 double2 *__restrict__ dst, size_t n) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    dst[k] = src[k];
+}
+
+void launch_copy(hipStream_t s, const void *src, void *dst, size_t n16) {
+  size_t blocks = (n16 + kBlock - 1) / kBlock;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(kBlock), 0, s,
+                     (const double2 *)src, (double2 *)dst, n16);
+}
+
 void launch_dt_reduce(hipStream_t s, const Geo &g, const double *ux, const double *uy,
                       unsigned long long *out) {
   const long long total = (long long)g.itot * g.jtot;

@@ -1,0 +1,149 @@
+"""Summaries of rocprofv3 output (kernel statistics, HBM traffic counters).
+
+  python -m simplenavierstokessolver_amd.profiling stats KERNEL_STATS_CSV [--out FILE]
+        [--command TEXT]
+  python -m simplenavierstokessolver_amd.profiling traffic --fetch CSV [CSV ...]
+        --write CSV [CSV ...] --copy-bytes N --cells N [--out FILE] [--json FILE]
+        [--command TEXT ...]
+
+`stats` condenses rocprofv3's kernel_stats.csv into a table of kernel groups
+(the names bench.py and the C-ABI timing use).  `traffic` averages the
+FETCH_SIZE and WRITE_SIZE counters (KiB per dispatch) of every kernel over
+the given counter_collection.csv files and scales each by the factor that
+makes the plain copy kernel (k_copy, which reads and writes --copy-bytes
+bytes) read exactly its size; the scaled bytes per dispatch are compared
+with the kernel's algorithmic bytes (bench.KERNEL_BYTES_PER_CELL x cells).
+"""
+import argparse
+import collections
+import csv
+import json
+import re
+import sys
+
+# kernel function -> group
+GROUPS = [
+    ("k_pred_row", "pred"), ("k_pred_exp", "pred"),
+    ("k_pfwd_a", "poisson_fwd"), ("k_pfwd_b", "poisson_fwd"),
+    ("k_poisson_x2", "poisson_x"),
+    ("k_pinv_b", "poisson_inv"), ("k_pinv_a", "poisson_inv"),
+    ("k_corr", "corr"), ("k_dt_reduce", "dt_reduce"), ("k_fill_u64", "dt_reduce"),
+    ("k_dt", "dt"), ("k_lu_factor", "lu_factor"), ("k_copy", "copy"),
+    ("k_transpose", "ysolve"), ("k_cny_rows", "ysolve"), ("k_add_out", "ysolve"),
+]
+
+
+def short_name(full):
+    m = re.search(r"sns::(\w+)", full)
+    return m.group(1) if m else full.split("(")[0]
+
+
+def group_of(full):
+    s = short_name(full)
+    for k, g in GROUPS:
+        if s == k:
+            return g
+    return s
+
+
+def stats(path):
+    """[(kernel, group, calls, avg_us, total_ms, pct)] from kernel_stats.csv."""
+    out = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            out.append((short_name(r["Name"]), group_of(r["Name"]), int(r["Calls"]),
+                        float(r["AverageNs"]) / 1e3, float(r["TotalDurationNs"]) / 1e6,
+                        float(r["Percentage"])))
+    out.sort(key=lambda t: -t[4])
+    return out
+
+
+def counter_avgs(paths, counter):
+    """kernel short name -> (dispatches, mean counter value per dispatch)."""
+    acc = collections.defaultdict(list)
+    for p in paths:
+        with open(p) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] != counter:
+                    continue
+                acc[short_name(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+    return {k: (len(v), sum(v) / len(v)) for k, v in acc.items()}
+
+
+def traffic(fetch_paths, write_paths, copy_bytes, cells):
+    fe = counter_avgs(fetch_paths, "FETCH_SIZE")
+    wr = counter_avgs(write_paths, "WRITE_SIZE")
+    if "k_copy" not in fe or "k_copy" not in wr:
+        raise SystemExit("traffic: the counter files hold no k_copy dispatch to calibrate with")
+    f_fac = copy_bytes / (fe["k_copy"][1] * 1024.0)
+    w_fac = copy_bytes / (wr["k_copy"][1] * 1024.0)
+    sys.path.insert(0, __file__.rsplit("/", 2)[0])
+    from bench import KERNEL_BYTES_PER_CELL  # algorithmic bytes per cell and launch
+    rows = []
+    for k in sorted(set(fe) | set(wr)):
+        if k == "k_copy":
+            continue
+        g = group_of("sns::" + k + "(")
+        fb = fe.get(k, (0, 0.0))[1] * 1024.0 * f_fac
+        wb = wr.get(k, (0, 0.0))[1] * 1024.0 * w_fac
+        alg = KERNEL_BYTES_PER_CELL.get(g)
+        rows.append({"kernel": k, "group": g, "dispatches": fe.get(k, (0, 0))[0],
+                     "fetch_bytes": fb, "write_bytes": wb, "traffic_bytes": fb + wb,
+                     "group_algorithmic_bytes": None if alg is None else alg * cells})
+    return {"fetch_factor": f_fac, "write_factor": w_fac, "copy_bytes": copy_bytes,
+            "copy_fetch_kib": fe["k_copy"][1], "copy_write_kib": wr["k_copy"][1],
+            "cells": cells, "kernels": rows}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s1 = sub.add_parser("stats")
+    s1.add_argument("csv")
+    s1.add_argument("--out")
+    s1.add_argument("--command", action="append", default=[])
+    s2 = sub.add_parser("traffic")
+    s2.add_argument("--fetch", nargs="+", required=True)
+    s2.add_argument("--write", nargs="+", required=True)
+    s2.add_argument("--copy-bytes", type=int, required=True)
+    s2.add_argument("--cells", type=int, required=True)
+    s2.add_argument("--out")
+    s2.add_argument("--json")
+    s2.add_argument("--command", action="append", default=[])
+    a = ap.parse_args(argv)
+    lines = []
+    if a.command:
+        lines += ["Commands:", ""] + ["    " + c for c in a.command] + [""]
+    if a.cmd == "stats":
+        rows = stats(a.csv)
+        lines += ["| kernel | group | calls | avg (us) | total (ms) | % |",
+                  "|---|---|---:|---:|---:|---:|"]
+        for k, g, n, avg, tot, pct in rows:
+            lines.append("| %s | %s | %d | %.1f | %.3f | %.2f |" % (k, g, n, avg, tot, pct))
+    else:
+        t = traffic(a.fetch, a.write, a.copy_bytes, a.cells)
+        lines += ["Calibration: k_copy of %d bytes read %.0f KiB (FETCH_SIZE) and wrote %.0f KiB "
+                  "(WRITE_SIZE) per dispatch; factors %.4f (fetch), %.4f (write)."
+                  % (t["copy_bytes"], t["copy_fetch_kib"], t["copy_write_kib"],
+                     t["fetch_factor"], t["write_factor"]), "",
+                  "| kernel | group | dispatches | fetch (MB) | write (MB) | traffic (MB) |"
+                  " group algorithmic (MB) |", "|---|---|---:|---:|---:|---:|---:|"]
+        for r in t["kernels"]:
+            alg = r["group_algorithmic_bytes"]
+            lines.append("| %s | %s | %d | %.1f | %.1f | %.1f | %s |" % (
+                r["kernel"], r["group"], r["dispatches"], r["fetch_bytes"] / 1e6,
+                r["write_bytes"] / 1e6, r["traffic_bytes"] / 1e6,
+                "-" if alg is None else "%.1f" % (alg / 1e6)))
+        if a.json:
+            with open(a.json, "w") as f:
+                json.dump(t, f, indent=1)
+    text = "\n".join(lines) + "\n"
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(text)
+    else:
+        sys.stdout.write(text)
+
+
+if __name__ == "__main__":
+    main()

@@ -348,3 +348,20 @@ def test_advance_matches_oracle_time_loop(sns, case):
     for n in ["ux", "uy", "temp"]:
         assert_close(interior(inst.state_download(n)), interior(o.f[n]), 1e-12, n)
     inst.close()
+
+
+def test_device_copy(sns):
+    """snsg_copy: a bitwise device-to-device copy; sizes or addresses that
+    are not multiples of 16 bytes are rejected."""
+    inst = sns.Instance(16, 16)
+    n = 1 << 20
+    src, dst = inst.malloc(8 * n), inst.malloc(8 * n)
+    x = np.random.default_rng(5).standard_normal(n)
+    assert inst.L.snsg_upload(inst.ctx, src, x.ctypes.data, x.nbytes) == 0
+    inst.copy(dst, src, 8 * n)
+    y = np.zeros(n)
+    assert inst.L.snsg_download(inst.ctx, y.ctypes.data, dst, y.nbytes) == 0
+    assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
+    with pytest.raises(sns.GpuError):
+        inst.copy(dst, src, 8 * n - 8)
+    inst.close()

@@ -386,3 +386,96 @@ def test_splitmix64_known_values():
     # Flood 2014): state increments by the golden gamma before mixing
     assert po.splitmix64(0) == 0xE220A8397B1DCDAF
     assert po.splitmix64(0x9E3779B97F4A7C15) == 0x6E789E6AA1B965F4
+
+
+# --------------------------------------- boundaries, time step, pressure
+
+def test_boundaries_match_padding():
+    """Boundary values (src/fluid/boundary_conditions.c, src/temperature/
+    boundary_conditions.c) against numpy padding: periodic rows in y, then
+    impermeable / no-slip walls (ux, uy), Neumann walls (p) and fixed wall
+    temperatures +-0.5 on the interior rows (T)."""
+    it, jt = 12, 10
+    o = po.Oracle(it, jt, stretch=3.0)
+    rng = np.random.default_rng(11)
+    for name in ("ux", "uy", "p", "temp"):
+        a = rng.standard_normal(o.f[name].shape)
+        before = a.copy()
+        o.bc(name, a)
+        if name == "ux":
+            inner = before[1:jt + 1, 1:it]                      # faces i = 2..itot
+            e = np.pad(np.pad(inner, ((1, 1), (0, 0)), mode="wrap"), ((0, 0), (1, 1)))
+        elif name == "uy":
+            inner = before[1:jt + 1, 1:it + 1]
+            e = np.pad(np.pad(inner, ((1, 1), (0, 0)), mode="wrap"), ((0, 0), (1, 1)))
+        elif name == "p":
+            inner = before[1:jt + 1, 1:it + 1]
+            e = np.pad(np.pad(inner, ((1, 1), (0, 0)), mode="wrap"), ((0, 0), (1, 1)),
+                       mode="edge")
+        else:
+            inner = before[1:jt + 1, 1:it + 1]
+            e = np.pad(inner, ((1, 1), (0, 0)), mode="wrap")
+            e = np.concatenate([before[:, :1], e, before[:, -1:]], axis=1)
+            e[1:jt + 1, 0] = 0.5
+            e[1:jt + 1, -1] = -0.5
+        assert np.array_equal(a, e), name
+
+
+def test_decide_dt_special_cases():
+    """Time-step limits (src/param/decide_dt.c:11-101) where the minimum is
+    known by hand: at rest the advective bound is 0.7*0.05; one fast interior
+    face sets dt_adv = 0.7*dx/|u| while a faster value on the wall face (not
+    part of the bound) is ignored; the diffusive bounds are 0.9*0.25*dx^2/
+    max(kappa_u, kappa_T); the forced treatment selects which bounds apply."""
+    it, jt = 32, 64                      # uniform: dx = 1/32, dy = 2/64
+    dx = 1.0 / it
+    o = po.Oracle(it, jt, stretch=100.0, Ra=1.0, Pr=4.0)   # kappa_u = 2, kappa_T = 0.5
+    o.set_treatment(1, 1)
+    assert o.decide_dt() == pytest.approx(0.7 * 0.05, rel=1e-15)
+    o.f["ux"][5, 9] = -3.0               # face i = 10, row j = 5
+    o.f["ux"][7, 0] = 1e3                # wall face i = 1
+    o.f["uy"][3, 4] = 2.0                # dy / 2 = dx / 2 > dx / 3
+    dt_adv = 0.7 * dx / 3.0
+    assert o.decide_dt() == pytest.approx(dt_adv, rel=1e-15)
+    assert (o.param.implicitx, o.param.implicity) == (1, 1)
+    dt_dif = 0.9 * 0.25 * dx * dx / 2.0
+    for imx, imy in ((0, 0), (1, 0), (0, 1)):
+        o.set_treatment(imx, imy)
+        assert o.decide_dt() == pytest.approx(dt_dif, rel=1e-15)
+        assert (o.param.implicitx, o.param.implicity) == (imx, imy)
+    o.f["uy"][3, 4] = -7.0               # now the y-velocity limits
+    o.set_treatment(1, 1)
+    assert o.decide_dt() == pytest.approx(0.7 * (2.0 / jt) / 7.0, rel=1e-15)
+
+
+@pytest.mark.parametrize("imx,imy", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_update_pressure_closed_forms(imx, imy):
+    """p += psi - gamma dt/(2 Re) Lap(psi) over the implicit directions
+    (src/fluid/update_pressure.c:7-60): on a uniform grid the x-Laplacian of
+    x^2 is exactly 2 away from the walls, and the y-Laplacian of a Fourier
+    mode is its discrete eigenvalue times the mode."""
+    it, jt, k = 24, 32, 2
+    Ra, Pr, dt, rk = 1e4, 2.0, 0.01, 1
+    o = po.Oracle(it, jt, stretch=100.0, Ra=Ra, Pr=Pr)
+    o.set_treatment(imx, imy)
+    o.dt = dt
+    gamma = o.rk()[2][rk]
+    Re = np.sqrt(Ra) / np.sqrt(Pr)
+    xc = o.xc[None, 1:-1]
+    ys = np.sin(2 * np.pi * k * o.yc / o.param.ly)[:, None]
+    lam = -4.0 / o.dy ** 2 * np.sin(np.pi * k / jt) ** 2
+    p0 = np.random.default_rng(2).standard_normal((jt, it))
+    for shape, lap_x, lap_y in ((xc ** 2 + 0 * ys, 2.0, 0.0), (ys + 0 * xc, 0.0, lam * ys)):
+        o.f["p"][...] = 0.0
+        o.f["p"][1:-1, 1:-1] = p0
+        o.f["psi"][...] = 0.0
+        o.f["psi"][1:-1, 1:-1] = shape
+        o.bc("p", o.f["psi"])
+        o.bc("p", o.f["p"])
+        o.update_pressure(rk)
+        lap = imx * lap_x + imy * lap_y
+        expect = p0 + shape - gamma * dt / (2 * Re) * lap
+        got = o.f["p"][1:-1, 1:-1]
+        cols = slice(1, it - 1)          # Neumann ghosts change the x stencil at the walls
+        assert np.allclose(got[:, cols], (expect * np.ones((jt, it)))[:, cols], rtol=0,
+                           atol=1e-10), (imx, imy)
