@@ -110,47 +110,6 @@ struct Dft<5, SIGN> {
   }
 };
 
-template <int NT, int EMAX, int R, int SIGN>
-__device__ __forceinline__ void fft_pass(double2 *buf, int S, int n, int Ns,
-                                         const double2 *__restrict__ tw) {
-  constexpr int BMAX = (EMAX + R - 1) / R;
-  const int nb = n / R;
-  const int total = S * nb;
-  const int tstep = n / (Ns * R);
-  double2 x[BMAX][R];
-#pragma unroll
-  for (int k = 0; k < BMAX; k++) {
-    const int bidx = threadIdx.x + k * NT;
-    if (bidx < total) {
-      const int s = bidx / nb, j = bidx - s * nb;
-#pragma unroll
-      for (int r = 0; r < R; r++) x[k][r] = buf[s * n + j + r * nb];
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < BMAX; k++) {
-    const int bidx = threadIdx.x + k * NT;
-    if (bidx < total) {
-      const int s = bidx / nb, j = bidx - s * nb;
-      const int kk = j % Ns;
-      if (kk != 0) {
-#pragma unroll
-        for (int r = 1; r < R; r++) {
-          double2 w = tw[r * kk * tstep];
-          if (SIGN > 0) w.y = -w.y;
-          x[k][r] = cmul(x[k][r], w);
-        }
-      }
-      Dft<R, SIGN>::run(x[k]);
-      const int base = s * n + (j - kk) * R + kk;
-#pragma unroll
-      for (int r = 0; r < R; r++) buf[base + r * Ns] = x[k][r];
-    }
-  }
-  __syncthreads();
-}
-
 // Interleaved layout: element idx of sequence s lives at buf[idx*S + s], so
 // consecutive threads touch consecutive LDS words in every pass.  Twiddles of
 // the length-n transform come from a table of length tw_n (a multiple of n):
@@ -208,25 +167,6 @@ __device__ void fft_il(double2 *buf, int S, int n, int npass, unsigned long long
       case 4: fft_pass_il<NT, EMAX, 4, SIGN>(buf, S, n, Ns, tw, twmul); break;
       case 5: fft_pass_il<NT, EMAX, 5, SIGN>(buf, S, n, Ns, tw, twmul); break;
       default: fft_pass_il<NT, EMAX, 8, SIGN>(buf, S, n, Ns, tw, twmul); break;
-    }
-    Ns *= R;
-  }
-}
-
-// In-place transform of S sequences of length n stored back to back in buf.
-// EMAX bounds S*n/NT (complex values per thread).
-template <int NT, int EMAX, int SIGN>
-__device__ void fft_lds(double2 *buf, int S, int n, int npass, unsigned long long code,
-                        const double2 *__restrict__ tw) {
-  int Ns = 1;
-  for (int p = 0; p < npass; p++) {
-    const int R = (int)((code >> (4 * p)) & 15ull);
-    switch (R) {
-      case 2: fft_pass<NT, EMAX, 2, SIGN>(buf, S, n, Ns, tw); break;
-      case 3: fft_pass<NT, EMAX, 3, SIGN>(buf, S, n, Ns, tw); break;
-      case 4: fft_pass<NT, EMAX, 4, SIGN>(buf, S, n, Ns, tw); break;
-      case 5: fft_pass<NT, EMAX, 5, SIGN>(buf, S, n, Ns, tw); break;
-      default: fft_pass<NT, EMAX, 8, SIGN>(buf, S, n, Ns, tw); break;
     }
     Ns *= R;
   }

@@ -67,20 +67,6 @@ void launch_cny_rows(hipStream_t s, const Geo &g, int kind, RK rk, const double 
 void launch_transpose(hipStream_t s, int g_isize, int g_jsize, size_t elemsize, const void *send,
                       void *recv);
 
-// ---- Poisson (src/fluid/compute_potential.c)
-struct FftPlan {
-  int n;                    // transform length (jtot)
-  int npass;                // number of radix passes
-  int S;                    // complex sequences per workgroup (= real columns / 2)
-  unsigned long long code;  // radix of pass p in bits [4p, 4p+4)
-};
-void launch_poisson_fwd(hipStream_t s, const Geo &g, const FftPlan &pl, double gamma,
-                        const double *dtp, const double *ux, const double *uy,
-                        const double2 *tw, double2 *psihat);
-void launch_poisson_x(hipStream_t s, const Geo &g, const double *eigen, double2 *psihat);
-void launch_poisson_inv(hipStream_t s, const Geo &g, const FftPlan &pl, const double2 *tw,
-                        const double2 *psihat, double *psi);
-
 // ---- correction and pressure (src/fluid/correct_velocity.c, update_pressure.c)
 void launch_correct(hipStream_t s, const Geo &g, double gamma, const double *dtp,
                     const double *psi, double *ux, double *uy);

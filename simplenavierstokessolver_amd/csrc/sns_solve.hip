@@ -1,7 +1,6 @@
 // sns_solve.hip -- line solvers, transposes and the Poisson chain (gfx950).
 #include "sns_launch.h"
 #include "sns_tridiag.h"
-#include "sns_fft.h"
 
 namespace sns {
 
@@ -268,174 +267,6 @@ void launch_transpose(hipStream_t s, int g_isize, int g_jsize, size_t elemsize, 
   else
     hipLaunchKernelGGL(k_transpose<uint4>, gr, dim3(256), 0, s, g_isize, g_jsize,
                        (const uint4 *)send, (uint4 *)recv);
-}
-
-// --------------------------------------------------------------- Poisson
-
-static constexpr int kFftThreads = 256;
-static constexpr int kFftMaxElems = 8192;  // complex values per workgroup (128 KB LDS)
-
-// Right-hand side of the Poisson equation at cell (i, j):
-// 1./(gamma*dt)*((ux_xp-ux_xm)/DXF(i) + (uy_yp-uy_ym)/dy)
-// (src/fluid/compute_potential.c:74-86)
-__device__ __forceinline__ double poisson_rhs(const Geo &g, const double *ux, const double *uy,
-                                              double inv_gdt, int i, int j) {
-  const double ux_xm = ux[ux_idx(g, i, j)];
-  const double ux_xp = ux[ux_idx(g, i + 1, j)];
-  const double uy_ym = uy[c_idx(g, i, j)];
-  const double uy_yp = uy[c_idx(g, i, j + 1)];
-  return inv_gdt * (+(ux_xp - ux_xm) / DXF(g, i) + (uy_yp - uy_ym) / g.dy);
-}
-
-// Forward half of the Poisson solve for a strip of 2*S columns: RHS, real
-// FFT along y of each column (two real columns packed as one complex
-// sequence), spectrum stored x-aligned in psihat[J*itot + (i-1)],
-// J = 0..jtot/2 (src/fluid/compute_potential.c:69-92).
-__global__ void __launch_bounds__(kFftThreads)
-    k_poisson_fwd(Geo g, FftPlan pl, double gamma, const double *__restrict__ dtp,
-                  const double *__restrict__ ux, const double *__restrict__ uy,
-                  const double2 *__restrict__ tw, double2 *__restrict__ psihat) {
-  extern __shared__ double2 buf[];
-  const int n = pl.n, S = pl.S;
-  const int ibase = 1 + 2 * S * blockIdx.x;
-  const double inv_gdt = 1. / (gamma * (*dtp));
-  for (int idx = threadIdx.x; idx < S * n; idx += kFftThreads) {
-    const int s = idx % S, j = idx / S;
-    const int ia = ibase + 2 * s, ib = ia + 1;
-    double2 z;
-    z.x = ia <= g.itot ? poisson_rhs(g, ux, uy, inv_gdt, ia, j + 1) : 0.;
-    z.y = ib <= g.itot ? poisson_rhs(g, ux, uy, inv_gdt, ib, j + 1) : 0.;
-    buf[s * n + j] = z;
-  }
-  __syncthreads();
-  fft_lds<kFftThreads, kFftMaxElems / kFftThreads, -1>(buf, S, pl.n, pl.npass, pl.code, tw);
-  const int nh = n / 2 + 1;
-  for (int idx = threadIdx.x; idx < S * nh; idx += kFftThreads) {
-    const int s = idx % S, k = idx / S;
-    const double2 zk = buf[s * n + k];
-    const double2 zn = buf[s * n + (k == 0 ? 0 : n - k)];
-    const int ia = ibase + 2 * s, ib = ia + 1;
-    double2 xa, xb;
-    xa.x = 0.5 * (zk.x + zn.x);
-    xa.y = 0.5 * (zk.y - zn.y);
-    xb.x = 0.5 * (zk.y + zn.y);
-    xb.y = 0.5 * (zn.x - zk.x);
-    if (ia <= g.itot) psihat[(size_t)k * g.itot + (ia - 1)] = xa;
-    if (ib <= g.itot) psihat[(size_t)k * g.itot + (ib - 1)] = xb;
-  }
-}
-
-// Inverse half: Hermitian-extended spectra of two columns packed as one
-// complex sequence, inverse FFT, psi = result/jtot with Neumann walls and
-// periodic halos (src/fluid/compute_potential.c:146-159).
-__global__ void __launch_bounds__(kFftThreads)
-    k_poisson_inv(Geo g, FftPlan pl, const double2 *__restrict__ tw,
-                  const double2 *__restrict__ psihat, double *__restrict__ psi) {
-  extern __shared__ double2 buf[];
-  const int n = pl.n, S = pl.S;
-  const int ibase = 1 + 2 * S * blockIdx.x;
-  const int nh = n / 2 + 1;
-  for (int idx = threadIdx.x; idx < S * nh; idx += kFftThreads) {
-    const int s = idx % S, k = idx / S;
-    const int ia = ibase + 2 * s, ib = ia + 1;
-    double2 xa = make_double2(0., 0.), xb = make_double2(0., 0.);
-    if (ia <= g.itot) xa = psihat[(size_t)k * g.itot + (ia - 1)];
-    if (ib <= g.itot) xb = psihat[(size_t)k * g.itot + (ib - 1)];
-    const bool selfconj = (k == 0) || (2 * k == n);
-    if (selfconj) { xa.y = 0.; xb.y = 0.; }
-    buf[s * n + k] = make_double2(xa.x - xb.y, xa.y + xb.x);
-    if (!selfconj) buf[s * n + (n - k)] = make_double2(xa.x + xb.y, -xa.y + xb.x);
-  }
-  __syncthreads();
-  fft_lds<kFftThreads, kFftMaxElems / kFftThreads, +1>(buf, S, pl.n, pl.npass, pl.code, tw);
-  const double njtot = 1. * g.jtot;
-  for (int idx = threadIdx.x; idx < S * n; idx += kFftThreads) {
-    const int s = idx % S, j = idx / S;
-    const int ia = ibase + 2 * s, ib = ia + 1;
-    const double2 z = buf[s * n + j];
-    if (ia <= g.itot) store_c_bc<1>(g, psi, ia, j + 1, z.x / njtot);
-    if (ib <= g.itot) store_c_bc<1>(g, psi, ib, j + 1, z.y / njtot);
-  }
-}
-
-static int fft_strips(const Geo &g, const FftPlan &pl) {
-  const int cols = 2 * pl.S;
-  return (g.itot + cols - 1) / cols;
-}
-
-// dynamic LDS above 64 KB must be enabled per kernel
-static void allow_lds(const void *fn, size_t bytes) {
-  if (bytes > 65536) hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-}
-
-void launch_poisson_fwd(hipStream_t s, const Geo &g, const FftPlan &pl, double gamma,
-                        const double *dtp, const double *ux, const double *uy, const double2 *tw,
-                        double2 *psihat) {
-  const size_t lds = sizeof(double2) * pl.S * pl.n;
-  allow_lds((const void *)k_poisson_fwd, lds);
-  hipLaunchKernelGGL(k_poisson_fwd, dim3(fft_strips(g, pl)), dim3(kFftThreads), lds, s, g, pl,
-                     gamma, dtp, ux, uy, tw, psihat);
-}
-
-void launch_poisson_inv(hipStream_t s, const Geo &g, const FftPlan &pl, const double2 *tw,
-                        const double2 *psihat, double *psi) {
-  const size_t lds = sizeof(double2) * pl.S * pl.n;
-  allow_lds((const void *)k_poisson_inv, lds);
-  hipLaunchKernelGGL(k_poisson_inv, dim3(fft_strips(g, pl)), dim3(kFftThreads), lds, s, g, pl,
-                     tw, psihat, psi);
-}
-
-// Per-wavenumber tri-diagonal solve along x (src/fluid/compute_potential.c:109-136):
-// l_i = 1/DXC(i)/DXF(i), u_i = 1/DXC(i+1)/DXF(i), c_i = -l_i-u_i+lambda_J,
-// Neumann rows c_1 += l_1, c_itot += u_itot.  The J = 0 system is singular
-// (the potential is defined up to a constant); its last equation is replaced
-// by psi_hat(itot) = 0, which fixes that constant.
-template <int NT, int M>
-__global__ void __launch_bounds__(NT)
-    k_poisson_x(Geo g, const double *__restrict__ eigen, double2 *__restrict__ psihat) {
-  __shared__ TridiagLds<NT, 2> lds;
-  const int J = blockIdx.x;
-  const int n = g.itot;
-  const double lam = eigen[J];
-  double2 *row = psihat + (size_t)J * n;
-  double a[M], b[M], c[M], d[2][M];
-#pragma unroll
-  for (int k = 0; k < M; k++) {
-    const int r = threadIdx.x * M + k;
-    if (r < n) {
-      const int i = r + 1;
-      const double l = 1. / DXC(g, i) / DXF(g, i);
-      const double u = 1. / DXC(g, i + 1) / DXF(g, i);
-      double cc = -l - u + lam;
-      if (r == 0) cc += l;
-      if (r == n - 1) cc += u;
-      a[k] = (r == 0) ? 0. : l;
-      b[k] = cc;
-      c[k] = (r == n - 1) ? 0. : u;
-      const double2 v = row[r];
-      d[0][k] = v.x;
-      d[1][k] = v.y;
-      if (J == 0 && r == n - 1) { a[k] = 0.; b[k] = 1.; d[0][k] = 0.; d[1][k] = 0.; }
-    } else {
-      a[k] = 0.; b[k] = 1.; c[k] = 0.; d[0][k] = 0.; d[1][k] = 0.;
-    }
-  }
-  block_tridiag<NT, M, 2>(a, b, c, d, lds);
-#pragma unroll
-  for (int k = 0; k < M; k++) {
-    const int r = threadIdx.x * M + k;
-    if (r < n) row[r] = make_double2(d[0][k], d[1][k]);
-  }
-}
-
-void launch_poisson_x(hipStream_t s, const Geo &g, const double *eigen, double2 *psihat) {
-  const dim3 gr(g.jtot / 2 + 1);
-  const int n = g.itot;
-  if (n <= 64 * 4) hipLaunchKernelGGL((k_poisson_x<64, 4>), gr, dim3(64), 0, s, g, eigen, psihat);
-  else if (n <= 256 * 4) hipLaunchKernelGGL((k_poisson_x<256, 4>), gr, dim3(256), 0, s, g, eigen, psihat);
-  else if (n <= 256 * 8) hipLaunchKernelGGL((k_poisson_x<256, 8>), gr, dim3(256), 0, s, g, eigen, psihat);
-  else if (n <= 256 * 16) hipLaunchKernelGGL((k_poisson_x<256, 16>), gr, dim3(256), 0, s, g, eigen, psihat);
-  else hipLaunchKernelGGL((k_poisson_x<256, 32>), gr, dim3(256), 0, s, g, eigen, psihat);
 }
 
 }  // namespace sns
