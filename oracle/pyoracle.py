@@ -291,6 +291,17 @@ def transpose_sim(nranks, g_isize, g_jsize, sendbufs, recvbufs):
     lib().orc_parallel_transpose_sim(nranks, g_isize, g_jsize, es, S, R)
 
 
+VARIANT_DIRECT_DFT = 1      # Poisson transforms as a direct DFT
+VARIANT_REVERSED_SOLVE = 2  # tri-diagonal systems eliminated last row first
+VARIANT_PIN_GAUGE = 4       # J = 0 Poisson system pinned at psi_hat(itot) = 0
+
+
+def set_variant(bits):
+    """Alternative operation orders for the same arithmetic (0: the
+    reference's), for rounding-floor runs."""
+    lib().orc_set_variant(int(bits))
+
+
 def num_threads():
     """Threads the oracle's OpenMP loops use (OMP_NUM_THREADS)."""
     return int(lib().orc_num_threads())

@@ -236,10 +236,27 @@ int orc_temperature_update_boundaries_temp(const orc_param_t *param, double *tem
   return 0;
 }
 
+/* ------------------------------------------------------------- variants */
+
+/* Alternative operation orders for the same arithmetic (default 0: the
+ * reference's).  ORC_VARIANT_DIRECT_DFT sums the Poisson transforms as a
+ * direct O(n^2) DFT instead of the mixed-radix FFT; ORC_VARIANT_REVERSED_SOLVE
+ * eliminates every tri-diagonal system from its last row to its first;
+ * ORC_VARIANT_PIN_GAUGE replaces the last equation of the singular J = 0
+ * Poisson system by psi_hat(itot) = 0 instead of letting the rounding-level
+ * last pivot set the additive constant (src/linalg.c:36-43).  Tests
+ * compare runs with and without them to measure how far last-bit differences
+ * travel in a given configuration (its rounding floor). */
+static int g_variant = 0;
+int orc_set_variant(int bits){
+  g_variant = bits;
+  return 0;
+}
+
 /* --------------------------------------------------------- linear algebra */
 
 /* src/linalg.c:9-30 */
-int orc_dgtsv_b(int n, double *l, double *c, double *u, double *q){
+static int thomas_d(int n, double *l, double *c, double *u, double *q){
   u[0] = u[0]/c[0];
   q[0] = q[0]/c[0];
   for(int i=1; i<=n-1; i++){
@@ -257,10 +274,33 @@ int orc_dgtsv_b(int n, double *l, double *c, double *u, double *q){
   return 0;
 }
 
+/* The system with its rows in reverse order (sub- and super-diagonals
+ * exchanged) solved by f, the solution written back in the original order. */
+static void reversed_copy(int n, const double *l, const double *c, const double *u,
+                          double *lr, double *cr, double *ur){
+  for(int i=0; i<n; i++){
+    lr[i] = u[n-1-i];
+    cr[i] = c[n-1-i];
+    ur[i] = l[n-1-i];
+  }
+}
+
+int orc_dgtsv_b(int n, double *l, double *c, double *u, double *q){
+  if(!(g_variant & ORC_VARIANT_REVERSED_SOLVE)) return thomas_d(n, l, c, u, q);
+  double *lr = xcalloc(n, sizeof(double)), *cr = xcalloc(n, sizeof(double));
+  double *ur = xcalloc(n, sizeof(double)), *qr = xcalloc(n, sizeof(double));
+  reversed_copy(n, l, c, u, lr, cr, ur);
+  for(int i=0; i<n; i++) qr[i] = q[n-1-i];
+  thomas_d(n, lr, cr, ur, qr);
+  for(int i=0; i<n; i++) q[i] = qr[n-1-i];
+  free(lr); free(cr); free(ur); free(qr);
+  return 0;
+}
+
 /* src/linalg.c:32-50; real matrix, complex right-hand side.  The
  * complex-by-real products and quotients are component-wise (C99 G.5.1),
  * written out explicitly here. */
-int orc_zgtsv_b(int n, double *l, double *c, double *u, double complex *q){
+static int thomas_z(int n, double *l, double *c, double *u, double complex *q){
   double *qr = xcalloc(n, sizeof(double));
   double *qi = xcalloc(n, sizeof(double));
   for(int i=0; i<n; i++){ qr[i] = creal(q[i]); qi[i] = cimag(q[i]); }
@@ -287,6 +327,19 @@ int orc_zgtsv_b(int n, double *l, double *c, double *u, double complex *q){
   for(int i=0; i<n; i++) q[i] = mkz(qr[i], qi[i]);
   free(qr);
   free(qi);
+  return 0;
+}
+
+int orc_zgtsv_b(int n, double *l, double *c, double *u, double complex *q){
+  if(!(g_variant & ORC_VARIANT_REVERSED_SOLVE)) return thomas_z(n, l, c, u, q);
+  double *lr = xcalloc(n, sizeof(double)), *cr = xcalloc(n, sizeof(double));
+  double *ur = xcalloc(n, sizeof(double));
+  double complex *qr = xcalloc(n, sizeof(double complex));
+  reversed_copy(n, l, c, u, lr, cr, ur);
+  for(int i=0; i<n; i++) qr[i] = q[n-1-i];
+  thomas_z(n, lr, cr, ur, qr);
+  for(int i=0; i<n; i++) q[i] = qr[n-1-i];
+  free(lr); free(cr); free(ur); free(qr);
   return 0;
 }
 
@@ -372,12 +425,32 @@ static double complex *twiddles(int n){
   return tw;
 }
 
+
+static void cdft_direct(int n, const double complex *x, double complex *y,
+                        const double complex *tw, int sign){
+  for(int k=0; k<n; k++){
+    double complex acc = 0.;
+    for(int j=0; j<n; j++){
+      double complex w = tw[((long)j*k) % n];
+      if(sign > 0) w = conj(w);
+      acc += w*x[j];
+    }
+    y[k] = acc;
+  }
+}
+
+static void cdft(int n, const double complex *x, double complex *y, const double complex *tw,
+                 int sign, double complex *scr){
+  if(g_variant & ORC_VARIANT_DIRECT_DFT) cdft_direct(n, x, y, tw, sign);
+  else cdft_rec(n, x, 1, y, tw, n, sign, scr);
+}
+
 /* r2c with precomputed twiddles; work holds 4n entries */
 static void rfft_tw(int n, const double *in, double complex *out, const double complex *tw,
                     double complex *work){
   double complex *x = work, *y = work + n;
   for(int j=0; j<n; j++) x[j] = mkz(in[j], 0.);
-  cdft_rec(n, x, 1, y, tw, n, -1, work + 2*(size_t)n);
+  cdft(n, x, y, tw, -1, work + 2*(size_t)n);
   for(int k=0; k<=n/2; k++) out[k] = y[k];
   out[0] = mkz(creal(out[0]), 0.);
   if(n % 2 == 0) out[n/2] = mkz(creal(out[n/2]), 0.);
@@ -391,7 +464,7 @@ static void irfft_tw(int n, const double complex *in, double *out, const double 
   for(int k=n/2+1; k<n; k++) x[k] = conj(in[n-k]);
   x[0] = mkz(creal(x[0]), 0.);
   if(n % 2 == 0) x[n/2] = mkz(creal(x[n/2]), 0.);
-  cdft_rec(n, x, 1, y, tw, n, +1, work + 2*(size_t)n);
+  cdft(n, x, y, tw, +1, work + 2*(size_t)n);
   for(int j=0; j<n; j++) out[j] = creal(y[j]);
 }
 
@@ -971,6 +1044,13 @@ int orc_fluid_compute_potential(const orc_param_t *param, int rkstep, orc_state_
       c[     0] += l[     0];
       c[itot-1] += u[itot-1];
       for(int i=1; i<=itot; i++) q[i-1] = QCX(i, j);
+      if(j == 1 && (g_variant & ORC_VARIANT_PIN_GAUGE)){
+        /* the singular J = 0 system with its last equation replaced by
+         * psi_hat(itot) = 0 */
+        l[itot-1] = 0.;
+        c[itot-1] = 1.;
+        q[itot-1] = 0.;
+      }
       orc_zgtsv_b(itot, l, c, u, q);
       for(int i=1; i<=itot; i++) QCX(i, j) = q[i-1];
     }

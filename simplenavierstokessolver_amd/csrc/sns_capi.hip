@@ -27,6 +27,7 @@ struct snsg_ctx {
   double *d_dxf = nullptr, *d_dxc = nullptr;
   double *d_dt = nullptr;
   double *d_eigen = nullptr;
+  double *d_pfac = nullptr;             // Thomas factors of the Poisson x systems
   double2 *d_tw = nullptr;
   double2 *d_psihat = nullptr;
   double *d_q = nullptr, *d_qy = nullptr;
@@ -236,6 +237,13 @@ extern "C" int snsg_create(int device, const snsg_config *cfg, const double *dxf
       hipMemcpy(c->d_tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice) !=
           hipSuccess)
     return fail(set_err(SNSG_E_ARG, "upload tables"));
+  {
+    const size_t nfac = (size_t)(jtot / 2 + 1) * 3 * poisson_factor_entries(itot);
+    if ((rc = ctx_alloc((void **)&c->d_pfac, sizeof(double) * nfac))) return fail(rc);
+    launch_poisson_factor(c->stream, c->geo, c->tab, c->d_eigen, c->d_pfac);
+    if (hipStreamSynchronize(c->stream) != hipSuccess)
+      return fail(set_err(SNSG_E_ARG, "Poisson factorisation"));
+  }
   *out = c;
   return 0;
 }
@@ -246,7 +254,7 @@ extern "C" int snsg_destroy(snsg_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *bufs[] = {c->d_dxf, c->d_dxc, c->d_dt, c->d_eigen, c->d_tw, c->d_psihat, c->d_Y, c->d_q,
                   c->d_qy, c->d_red, c->d_tab, c->d_time, c->d_qf[0], c->d_qf[1],
-                  c->d_qf[2], c->d_lu, c->d_part};
+                  c->d_qf[2], c->d_lu, c->d_part, c->d_pfac};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (c->h_red) hipHostFree(c->h_red);
@@ -355,17 +363,17 @@ static RK rk_of(const snsg_ctx *c, int rkstep) {
 static int potential(snsg_ctx *c, double gamma, const double *ux, const double *uy, double *psi) {
   int rc = run(c, "poisson_fwd", [&] {
     launch_poisson2(c->stream, c->geo, c->tab, c->plan, gamma, c->d_dt, ux, uy, c->d_tw,
-                    c->d_eigen, c->d_Y, c->d_psihat, psi, 0);
+                    c->d_pfac, c->d_Y, c->d_psihat, psi, 0);
   });
   if (rc) return rc;
   if ((rc = run(c, "poisson_x", [&] {
          launch_poisson2(c->stream, c->geo, c->tab, c->plan, gamma, c->d_dt, ux, uy, c->d_tw,
-                         c->d_eigen, c->d_Y, c->d_psihat, psi, 1);
+                         c->d_pfac, c->d_Y, c->d_psihat, psi, 1);
        })))
     return rc;
   return run(c, "poisson_inv", [&] {
     launch_poisson2(c->stream, c->geo, c->tab, c->plan, gamma, c->d_dt, ux, uy, c->d_tw,
-                    c->d_eigen, c->d_Y, c->d_psihat, psi, 2);
+                    c->d_pfac, c->d_Y, c->d_psihat, psi, 2);
   });
 }
 
@@ -543,7 +551,8 @@ static int integrate_fused(snsg_ctx *c, snsg_state *st, bool reduce) {
   int rc;
   const bool impx = c->treat.implicitx, impy = c->treat.implicity;
   if (impx && !pred_impx_supported(itot))
-    return set_err(SNSG_E_SIZE, "fused step: implicit x needs itot <= 4096");
+    return set_err(SNSG_E_SIZE, "fused step: implicit x needs itot <= 8192");
+  if (jtot < 2) return set_err(SNSG_E_SIZE, "fused step: jtot must be at least 2");
   if (impy && !c->d_qf[0]) {
     const size_t nb[3] = {(size_t)itot * jtot, (size_t)(itot - 1) * jtot, (size_t)itot * jtot};
     for (int k = 0; k < 3; k++)
@@ -553,7 +562,7 @@ static int integrate_fused(snsg_ctx *c, snsg_state *st, bool reduce) {
   if (impx) {
     if (!c->d_lu) {
       c->lu_ntm = lu_table_entries(itot);
-      if ((rc = ctx_alloc((void **)&c->d_lu, sizeof(double) * 45 * (size_t)c->lu_ntm))) return rc;
+      if ((rc = ctx_alloc((void **)&c->d_lu, sizeof(double) * 27 * (size_t)c->lu_ntm))) return rc;
     }
     // the CN-x operators of all three substages depend on dt only
     LuFactorArgs la;
@@ -582,7 +591,7 @@ static int integrate_fused(snsg_ctx *c, snsg_state *st, bool reduce) {
     pa.write_hist = rk < 2;
     pa.with_temp = with_temp;
     pa.forcing = with_temp && c->cfg.with_thermal_forcing;
-    const double *lu = c->d_lu ? c->d_lu + (size_t)(rk * 3 * 5) * c->lu_ntm : nullptr;
+    const double *lu = c->d_lu ? c->d_lu + (size_t)(rk * 3 * 3) * c->lu_ntm : nullptr;
     if ((rc = run(c, "pred", [&] { launch_pred(c->stream, pa, impx, lu, c->lu_ntm); })))
       return rc;
     if (impy) {

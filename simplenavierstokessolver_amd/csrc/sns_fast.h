@@ -43,80 +43,206 @@ struct Tables {
   double dy, rdy;
 };
 
-// Temperature sources at (i, j): src/temperature/update.c:33-84
-__device__ __forceinline__ void temp_src_t(const Geo &g, const Tables &tb, const double *ux,
-                                           const double *uy, const double *temp, int i, int j,
-                                           double wex, double wey, double wix, double wiy,
-                                           double &srca, double &srcg) {
+// Loads and stores through a row base (uniform across the workgroup, so it
+// stays in scalar registers) plus a 32-bit byte offset: the SGPR-base +
+// VGPR-offset addressing form, one VGPR per address instead of a 64-bit
+// pair.  k >= 0 and k*8 < 2^32.
+__device__ __forceinline__ double ldr(const double *row, int k) {
+  return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(row) +
+                                           ((unsigned)k << 3));
+}
+__device__ __forceinline__ void str(double *row, int k, double v) {
+  *reinterpret_cast<double *>(reinterpret_cast<char *>(row) + ((unsigned)k << 3)) = v;
+}
+
+// Rows j-1, j, j+1 of a field: pointers to the first stored element of each
+// row (I = 0 for uy, p, psi, T; I = 1 for ux).
+struct Rows3 {
+  const double *m, *c, *p;
+};
+__device__ __forceinline__ Rows3 ux_rows(const Geo &g, const double *f, int j) {
+  const size_t w = (size_t)(g.itot + 1);
+  return {f + (size_t)(j - 1) * w, f + (size_t)j * w, f + (size_t)(j + 1) * w};
+}
+__device__ __forceinline__ Rows3 c_rows(const Geo &g, const double *f, int j) {
+  const size_t w = (size_t)(g.itot + 2);
+  return {f + (size_t)(j - 1) * w, f + (size_t)j * w, f + (size_t)(j + 1) * w};
+}
+
+// ------------------------------------------------------------ source terms
+//
+// Each source term is written once, on the values around the point; the
+// loaders below (pointer form) and the row kernels (values shared across
+// lanes and rows) feed it.  Operation order follows the reference
+// expression by expression.
+
+// Temperature at (i, j) and its neighbours, with the velocities it is
+// advected by: src/temperature/update.c:33-84
+struct TempPt {
+  double t_c, t_xm, t_xp, t_ym, t_yp;  // T(i,j), T(i-1,j), T(i+1,j), T(i,j-1), T(i,j+1)
+  double ux_c, ux_p;                   // UX(i,j), UX(i+1,j)
+  double uy_c, uy_p;                   // UY(i,j), UY(i,j+1)
+};
+// Column metrics of cell i: DXC(i), DXC(i+1), DXF(i) with reciprocals and the
+// interpolation weights DXC(i)/(2DXF(i)), DXC(i+1)/(2DXF(i))
+struct CellX {
+  double dxc, rdxc, dxc_p, rdxc_p, dxf, rdxf, c_xm, c_xp;
+};
+__device__ __forceinline__ void temp_src_v(const Geo &g, const Tables &tb, const CellX &x,
+                                           const TempPt &v, double wex, double wey, double wix,
+                                           double wiy, double &srca, double &srcg) {
   const double dy = tb.dy, rdy = tb.rdy;
-  const double dxc_i = tb.dxc[i - 1], dxc_ip = tb.dxc[i], dxf_i = tb.dxf[i - 1];
-  const double t_c = temp[c_idx(g, i, j)];
-  const double dtdx_xm = div_cr(-temp[c_idx(g, i - 1, j)] + t_c, dxc_i, tb.rdxc[i - 1]);
-  const double dtdx_xp = div_cr(-t_c + temp[c_idx(g, i + 1, j)], dxc_ip, tb.rdxc[i]);
-  const double dtdy_ym = div_cr(-temp[c_idx(g, i, j - 1)] + t_c, dy, rdy);
-  const double dtdy_yp = div_cr(-t_c + temp[c_idx(g, i, j + 1)], dy, rdy);
-  const double c_xm = tb.cT_xm[i - 1], c_xp = tb.cT_xp[i - 1];
-  const double adv1 = -c_xm * ux[ux_idx(g, i, j)] * dtdx_xm - c_xp * ux[ux_idx(g, i + 1, j)] * dtdx_xp;
-  const double adv2 = -0.5 * uy[c_idx(g, i, j)] * dtdy_ym - 0.5 * uy[c_idx(g, i, j + 1)] * dtdy_yp;
-  const double dif1 = div_cr(g.kappa_T * (dtdx_xp - dtdx_xm), dxf_i, tb.rdxf[i - 1]);
+  const double dtdx_xm = div_cr(-v.t_xm + v.t_c, x.dxc, x.rdxc);
+  const double dtdx_xp = div_cr(-v.t_c + v.t_xp, x.dxc_p, x.rdxc_p);
+  const double dtdy_ym = div_cr(-v.t_ym + v.t_c, dy, rdy);
+  const double dtdy_yp = div_cr(-v.t_c + v.t_yp, dy, rdy);
+  const double adv1 = -x.c_xm * v.ux_c * dtdx_xm - x.c_xp * v.ux_p * dtdx_xp;
+  const double adv2 = -0.5 * v.uy_c * dtdy_ym - 0.5 * v.uy_p * dtdy_yp;
+  const double dif1 = div_cr(g.kappa_T * (dtdx_xp - dtdx_xm), x.dxf, x.rdxf);
   const double dif2 = div_cr(g.kappa_T * (dtdy_yp - dtdy_ym), dy, rdy);
   srca = +(adv1 + adv2) + wex * dif1 + wey * dif2;
   srcg = +wix * dif1 + wiy * dif2;
 }
 
-// x-momentum sources at (i, j), i = 2..itot: src/fluid/update_velocity.c:34-101
-__device__ __forceinline__ void ux_src_t(const Geo &g, const Tables &tb, const double *ux,
-                                         const double *uy, const double *p, double tf, int i,
-                                         int j, double wex, double wey, double wix, double wiy,
-                                         double &srca, double &srcg) {
+// x-momentum at face (i, j), i = 2..itot: src/fluid/update_velocity.c:34-101
+struct UxPt {
+  double u_c, u_xm, u_xp, u_ym, u_yp;  // UX(i,j), UX(i-1,j), UX(i+1,j), UX(i,j-1), UX(i,j+1)
+  double uy_cm, uy_cc, uy_pm, uy_pc;   // UY(i-1,j), UY(i,j), UY(i-1,j+1), UY(i,j+1)
+  double p_c, p_m;                     // P(i,j), P(i-1,j)
+  double tf;                           // buoyancy at the face
+};
+// Face metrics of face i: DXF(i-1), DXF(i) with reciprocals, the weights
+// DXF(i-1)/(2DXC(i)), DXF(i)/(2DXC(i)) and DXC(i) with its reciprocal
+struct FaceX {
+  double dxf_m, rdxf_m, dxf, rdxf, c_xm, c_xp, dxc, rdxc;
+};
+__device__ __forceinline__ void ux_src_v(const Geo &g, const Tables &tb, const FaceX &x,
+                                         const UxPt &v, double wex, double wey, double wix,
+                                         double wiy, double &srca, double &srcg) {
   const double dy = tb.dy, rdy = tb.rdy;
-  const double u_c = ux[ux_idx(g, i, j)];
-  const double u_xm = ux[ux_idx(g, i - 1, j)];
-  const double u_xp = ux[ux_idx(g, i + 1, j)];
-  const double duxdx_xm = div_cr(-u_xm + u_c, tb.dxf[i - 2], tb.rdxf[i - 2]);
-  const double duxdx_xp = div_cr(-u_c + u_xp, tb.dxf[i - 1], tb.rdxf[i - 1]);
-  const double duxdy_ym = div_cr(-ux[ux_idx(g, i, j - 1)] + u_c, dy, rdy);
-  const double duxdy_yp = div_cr(-u_c + ux[ux_idx(g, i, j + 1)], dy, rdy);
-  const double c_xm = tb.cU_xm[i - 1], c_xp = tb.cU_xp[i - 1];
-  const double ux_xm = 0.5 * u_xm + 0.5 * u_c;
-  const double ux_xp = 0.5 * u_c + 0.5 * u_xp;
-  const double adv1 = -c_xm * ux_xm * duxdx_xm - c_xp * ux_xp * duxdx_xp;
-  const double uy_ym = c_xm * uy[c_idx(g, i - 1, j)] + c_xp * uy[c_idx(g, i, j)];
-  const double uy_yp = c_xm * uy[c_idx(g, i - 1, j + 1)] + c_xp * uy[c_idx(g, i, j + 1)];
+  const double duxdx_xm = div_cr(-v.u_xm + v.u_c, x.dxf_m, x.rdxf_m);
+  const double duxdx_xp = div_cr(-v.u_c + v.u_xp, x.dxf, x.rdxf);
+  const double duxdy_ym = div_cr(-v.u_ym + v.u_c, dy, rdy);
+  const double duxdy_yp = div_cr(-v.u_c + v.u_yp, dy, rdy);
+  const double ux_xm = 0.5 * v.u_xm + 0.5 * v.u_c;
+  const double ux_xp = 0.5 * v.u_c + 0.5 * v.u_xp;
+  const double adv1 = -x.c_xm * ux_xm * duxdx_xm - x.c_xp * ux_xp * duxdx_xp;
+  const double uy_ym = x.c_xm * v.uy_cm + x.c_xp * v.uy_cc;
+  const double uy_yp = x.c_xm * v.uy_pm + x.c_xp * v.uy_pc;
   const double adv2 = -0.5 * uy_ym * duxdy_ym - 0.5 * uy_yp * duxdy_yp;
-  const double dxc_i = tb.dxc[i - 1], rdxc_i = tb.rdxc[i - 1];
-  const double dif1 = div_cr(g.kappa_u * (duxdx_xp - duxdx_xm), dxc_i, rdxc_i);
+  const double dif1 = div_cr(g.kappa_u * (duxdx_xp - duxdx_xm), x.dxc, x.rdxc);
   const double dif2 = div_cr(g.kappa_u * (duxdy_yp - duxdy_ym), dy, rdy);
-  const double pre = div_cr(-(p[c_idx(g, i, j)] - p[c_idx(g, i - 1, j)]), dxc_i, rdxc_i);
-  srca = +(adv1 + adv2) + wex * dif1 + wey * dif2 + tf;
+  const double pre = div_cr(-(v.p_c - v.p_m), x.dxc, x.rdxc);
+  srca = +(adv1 + adv2) + wex * dif1 + wey * dif2 + v.tf;
   srcg = +wix * dif1 + wiy * dif2 + pre;
 }
 
-// y-momentum sources at (i, j), i = 1..itot: src/fluid/update_velocity.c:128-189
-__device__ __forceinline__ void uy_src_t(const Geo &g, const Tables &tb, const double *ux,
-                                         const double *uy, const double *p, int i, int j,
-                                         double wex, double wey, double wix, double wiy,
-                                         double &srca, double &srcg) {
+// y-momentum at (i, j), i = 1..itot: src/fluid/update_velocity.c:128-189
+struct UyPt {
+  double v_c, v_xm, v_xp, v_ym, v_yp;  // UY(i,j), UY(i-1,j), UY(i+1,j), UY(i,j-1), UY(i,j+1)
+  double ux_mc, ux_cc, ux_mp, ux_cp;   // UX(i,j-1), UX(i,j), UX(i+1,j-1), UX(i+1,j)
+  double p_c, p_ym;                    // P(i,j), P(i,j-1)
+};
+__device__ __forceinline__ void uy_src_v(const Geo &g, const Tables &tb, const CellX &x,
+                                         const UyPt &v, double wex, double wey, double wix,
+                                         double wiy, double &srca, double &srcg) {
   const double dy = tb.dy, rdy = tb.rdy;
-  const double v_c = uy[c_idx(g, i, j)];
-  const double v_ym = uy[c_idx(g, i, j - 1)];
-  const double v_yp = uy[c_idx(g, i, j + 1)];
-  const double duydx_xm = div_cr(-uy[c_idx(g, i - 1, j)] + v_c, tb.dxc[i - 1], tb.rdxc[i - 1]);
-  const double duydx_xp = div_cr(-v_c + uy[c_idx(g, i + 1, j)], tb.dxc[i], tb.rdxc[i]);
-  const double duydy_ym = div_cr(-v_ym + v_c, dy, rdy);
-  const double duydy_yp = div_cr(-v_c + v_yp, dy, rdy);
-  const double c_xm = tb.cT_xm[i - 1], c_xp = tb.cT_xp[i - 1];
-  const double ux_xm = 0.5 * ux[ux_idx(g, i, j - 1)] + 0.5 * ux[ux_idx(g, i, j)];
-  const double ux_xp = 0.5 * ux[ux_idx(g, i + 1, j - 1)] + 0.5 * ux[ux_idx(g, i + 1, j)];
-  const double adv1 = -c_xm * ux_xm * duydx_xm - c_xp * ux_xp * duydx_xp;
-  const double uy_ym = 0.5 * v_ym + 0.5 * v_c;
-  const double uy_yp = 0.5 * v_c + 0.5 * v_yp;
+  const double duydx_xm = div_cr(-v.v_xm + v.v_c, x.dxc, x.rdxc);
+  const double duydx_xp = div_cr(-v.v_c + v.v_xp, x.dxc_p, x.rdxc_p);
+  const double duydy_ym = div_cr(-v.v_ym + v.v_c, dy, rdy);
+  const double duydy_yp = div_cr(-v.v_c + v.v_yp, dy, rdy);
+  const double ux_xm = 0.5 * v.ux_mc + 0.5 * v.ux_cc;
+  const double ux_xp = 0.5 * v.ux_mp + 0.5 * v.ux_cp;
+  const double adv1 = -x.c_xm * ux_xm * duydx_xm - x.c_xp * ux_xp * duydx_xp;
+  const double uy_ym = 0.5 * v.v_ym + 0.5 * v.v_c;
+  const double uy_yp = 0.5 * v.v_c + 0.5 * v.v_yp;
   const double adv2 = -0.5 * uy_ym * duydy_ym - 0.5 * uy_yp * duydy_yp;
-  const double dif1 = div_cr(g.kappa_u * (duydx_xp - duydx_xm), tb.dxf[i - 1], tb.rdxf[i - 1]);
+  const double dif1 = div_cr(g.kappa_u * (duydx_xp - duydx_xm), x.dxf, x.rdxf);
   const double dif2 = div_cr(g.kappa_u * (duydy_yp - duydy_ym), dy, rdy);
-  const double pre = div_cr(-(p[c_idx(g, i, j)] - p[c_idx(g, i, j - 1)]), dy, rdy);
+  const double pre = div_cr(-(v.p_c - v.p_ym), dy, rdy);
   srca = +(adv1 + adv2) + wex * dif1 + wey * dif2;
   srcg = +wix * dif1 + wiy * dif2 + pre;
+}
+
+// ------------------------------------------------ loaders (pointer form)
+
+__device__ __forceinline__ CellX cell_x(const Tables &tb, int i) {
+  return {ldr(tb.dxc, i - 1), ldr(tb.rdxc, i - 1), ldr(tb.dxc, i),   ldr(tb.rdxc, i),
+          ldr(tb.dxf, i - 1), ldr(tb.rdxf, i - 1), ldr(tb.cT_xm, i - 1), ldr(tb.cT_xp, i - 1)};
+}
+__device__ __forceinline__ FaceX face_x(const Tables &tb, int i) {
+  return {ldr(tb.dxf, i - 2),   ldr(tb.rdxf, i - 2),  ldr(tb.dxf, i - 1), ldr(tb.rdxf, i - 1),
+          ldr(tb.cU_xm, i - 1), ldr(tb.cU_xp, i - 1), ldr(tb.dxc, i - 1), ldr(tb.rdxc, i - 1)};
+}
+
+__device__ __forceinline__ void temp_src_t(const Geo &g, const Tables &tb, const Rows3 &ux,
+                                           const Rows3 &uy, const Rows3 &temp, int i, double wex,
+                                           double wey, double wix, double wiy, double &srca,
+                                           double &srcg) {
+  const TempPt v = {ldr(temp.c, i), ldr(temp.c, i - 1), ldr(temp.c, i + 1), ldr(temp.m, i),
+                    ldr(temp.p, i), ldr(ux.c, i - 1),   ldr(ux.c, i),       ldr(uy.c, i),
+                    ldr(uy.p, i)};
+  temp_src_v(g, tb, cell_x(tb, i), v, wex, wey, wix, wiy, srca, srcg);
+}
+
+__device__ __forceinline__ void ux_src_t(const Geo &g, const Tables &tb, const Rows3 &ux,
+                                         const Rows3 &uy, const Rows3 &p, double tf, int i,
+                                         double wex, double wey, double wix, double wiy,
+                                         double &srca, double &srcg) {
+  const UxPt v = {ldr(ux.c, i - 1), ldr(ux.c, i - 2), ldr(ux.c, i), ldr(ux.m, i - 1),
+                  ldr(ux.p, i - 1), ldr(uy.c, i - 1), ldr(uy.c, i), ldr(uy.p, i - 1),
+                  ldr(uy.p, i),     ldr(p.c, i),      ldr(p.c, i - 1), tf};
+  ux_src_v(g, tb, face_x(tb, i), v, wex, wey, wix, wiy, srca, srcg);
+}
+
+__device__ __forceinline__ void uy_src_t(const Geo &g, const Tables &tb, const Rows3 &ux,
+                                         const Rows3 &uy, const Rows3 &p, int i, double wex,
+                                         double wey, double wix, double wiy, double &srca,
+                                         double &srcg) {
+  const UyPt v = {ldr(uy.c, i),     ldr(uy.c, i - 1), ldr(uy.c, i + 1), ldr(uy.m, i),
+                  ldr(uy.p, i),     ldr(ux.m, i - 1), ldr(ux.c, i - 1), ldr(ux.m, i),
+                  ldr(ux.c, i),     ldr(p.c, i),      ldr(p.m, i)};
+  uy_src_v(g, tb, cell_x(tb, i), v, wex, wey, wix, wiy, srca, srcg);
+}
+
+// ---------------------------------------------------------------- stores
+
+// Value of ux at face (i, j) (j uniform across the workgroup) with its
+// periodic copy and the wall values of the row.
+__device__ __forceinline__ void put_ux(const Geo &g, double *f, int i, int j, double v) {
+  const size_t w = (size_t)(g.itot + 1);
+  double *row = f + (size_t)j * w;
+  double *halo = j == 1 ? f + (size_t)(g.jtot + 1) * w : f;  // used when j is 1 or jtot
+  const bool copy = j == 1 || j == g.jtot;
+  str(row, i - 1, v);
+  if (copy) str(halo, i - 1, v);
+  if (i == 2 || i == g.itot) {
+    const int kw = (i == 2) ? 0 : g.itot;
+    str(row, kw, 0.);
+    if (copy) str(halo, kw, 0.);
+  }
+}
+
+// Value at cell (i, j) of uy / p / T (j uniform across the workgroup), with
+// its periodic copy and wall values.  WALL: 0 zero walls (uy), 1 Neumann
+// copies (p), 2 temperature walls (rows 1..jtot only).
+template <int WALL>
+__device__ __forceinline__ void put_c(const Geo &g, double *f, int i, int j, double v) {
+  const size_t w = (size_t)(g.itot + 2);
+  double *row = f + (size_t)j * w;
+  double *halo = j == 1 ? f + (size_t)(g.jtot + 1) * w : f;
+  const bool copy = j == 1 || j == g.jtot;
+  str(row, i, v);
+  if (copy) str(halo, i, v);
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    const bool hit = side == 0 ? (i == 1) : (i == g.itot);
+    if (!hit) continue;
+    const int iw = side == 0 ? 0 : g.itot + 1;
+    const double wv = WALL == 0 ? 0. : (WALL == 1 ? v : (side == 0 ? +0.5 : -0.5));
+    str(row, iw, wv);
+    if (WALL != 2 && copy) str(halo, iw, wv);
+  }
 }
 
 // Block index -> work index so that consecutive work items run on the same

@@ -32,7 +32,7 @@ struct CorrArgs {
 };
 
 // Thomas factorisation tables of the three CN-x operators for the three
-// substages: out holds 9 groups (substage s, field f at (s*3+f)) of 5 tables
+// substages: out holds 9 groups (substage s, field f at (s*3+f)) of 3 tables
 // of ntm entries each (see sns_lu.h)
 struct LuFactorArgs {
   Geo g;
@@ -47,6 +47,8 @@ bool pred_impx_supported(int itot);
 int lu_table_entries(int itot);
 void launch_lu_factor(hipStream_t s, const LuFactorArgs &a);
 void launch_pred(hipStream_t s, const PredArgs &a, int implicitx, const double *lu, int ntm);
+// implicit-x predictor (sns_pred.hip); lu: the 3 field groups of one substage
+void launch_pred_rows(hipStream_t s, const PredArgs &a, const double *lu, int ntm);
 void launch_add_out(hipStream_t s, const Geo &g, int kind, const double *in, const double *q,
                     double *out);
 int corr_partials(const Geo &g);

@@ -12,7 +12,7 @@
 // evaluated in parallel: every thread runs them over its chunk of M rows with
 // zero carries, the chunk-end values are combined by an affine scan across
 // the workgroup (wave shuffles, then one value per wave), and each element
-// is corrected with the precomputed products of al (P) and be (Q) across its
+// is corrected with the running products of al (P) and be (Q) across its
 // chunk.  Rounding differs from the sequential sweep at the ulp level.
 //
 // Tables are stored interleaved, entry (t, k) of row i = t*M + k at
@@ -23,7 +23,7 @@
 namespace sns {
 
 struct LuTab {
-  const double *r, *al, *be, *P, *Q;  // each NT*M entries, interleaved
+  const double *r, *al, *be;  // each NT*M entries, interleaved
 };
 
 // composition of affine maps x -> A x + B: (A2,B2) after (A1,B1)
@@ -78,51 +78,66 @@ __device__ __forceinline__ void affine_scan_excl(double (&A)[NF], double (&B)[NF
   __syncthreads();
 }
 
-// Solve NF systems (one per field) for this thread's chunk d[f][0..M-1]
-// (rows t*M .. t*M+M-1); tab[f].r == nullptr skips field f.
-template <int NT, int M, int NF>
-__device__ __forceinline__ void lu_solve(double (&d)[NF][M], const LuTab (&tab)[NF], double *scr) {
+// Solve NS systems that share one matrix (tab) for this thread's chunk
+// d[s][0..M-1] (unknowns t*M .. t*M+M-1).
+template <int NT, int M, int NS>
+__device__ __forceinline__ void lu_solve_shared(double (&d)[NS][M], const LuTab &tab,
+                                                double *scr) {
   const int t = threadIdx.x;
-  double A[NF], B[NF], C[NF];
-  // forward recurrences with zero carry in
+  double A[NS], B[NS], C[NS];
+  double P[M];
+  {
+    double y[NS];
 #pragma unroll
-  for (int f = 0; f < NF; f++) {
-    A[f] = 0.;
-    B[f] = 0.;
-    if (!tab[f].r) continue;
-    double y = 0.;
+    for (int s = 0; s < NS; s++) y[s] = 0.;
+    double pr = 1.;
 #pragma unroll
     for (int k = 0; k < M; k++) {
-      const double r = tab[f].r[k * NT + t], al = tab[f].al[k * NT + t];
-      y = r * d[f][k] + al * y;
-      d[f][k] = y;
+      const double r = tab.r[k * NT + t], al = tab.al[k * NT + t];
+      pr *= al;
+      P[k] = pr;
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        y[s] = r * d[s][k] + al * y[s];
+        d[s][k] = y[s];
+      }
     }
-    A[f] = tab[f].P[(M - 1) * NT + t];
-    B[f] = y;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      A[s] = P[M - 1];
+      B[s] = y[s];
+    }
   }
-  affine_scan_excl<NT, NF, false>(A, B, C, scr);
+  affine_scan_excl<NT, NS, false>(A, B, C, scr);
+  double Q[M];
+  {
+    double x[NS];
 #pragma unroll
-  for (int f = 0; f < NF; f++) {
-    if (!tab[f].r) continue;
-#pragma unroll
-    for (int k = 0; k < M; k++) d[f][k] += tab[f].P[k * NT + t] * C[f];
-    // backward recurrences with zero carry in
-    double x = 0.;
+    for (int s = 0; s < NS; s++) x[s] = 0.;
+    double pr = 1.;
 #pragma unroll
     for (int k = M - 1; k >= 0; k--) {
-      x = d[f][k] + tab[f].be[k * NT + t] * x;
-      d[f][k] = x;
+      const double be = tab.be[k * NT + t];
+      pr *= be;
+      Q[k] = pr;
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        d[s][k] += P[k] * C[s];
+        x[s] = d[s][k] + be * x[s];
+        d[s][k] = x[s];
+      }
     }
-    A[f] = tab[f].Q[t];
-    B[f] = x;
-  }
-  affine_scan_excl<NT, NF, true>(A, B, C, scr);
 #pragma unroll
-  for (int f = 0; f < NF; f++) {
-    if (!tab[f].r) continue;
-#pragma unroll
-    for (int k = 0; k < M; k++) d[f][k] += tab[f].Q[k * NT + t] * C[f];
+    for (int s = 0; s < NS; s++) {
+      A[s] = Q[0];
+      B[s] = x[s];
+    }
   }
+  affine_scan_excl<NT, NS, true>(A, B, C, scr);
+#pragma unroll
+  for (int s = 0; s < NS; s++)
+#pragma unroll
+    for (int k = 0; k < M; k++) d[s][k] += Q[k] * C[s];
 }
 
 }  // namespace sns

@@ -16,13 +16,20 @@ struct Fft2Plan {
 
 bool fft2_plan(int n, int itot, Fft2Plan &pl);
 
+// Thomas factors of the x systems of all wavenumbers (once per instance):
+// fac holds (jtot/2+1) groups of 3 tables of poisson_factor_entries(itot)
+// entries (sns_lu.h layout)
+int poisson_factor_entries(int itot);
+void launch_poisson_factor(hipStream_t s, const Geo &g, const Tables &tb, const double *eigen,
+                           double *fac);
+
 // stage 0: RHS + forward transform -> psihat (x-aligned spectrum)
-// stage 1: tri-diagonal solves along x, one per wavenumber
+// stage 1: tri-diagonal solves along x, one per wavenumber (factors in fac)
 // stage 2: inverse transform -> psi with boundary values
 // stage < 0: all three
 void launch_poisson2(hipStream_t s, const Geo &g, const Tables &tb, const Fft2Plan &pl,
                      double gamma, const double *dtp, const double *ux, const double *uy,
-                     const double2 *tw, const double *eigen, double2 *Y, double2 *psihat,
+                     const double2 *tw, const double *fac, double2 *Y, double2 *psihat,
                      double *psi, int stage);
 
 }  // namespace sns

@@ -15,11 +15,12 @@
 // columns are packed as one complex sequence, and the spectra of the pair are
 // separated (forward) or combined (inverse) in pass B / B', which therefore
 // handles the wavenumber rows k1 and n1-k1 together.
+#include <float.h>
 #include "sns_launch.h"
 #include "sns_fast.h"
 #include "sns_fft.h"
 #include "sns_poisson.h"
-#include "sns_tridiag.h"
+#include "sns_lu.h"
 
 namespace sns {
 
@@ -213,52 +214,87 @@ __global__ void __launch_bounds__(NTF)
 
 // --------------------------------------------------- x solve per wavenumber
 
-// One workgroup per wavenumber row; the row is staged through LDS (coalesced
-// global access, chunk order for the solver) and the same LDS then holds the
-// reduced system.  Matrix: src/fluid/compute_potential.c:117-127.  The J = 0
-// system is singular; its last equation is replaced by psi_hat(itot) = 0.
-template <int NT, int M>
-struct PxLds {
-  union {
-    double2 row[NT * (M + 1)];
-    TridiagLds<NT, 2> s;
+// Matrix of wavenumber J (src/fluid/compute_potential.c:117-127): l_i =
+// 1/(DXC(i)DXF(i)), u_i = 1/(DXC(i+1)DXF(i)), c_i = -l_i - u_i + lambda_J with
+// the Neumann walls folded into c_0 and c_{n-1}.  It does not change during
+// a run, so its Thomas factorisation (src/linalg.c:32-50: u'_i and the
+// reciprocal pivots, in the reference's sequential order) is computed once
+// per instance by k_poisson_factor, one thread per wavenumber, and stored as
+// the interleaved tables of sns_lu.h (r = 1/pivot, al = -r l, be = -u').
+// The J = 0 system is singular; its last equation is replaced by
+// psi_hat(itot) = 0 (r = al = 0 on that row).  Each solve then needs only
+// the two substitutions, evaluated by lu_solve_shared for the real and
+// imaginary parts together; with the factors exact, the substitutions keep
+// the accuracy of the sequential sweep on the nearly singular low-J systems.
+__device__ __forceinline__ int lu_slot(int i, int NT, int M) { return (i % M) * NT + i / M; }
+
+__global__ void __launch_bounds__(64)
+    k_poisson_factor(Geo g, Tables tb, const double *__restrict__ eigen, int NT, int M, int ntm,
+                     double *__restrict__ fac) {
+  const int J = blockIdx.x * blockDim.x + threadIdx.x;
+  if (J > g.jtot / 2) return;
+  const int n = g.itot;
+  const double lam = eigen[J];
+  double *R = fac + (size_t)J * 3 * ntm, *AL = R + ntm, *BE = R + 2 * ntm;
+  auto coef = [&](int i, double &l, double &c, double &u) {
+    l = tb.pl[i];
+    u = tb.pu[i];
+    c = -l - u + lam;
+    if (i == 0) c += l;
+    if (i == n - 1) c += u;
   };
-};
+  double l, c, u;
+  coef(0, l, c, u);
+  double up = u / c;  // u'_0
+  double r = 1. / c;
+  R[lu_slot(0, NT, M)] = r;
+  AL[lu_slot(0, NT, M)] = 0.;
+  for (int i = 1; i < n; i++) {
+    BE[lu_slot(i - 1, NT, M)] = -up;
+    coef(i, l, c, u);
+    const double val = c - l * up;
+    if (fabs(val) < DBL_EPSILON || (J == 0 && i == n - 1)) {
+      up = 0.;
+      r = 0.;
+    } else {
+      r = 1. / val;
+      up = r * u;
+    }
+    R[lu_slot(i, NT, M)] = r;
+    AL[lu_slot(i, NT, M)] = -(r * l);
+  }
+  BE[lu_slot(n - 1, NT, M)] = 0.;
+}
 
 template <int NT, int M>
+struct PxLds {
+  double2 row[NT * (M + 1)];
+  double scr[2 * 2 * (NT / 64)];
+};
+
+// One workgroup per wavenumber row; the row is staged through LDS (coalesced
+// global access, chunk order for the substitutions).
+template <int NT, int M>
 __global__ void __launch_bounds__(NT)
-    k_poisson_x2(Geo g, Tables tb, const double *__restrict__ eigen, double2 *__restrict__ psihat) {
+    k_poisson_x3(Geo g, const double *__restrict__ fac, int ntm, double2 *__restrict__ psihat) {
   extern __shared__ double lds_raw[];
   PxLds<NT, M> &L = *reinterpret_cast<PxLds<NT, M> *>(lds_raw);
-  const int J = blockIdx.x;
+  const int J = xcd_remap(blockIdx.x, gridDim.x);
   const int n = g.itot;
   double2 *row = psihat + (size_t)J * n;
   for (int r = threadIdx.x; r < n; r += NT) L.row[r + r / M] = row[r];
   __syncthreads();
-  const double lam = eigen[J];
-  double a[M], b[M], c[M], d[2][M];
+  double d[2][M];
 #pragma unroll
   for (int k = 0; k < M; k++) {
     const int r = threadIdx.x * M + k;
-    if (r < n) {
-      const double l = tb.pl[r], u = tb.pu[r];
-      double cc = -l - u + lam;
-      if (r == 0) cc += l;
-      if (r == n - 1) cc += u;
-      a[k] = (r == 0) ? 0. : l;
-      b[k] = cc;
-      c[k] = (r == n - 1) ? 0. : u;
-      const double2 v = L.row[r + r / M];
-      d[0][k] = v.x;
-      d[1][k] = v.y;
-      if (J == 0 && r == n - 1) { a[k] = 0.; b[k] = 1.; d[0][k] = 0.; d[1][k] = 0.; }
-    } else {
-      a[k] = 0.; b[k] = 1.; c[k] = 0.; d[0][k] = 0.; d[1][k] = 0.;
-    }
+    const double2 v = r < n ? L.row[r + r / M] : make_double2(0., 0.);
+    d[0][k] = v.x;
+    d[1][k] = v.y;
   }
-  __syncthreads();  // the row buffer becomes the reduced-system storage
-  block_tridiag<NT, M, 2>(a, b, c, d, L.s);
-  __syncthreads();
+  const double *R = fac + (size_t)J * 3 * ntm;
+  const LuTab tab = {R, R + ntm, R + 2 * ntm};
+  lu_solve_shared<NT, M, 2>(d, tab, L.scr);
 #pragma unroll
   for (int k = 0; k < M; k++) {
     const int r = threadIdx.x * M + k;
@@ -268,19 +304,37 @@ __global__ void __launch_bounds__(NT)
   for (int r = threadIdx.x; r < n; r += NT) row[r] = L.row[r + r / M];
 }
 
+static constexpr int kPxM = 8;
+// threads of the x solve for n = itot unknowns (NT*kPxM >= n)
+static int px_threads(int n) {
+  if (n <= 64 * kPxM) return 64;
+  if (n <= 128 * kPxM) return 128;
+  if (n <= 256 * kPxM) return 256;
+  if (n <= 512 * kPxM) return 512;
+  return 1024;
+}
+
+int poisson_factor_entries(int itot) { return px_threads(itot) * kPxM; }
+
+void launch_poisson_factor(hipStream_t s, const Geo &g, const Tables &tb, const double *eigen,
+                           double *fac) {
+  const int NT = px_threads(g.itot), ntm = NT * kPxM;
+  const int nJ = g.jtot / 2 + 1;
+  hipLaunchKernelGGL(k_poisson_factor, dim3((nJ + 63) / 64), dim3(64), 0, s, g, tb, eigen, NT,
+                     kPxM, ntm, fac);
+}
+
 template <int NT, int M>
-static void px_launch(hipStream_t s, const Geo &g, const Tables &tb, const double *eigen,
-                      double2 *psihat) {
+static void px_launch(hipStream_t s, const Geo &g, const double *fac, double2 *psihat) {
   const size_t lds = sizeof(PxLds<NT, M>);
   static bool attr = false;
   if (!attr) {
-    if (lds > 65536)
-      hipFuncSetAttribute((const void *)k_poisson_x2<NT, M>,
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipFuncSetAttribute((const void *)k_poisson_x3<NT, M>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr = true;
   }
-  hipLaunchKernelGGL((k_poisson_x2<NT, M>), dim3(g.jtot / 2 + 1), dim3(NT), lds, s, g, tb, eigen,
-                     psihat);
+  hipLaunchKernelGGL((k_poisson_x3<NT, M>), dim3(g.jtot / 2 + 1), dim3(NT), lds, s, g, fac,
+                     NT * M, psihat);
 }
 
 // ------------------------------------------------------------- plan, launch
@@ -335,19 +389,20 @@ static dim3 grid_b(const Fft2Plan &pl) { return dim3((pl.npair + TILE - 1) / TIL
 
 void launch_poisson2(hipStream_t s, const Geo &g, const Tables &tb, const Fft2Plan &pl,
                      double gamma, const double *dtp, const double *ux, const double *uy,
-                     const double2 *tw, const double *eigen, double2 *Y, double2 *psihat,
+                     const double2 *tw, const double *fac, double2 *Y, double2 *psihat,
                      double *psi, int stage) {
   if (stage == 0 || stage < 0) {
     hipLaunchKernelGGL(k_pfwd_a, grid_a(pl), dim3(NTF), 0, s, g, tb, pl, gamma, dtp, ux, uy, tw, Y);
     hipLaunchKernelGGL(k_pfwd_b, grid_b(pl), dim3(NTF), 0, s, g, pl, tw, Y, psihat);
   }
   if (stage == 1 || stage < 0) {
-    const int n = g.itot;
-    if (n <= 64 * 4) px_launch<64, 4>(s, g, tb, eigen, psihat);
-    else if (n <= 256 * 4) px_launch<256, 4>(s, g, tb, eigen, psihat);
-    else if (n <= 512 * 4) px_launch<512, 4>(s, g, tb, eigen, psihat);
-    else if (n <= 512 * 8) px_launch<512, 8>(s, g, tb, eigen, psihat);
-    else px_launch<512, 16>(s, g, tb, eigen, psihat);
+    switch (px_threads(g.itot)) {
+      case 64: px_launch<64, kPxM>(s, g, fac, psihat); break;
+      case 128: px_launch<128, kPxM>(s, g, fac, psihat); break;
+      case 256: px_launch<256, kPxM>(s, g, fac, psihat); break;
+      case 512: px_launch<512, kPxM>(s, g, fac, psihat); break;
+      default: px_launch<1024, kPxM>(s, g, fac, psihat); break;
+    }
   }
   if (stage == 2 || stage < 0) {
     hipLaunchKernelGGL(k_pinv_b, grid_b(pl), dim3(NTF), 0, s, g, pl, tw, psihat, Y);

@@ -1,0 +1,427 @@
+// sns_pred.hip -- implicit-x predictor: shared LU factorisation of the
+// Crank-Nicolson x operators and the row kernel (gfx950).
+//
+// k_pred_rows evaluates, for one field and two adjacent rows per workgroup,
+// the source terms and RK increments of src/temperature/update.c:11-122 and
+// src/fluid/update_velocity.c:11-229, then solves the Crank-Nicolson system
+// in x of both rows (src/temperature/update.c:124-156,
+// src/fluid/update_velocity.c:231-270, 381-414) with the shared LU tables.
+//
+// Memory traffic is set by the number of load instructions, so neighbours
+// are not loaded twice:
+//  * x neighbours come from the adjacent lane (DPP wave shifts); each wave
+//    covers 62 columns and its first and last lanes only hold the halo
+//    values of the columns on either side;
+//  * the y neighbours of one row are the centre values of the other, and the
+//    per-column metrics and the LU tables serve both rows.
+#include "sns_launch.h"
+#include "sns_fast.h"
+#include "sns_fused.h"
+#include "sns_lu.h"
+
+namespace sns {
+
+// ------------------------------------------ shared LU factorisation of CN-x
+
+// Coefficients of row r (0-based unknown index) of the CN-x operator of
+// field f (0: T, 1: ux, 2: uy): src/temperature/update.c:141-149,
+// src/fluid/update_velocity.c:253-261,398-406.
+__device__ __forceinline__ void cnx_row(const Geo &g, const Tables &tb, int f, double pf, int r,
+                                        double &l, double &c, double &u) {
+  const int i = r + (f == 1 ? 2 : 1);
+  const double coef_xm = f == 1 ? tb.nU_xm[i - 1] : tb.nT_xm[i - 1];
+  const double coef_xp = f == 1 ? tb.nU_xp[i - 1] : tb.nT_xp[i - 1];
+  const double coef_x0 = -coef_xm - coef_xp;
+  l = -pf * coef_xm;
+  u = -pf * coef_xp;
+  c = 1. - pf * coef_x0;
+}
+
+__device__ __forceinline__ double cn_pf(const Geo &g, int f, double gdt) {
+  return f == 0 ? gdt / g.two_sqrtPr_sqrtRa : (gdt * g.sqrtPr) / g.two_sqrtRa;
+}
+
+struct M22 {
+  double a, b, c, d;
+};
+__device__ __forceinline__ M22 mmul(const M22 &x, const M22 &y) {  // x*y
+  return {x.a * y.a + x.b * y.c, x.a * y.b + x.b * y.d, x.c * y.a + x.d * y.c,
+          x.c * y.b + x.d * y.d};
+}
+// exact power-of-two rescaling (ratios of entries are unchanged)
+__device__ __forceinline__ M22 mnorm(M22 m) {
+  const double mx = fmax(fmax(fabs(m.a), fabs(m.b)), fmax(fabs(m.c), fabs(m.d)));
+  if (mx == 0. || !isfinite(mx)) return m;
+  const int e = ilogb(mx);
+  return {ldexp(m.a, -e), ldexp(m.b, -e), ldexp(m.c, -e), ldexp(m.d, -e)};
+}
+__device__ __forceinline__ M22 mshfl_up(const M22 &m, int off) {
+  return {__shfl_up(m.a, off), __shfl_up(m.b, off), __shfl_up(m.c, off), __shfl_up(m.d, off)};
+}
+
+// Thomas factorisation of the CN-x operator of field blockIdx.x for the
+// substage blockIdx.y, written as the interleaved tables of sns_lu.h.  The
+// pivot recurrence u'_i = u_i/(c_i - l_i u'_{i-1}) is the Moebius map of the
+// vector v_i = [[0, u_i], [-l_i, c_i]] v_{i-1}, v_0 = (u_0, c_0), with
+// u'_i = p_i/q_i and 1/pivot_i = q_{i-1}/q_i; the products of these 2x2
+// matrices are scanned across the workgroup.
+template <int NT, int M>
+__global__ void __launch_bounds__(NT) k_lu_factor(LuFactorArgs a) {
+  constexpr int NW = NT / 64;
+  __shared__ double wtot[NW][4];
+  const Geo &g = a.g;
+  const int f = blockIdx.x, s = blockIdx.y;
+  const int n = f == 1 ? g.itot - 1 : g.itot;
+  const double pf = cn_pf(g, f, a.gamma[s] * (*a.dtp));
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  // chunk product of the row matrices (rows >= 1)
+  M22 T = {1., 0., 0., 1.};
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const int r = t * M + k;
+    if (r >= 1 && r < n) {
+      double l, c, u;
+      cnx_row(g, a.tb, f, pf, r, l, c, u);
+      T = mnorm(mmul(M22{0., u, -l, c}, T));
+    }
+  }
+  // inclusive scan within the wave (later chunks multiply from the left)
+  M22 X = T;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const M22 o = mshfl_up(X, off);
+    if (lane >= off) X = mnorm(mmul(X, o));
+  }
+  if (lane == 63) {
+    wtot[wave][0] = X.a; wtot[wave][1] = X.b; wtot[wave][2] = X.c; wtot[wave][3] = X.d;
+  }
+  __syncthreads();
+  M22 cin = {1., 0., 0., 1.};
+  for (int w = 0; w < wave; w++)
+    cin = mnorm(mmul(M22{wtot[w][0], wtot[w][1], wtot[w][2], wtot[w][3]}, cin));
+  const M22 incl = mnorm(mmul(X, cin));
+  M22 excl = mshfl_up(incl, 1);
+  if (lane == 0) excl = cin;
+  // v before this chunk's first row
+  double l0, c0, u0;
+  cnx_row(g, a.tb, f, pf, 0, l0, c0, u0);
+  double p = excl.a * u0 + excl.b * c0, q = excl.c * u0 + excl.d * c0;
+  double rr[M], al[M], be[M];
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const int r = t * M + k;
+    rr[k] = al[k] = be[k] = 0.;
+    if (r >= n) continue;
+    double l, c, u;
+    cnx_row(g, a.tb, f, pf, r, l, c, u);
+    double up;
+    if (r == 0) {
+      rr[k] = 1. / c;
+      up = u / c;
+      p = u0;
+      q = c0;
+    } else {
+      const double pn = u * q, qn = -l * p + c * q;
+      rr[k] = q / qn;
+      up = pn / qn;
+      al[k] = -rr[k] * l;
+      const double mx = fmax(fabs(pn), fabs(qn));
+      const int e = ilogb(mx);
+      p = ldexp(pn, -e);
+      q = ldexp(qn, -e);
+    }
+    be[k] = (r == n - 1) ? 0. : -up;
+  }
+  double *base = a.out + (size_t)((s * 3 + f) * 3) * a.ntm;
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    base[0 * a.ntm + k * NT + t] = rr[k];
+    base[1 * a.ntm + k * NT + t] = al[k];
+    base[2 * a.ntm + k * NT + t] = be[k];
+  }
+}
+
+// ---------------------------------------------------------- lane shifts
+
+// Value of v in lane l-1 (lane 0: 0).
+__device__ __forceinline__ double from_left(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), 0x138, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// Value of v in lane l+1 (lane 63: 0).
+__device__ __forceinline__ double from_right(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), 0x130, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// -------------------------------------------------------- row predictor
+
+template <int NT, int M>
+struct PredRowsLds {
+  double d[2][NT * (M + 1)];  // padded: chunk t starts at t*(M+1)
+  double scr[2 * 2 * (NT / 64)];
+};
+
+template <int M>
+__device__ __forceinline__ int pad_idx(int r) {
+  return r + r / M;
+}
+
+constexpr int kWaveCols = 62;  // output columns per wave (lanes 1..62)
+
+// Workgroup w: field w % nfld (+1 without the temperature equation) of rows
+// a = 2q+1 and b = a+1, q = w / nfld.  For odd jtot the last pair is shifted
+// down to (jtot-1, jtot) and writes row b only.  Consecutive w share an XCD.
+template <int NT, int M>
+__global__ void __launch_bounds__(NT) k_pred_rows(PredArgs a, const double *lu, int ntm, int nfld) {
+  extern __shared__ double lds_raw[];
+  PredRowsLds<NT, M> &L = *reinterpret_cast<PredRowsLds<NT, M> *>(lds_raw);
+  constexpr int NW = NT / 64;
+  const Geo &g = a.g;
+  const Tables &tb = a.tb;
+  const int itot = g.itot;
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  const int q = w / nfld;
+  const int f = w % nfld + (nfld == 3 ? 0 : 1);
+  const int ja = min(2 * q + 1, g.jtot - 1), jb = ja + 1;
+  const bool write_a = ja == 2 * q + 1;
+  const int n = f == 1 ? itot - 1 : itot;
+  const int i0 = f == 1 ? 2 : 1;
+  const double dt = *a.dtp;
+  const double wex = a.tr.w_ex_x, wey = a.tr.w_ex_y, wix = a.tr.w_im_x, wiy = a.tr.w_im_y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const size_t wc = (size_t)(itot + 2), wu = (size_t)(itot + 1);
+  const int nk = ((n + kWaveCols - 1) / kWaveCols + NW - 1) / NW;
+
+  // ---- stencils and RK increments of both rows into LDS (lane order)
+  for (int k = 0; k < nk; k++) {
+    const int ii = (k * NW + wave) * kWaveCols + lane - 1;  // unknown index of this lane
+    const int i = ii + i0;                                    // column
+    const bool out = lane >= 1 && lane <= kWaveCols && ii < n;
+    const int ci = clampi(i, 0, itot + 1);      // cell arrays, columns 0..itot+1
+    const int ui = clampi(i - 1, 0, itot);      // ux, faces 1..itot+1
+    const int ti = clampi(i - 1, 0, itot + 1);  // column tables, entry i-1
+    const int hi = clampi(ii, 0, n - 1);        // RK history, unknown index
+    double da = 0., db = 0.;
+    if (f == 0) {
+      const double *T = a.temp, *U = a.ux, *V = a.uy;
+      const double t_am = ldr(T + (ja - 1) * wc, ci), t_a = ldr(T + ja * wc, ci);
+      const double t_b = ldr(T + jb * wc, ci), t_bp = ldr(T + (jb + 1) * wc, ci);
+      const double u_a = ldr(U + ja * wu, ui), u_b = ldr(U + jb * wu, ui);
+      const double v_a = ldr(V + ja * wc, ci), v_b = ldr(V + jb * wc, ci);
+      const double v_bp = ldr(V + (jb + 1) * wc, ci);
+      CellX x;
+      x.dxc = ldr(tb.dxc, ti);
+      x.rdxc = ldr(tb.rdxc, ti);
+      x.dxf = ldr(tb.dxf, ti);
+      x.rdxf = ldr(tb.rdxf, ti);
+      x.c_xm = ldr(tb.cT_xm, ti);
+      x.c_xp = ldr(tb.cT_xp, ti);
+      x.dxc_p = from_right(x.dxc);
+      x.rdxc_p = from_right(x.rdxc);
+      const double *HA = a.hist_t + (size_t)(ja - 1) * n, *HB = a.hist_t + (size_t)(jb - 1) * n;
+      const double ba = a.read_hist ? ldr(HA, hi) : 0., bb = a.read_hist ? ldr(HB, hi) : 0.;
+      const TempPt pa = {t_a, from_left(t_a), from_right(t_a), t_am, t_b,
+                         u_a, from_right(u_a), v_a, v_b};
+      const TempPt pb = {t_b, from_left(t_b), from_right(t_b), t_a, t_bp,
+                         u_b, from_right(u_b), v_b, v_bp};
+      double A, G;
+      temp_src_v(g, tb, x, pa, wex, wey, wix, wiy, A, G);
+      if (a.write_hist && out && write_a) str((double *)HA, ii, A);
+      da = rk_incr(a.alpha, a.beta, a.gamma, dt, A, ba, G);
+      temp_src_v(g, tb, x, pb, wex, wey, wix, wiy, A, G);
+      if (a.write_hist && out) str((double *)HB, ii, A);
+      db = rk_incr(a.alpha, a.beta, a.gamma, dt, A, bb, G);
+    } else if (f == 1) {
+      const double *T = a.temp, *U = a.ux, *V = a.uy, *P = a.p;
+      const double u_am = ldr(U + (ja - 1) * wu, ui), u_a = ldr(U + ja * wu, ui);
+      const double u_b = ldr(U + jb * wu, ui), u_bp = ldr(U + (jb + 1) * wu, ui);
+      const double v_a = ldr(V + ja * wc, ci), v_b = ldr(V + jb * wc, ci);
+      const double v_bp = ldr(V + (jb + 1) * wc, ci);
+      const double p_a = ldr(P + ja * wc, ci), p_b = ldr(P + jb * wc, ci);
+      FaceX x;
+      x.dxf = ldr(tb.dxf, ti);
+      x.rdxf = ldr(tb.rdxf, ti);
+      x.c_xm = ldr(tb.cU_xm, ti);
+      x.c_xp = ldr(tb.cU_xp, ti);
+      x.dxc = ldr(tb.dxc, ti);
+      x.rdxc = ldr(tb.rdxc, ti);
+      x.dxf_m = from_left(x.dxf);
+      x.rdxf_m = from_left(x.rdxf);
+      double tf_a = 0., tf_b = 0.;
+      if (a.forcing) {
+        const double t_a = ldr(T + ja * wc, ci), t_b = ldr(T + jb * wc, ci);
+        tf_a = 0.5 * from_left(t_a) + 0.5 * t_a;
+        tf_b = 0.5 * from_left(t_b) + 0.5 * t_b;
+      }
+      const double *HA = a.hist_ux + (size_t)(ja - 1) * n, *HB = a.hist_ux + (size_t)(jb - 1) * n;
+      const double ba = a.read_hist ? ldr(HA, hi) : 0., bb = a.read_hist ? ldr(HB, hi) : 0.;
+      const double v_bm = from_left(v_b);
+      const UxPt pa = {u_a, from_left(u_a), from_right(u_a), u_am, u_b,
+                       from_left(v_a), v_a, v_bm, v_b, p_a, from_left(p_a), tf_a};
+      const UxPt pb = {u_b, from_left(u_b), from_right(u_b), u_a, u_bp,
+                       v_bm, v_b, from_left(v_bp), v_bp, p_b, from_left(p_b), tf_b};
+      double A, G;
+      ux_src_v(g, tb, x, pa, wex, wey, wix, wiy, A, G);
+      if (a.write_hist && out && write_a) str((double *)HA, ii, A);
+      da = rk_incr(a.alpha, a.beta, a.gamma, dt, A, ba, G);
+      ux_src_v(g, tb, x, pb, wex, wey, wix, wiy, A, G);
+      if (a.write_hist && out) str((double *)HB, ii, A);
+      db = rk_incr(a.alpha, a.beta, a.gamma, dt, A, bb, G);
+    } else {
+      const double *U = a.ux, *V = a.uy, *P = a.p;
+      const double v_am = ldr(V + (ja - 1) * wc, ci), v_a = ldr(V + ja * wc, ci);
+      const double v_b = ldr(V + jb * wc, ci), v_bp = ldr(V + (jb + 1) * wc, ci);
+      const double u_am = ldr(U + (ja - 1) * wu, ui), u_a = ldr(U + ja * wu, ui);
+      const double u_b = ldr(U + jb * wu, ui);
+      const double p_am = ldr(P + (ja - 1) * wc, ci), p_a = ldr(P + ja * wc, ci);
+      const double p_b = ldr(P + jb * wc, ci);
+      CellX x;
+      x.dxc = ldr(tb.dxc, ti);
+      x.rdxc = ldr(tb.rdxc, ti);
+      x.dxf = ldr(tb.dxf, ti);
+      x.rdxf = ldr(tb.rdxf, ti);
+      x.c_xm = ldr(tb.cT_xm, ti);
+      x.c_xp = ldr(tb.cT_xp, ti);
+      x.dxc_p = from_right(x.dxc);
+      x.rdxc_p = from_right(x.rdxc);
+      const double *HA = a.hist_uy + (size_t)(ja - 1) * n, *HB = a.hist_uy + (size_t)(jb - 1) * n;
+      const double ba = a.read_hist ? ldr(HA, hi) : 0., bb = a.read_hist ? ldr(HB, hi) : 0.;
+      const double u_ap = from_right(u_a);
+      const UyPt pa = {v_a, from_left(v_a), from_right(v_a), v_am, v_b,
+                       u_am, u_a, from_right(u_am), u_ap, p_a, p_am};
+      const UyPt pb = {v_b, from_left(v_b), from_right(v_b), v_a, v_bp,
+                       u_a, u_b, u_ap, from_right(u_b), p_b, p_a};
+      double A, G;
+      uy_src_v(g, tb, x, pa, wex, wey, wix, wiy, A, G);
+      if (a.write_hist && out && write_a) str((double *)HA, ii, A);
+      da = rk_incr(a.alpha, a.beta, a.gamma, dt, A, ba, G);
+      uy_src_v(g, tb, x, pb, wex, wey, wix, wiy, A, G);
+      if (a.write_hist && out) str((double *)HB, ii, A);
+      db = rk_incr(a.alpha, a.beta, a.gamma, dt, A, bb, G);
+    }
+    if (out) {
+      L.d[0][pad_idx<M>(ii)] = da;
+      L.d[1][pad_idx<M>(ii)] = db;
+    }
+  }
+  __syncthreads();
+
+  // ---- Crank-Nicolson in x of both rows (chunk order)
+  double dd[2][M];
+#pragma unroll
+  for (int s = 0; s < 2; s++)
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+      const int r = threadIdx.x * M + k;
+      dd[s][k] = r < n ? L.d[s][pad_idx<M>(r)] : 0.;
+    }
+  const double *b = lu + (size_t)(f * 3) * ntm;
+  const LuTab tab = {b, b + ntm, b + 2 * ntm};
+  lu_solve_shared<NT, M, 2>(dd, tab, L.scr);
+#pragma unroll
+  for (int s = 0; s < 2; s++)
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+      const int r = threadIdx.x * M + k;
+      if (r < n) L.d[s][pad_idx<M>(r)] = dd[s][k];
+    }
+  __syncthreads();
+
+  // ---- final values with boundary values, or the increments for a y solve
+  for (int ii = threadIdx.x; ii < n; ii += NT) {
+    const int i = ii + i0;
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+      if (s == 0 && !write_a) continue;
+      const int j = s == 0 ? ja : jb;
+      const double d = L.d[s][pad_idx<M>(ii)];
+      if (f == 0) {
+        if (a.q_t) str(a.q_t + (size_t)(j - 1) * n, ii, d);
+        else put_c<2>(g, a.out_t, i, j, ldr(a.temp + j * wc, i) + d);
+      } else if (f == 1) {
+        if (a.q_u) str(a.q_u + (size_t)(j - 1) * n, ii, d);
+        else put_ux(g, a.out_u, i, j, ldr(a.ux + j * wu, i - 1) + d);
+      } else {
+        if (a.q_v) str(a.q_v + (size_t)(j - 1) * n, ii, d);
+        else put_c<0>(g, a.out_v, i, j, ldr(a.uy + j * wc, i) + d);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- launch
+
+template <int NT, int M>
+static void pred_rows_launch(hipStream_t s, const PredArgs &a, const double *lu, int ntm) {
+  const size_t lds = sizeof(PredRowsLds<NT, M>);
+  static bool attr = false;
+  if (!attr) {
+    hipFuncSetAttribute((const void *)k_pred_rows<NT, M>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr = true;
+  }
+  const int nfld = a.with_temp ? 3 : 2;
+  const int npair = (a.g.jtot + 1) / 2;
+  hipLaunchKernelGGL((k_pred_rows<NT, M>), dim3(npair * nfld), dim3(NT), lds, s, a, lu, ntm, nfld);
+}
+
+template <int NT, int M>
+static void lu_factor_launch(hipStream_t s, const LuFactorArgs &a) {
+  hipLaunchKernelGGL((k_lu_factor<NT, M>), dim3(3, 3), dim3(NT), 0, s, a);
+}
+
+// (NT, M) of the row kernels for a given itot; ntm = NT*M >= itot
+static constexpr int kRowM = 8;
+static int row_config(int itot, int &NT, int &M) {
+  M = kRowM;
+  if (itot <= 64 * kRowM) NT = 64;
+  else if (itot <= 128 * kRowM) NT = 128;
+  else if (itot <= 256 * kRowM) NT = 256;
+  else if (itot <= 512 * kRowM) NT = 512;
+  else if (itot <= 1024 * kRowM) NT = 1024;
+  else return 0;
+  return NT * M;
+}
+
+bool pred_impx_supported(int itot) {
+  int NT, M;
+  return row_config(itot, NT, M) > 0;
+}
+
+int lu_table_entries(int itot) {
+  int NT, M;
+  return row_config(itot, NT, M);
+}
+
+void launch_lu_factor(hipStream_t s, const LuFactorArgs &a) {
+  int NT, M;
+  row_config(a.g.itot, NT, M);
+  switch (NT) {
+    case 64: lu_factor_launch<64, kRowM>(s, a); break;
+    case 128: lu_factor_launch<128, kRowM>(s, a); break;
+    case 256: lu_factor_launch<256, kRowM>(s, a); break;
+    case 512: lu_factor_launch<512, kRowM>(s, a); break;
+    default: lu_factor_launch<1024, kRowM>(s, a); break;
+  }
+}
+
+void launch_pred_rows(hipStream_t s, const PredArgs &a, const double *lu, int ntm) {
+  int NT, M;
+  row_config(a.g.itot, NT, M);
+  switch (NT) {
+    case 64: pred_rows_launch<64, kRowM>(s, a, lu, ntm); break;
+    case 128: pred_rows_launch<128, kRowM>(s, a, lu, ntm); break;
+    case 256: pred_rows_launch<256, kRowM>(s, a, lu, ntm); break;
+    case 512: pred_rows_launch<512, kRowM>(s, a, lu, ntm); break;
+    default: pred_rows_launch<1024, kRowM>(s, a, lu, ntm); break;
+  }
+}
+
+}  // namespace sns

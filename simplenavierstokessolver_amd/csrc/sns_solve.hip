@@ -223,7 +223,8 @@ static void cny_dispatch(hipStream_t s, const Geo &g, RK rk, const double *dtp, 
   else if (m <= 256 * 4) cny_launch<KIND, 256, 4>(s, g, rk, dtp, nrows, qy);
   else if (m <= 256 * 8) cny_launch<KIND, 256, 8>(s, g, rk, dtp, nrows, qy);
   else if (m <= 256 * 16) cny_launch<KIND, 256, 16>(s, g, rk, dtp, nrows, qy);
-  else cny_launch<KIND, 256, 32>(s, g, rk, dtp, nrows, qy);
+  else if (m <= 256 * 32) cny_launch<KIND, 256, 32>(s, g, rk, dtp, nrows, qy);
+  else cny_launch<KIND, 512, 32>(s, g, rk, dtp, nrows, qy);  // jtot <= 16384 (snsg_create)
 }
 
 void launch_cny_rows(hipStream_t s, const Geo &g, int kind, RK rk, const double *dtp, int nrows,

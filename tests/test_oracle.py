@@ -479,3 +479,58 @@ def test_update_pressure_closed_forms(imx, imy):
         cols = slice(1, it - 1)          # Neumann ghosts change the x stencil at the walls
         assert np.allclose(got[:, cols], (expect * np.ones((jt, it)))[:, cols], rtol=0,
                            atol=1e-10), (imx, imy)
+
+
+# -------------------------------------------- alternative operation orders
+
+@pytest.fixture
+def variant():
+    def set_(bits):
+        po.set_variant(bits)
+    yield set_
+    po.set_variant(0)
+
+
+def test_reversed_solve_matches_dense(variant):
+    """The reversed elimination order solves the same systems."""
+    rng = np.random.default_rng(7)
+    n = 40
+    l, u = rng.uniform(-1, 0, n), rng.uniform(-1, 0, n)
+    c = 2.5 + rng.uniform(0, 1, n)
+    q = rng.standard_normal(n)
+    A = np.diag(c) + np.diag(l[1:], -1) + np.diag(u[:-1], 1)
+    variant(po.VARIANT_REVERSED_SOLVE)
+    x = po.dgtsv_b(l, c, u, q)
+    assert np.allclose(x, np.linalg.solve(A, q), rtol=0, atol=1e-13)
+
+
+def potential_state(it=32, jt=24, seed=3):
+    o = po.Oracle(it, jt, stretch=3.0)
+    smooth_divfree_velocity(o, np.random.default_rng(seed), 0.02)
+    o.f["ux"][1:-1, 1:it] += 1e-3 * np.random.default_rng(seed + 1).standard_normal((jt, it - 1))
+    o.apply_boundaries()
+    o.dt = 1e-2
+    return o
+
+
+def test_direct_dft_variant_same_potential(variant):
+    o = potential_state()
+    o.compute_potential(0)
+    ref = o.f["psi"][1:-1, 1:-1].copy()
+    variant(po.VARIANT_DIRECT_DFT)
+    o.compute_potential(0)
+    got = o.f["psi"][1:-1, 1:-1]
+    assert np.abs((got - got.mean()) - (ref - ref.mean())).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_pinned_gauge_variant(variant):
+    """Pinned J = 0 closure: the y-sum of psi vanishes in the last column and
+    psi differs from the reference closure by a constant only (to rounding)."""
+    o = potential_state()
+    o.compute_potential(0)
+    ref = o.f["psi"][1:-1, 1:-1].copy()
+    variant(po.VARIANT_PIN_GAUGE)
+    o.compute_potential(0)
+    got = o.f["psi"][1:-1, 1:-1]
+    assert abs(got[:, -1].sum()) <= 1e-13 * np.abs(got).max() * got.shape[0]
+    assert np.abs((got - got.mean()) - (ref - ref.mean())).max() <= 1e-12 * np.abs(ref).max()
