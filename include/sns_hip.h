@@ -53,14 +53,26 @@ typedef struct {
 } snsg_config;
 
 /* Device-resident state advanced by the fused step (snsg_integrate).  The
- * layouts are the reference's; temp and temp_alt are exchanged by every
- * substage (the caller's pointers are updated). */
+ * halo'd fields keep the reference's index order with padded rows: element
+ * (I, J) of ux / ux_star is ux[J*ld_ux + (I-1)], of uy, p, psi, temp, temp_alt,
+ * uy_star is uy[J*ld_c + I]; ld_ux and ld_c are multiples of 16 and cell
+ * I = 1 of every row is 128-byte aligned.  The RK histories have the
+ * reference layouts (no halo, no padding).  temp and temp_alt are exchanged
+ * by every substage (the caller's pointers are updated).  snsg_state_upload /
+ * snsg_state_download convert from / to the reference layouts. */
 typedef struct {
   double *ux, *uy, *p, *psi;
   double *temp, *temp_alt;
   double *srcuxa, *srcuya, *srctempa;   /* RK history of the explicit terms */
   double *ux_star, *uy_star;            /* predictor output, (jtot+2) rows */
+  long ld_ux, ld_c;                     /* row pitches in elements */
 } snsg_state;
+
+/* field ids of snsg_state_upload / snsg_state_download */
+enum {
+  SNSG_UX = 0, SNSG_UY, SNSG_P, SNSG_PSI, SNSG_TEMP, SNSG_UX_STAR, SNSG_UY_STAR,
+  SNSG_SRCUXA, SNSG_SRCUYA, SNSG_SRCTEMPA
+};
 
 /* ---- instance management */
 int snsg_create(int device, const snsg_config *cfg, const double *dxf, const double *dxc,
@@ -131,6 +143,12 @@ int snsg_state_free(snsg_ctx *ctx, snsg_state *st);
 int snsg_integrate(snsg_ctx *ctx, snsg_state *st);
 int snsg_advance(snsg_ctx *ctx, snsg_state *st, int nsteps);
 int snsg_get_time(snsg_ctx *ctx, double *time, double *dt);
+/* copy one state field from / to a host array in the reference layout
+ * (blocking) */
+int snsg_state_upload(snsg_ctx *ctx, snsg_state *st, int field, const double *host);
+int snsg_state_download(snsg_ctx *ctx, const snsg_state *st, int field, double *host);
+/* advective time-step bound of the state's ux, uy (as snsg_param_decide_dt) */
+int snsg_state_decide_dt(snsg_ctx *ctx, const snsg_state *st, double *dt_adv);
 int snsg_set_time(snsg_ctx *ctx, double time);
 
 /* ---- self-check of the fused kernels' division scheme (no counterpart in the

@@ -37,7 +37,13 @@ class SnsgConfig(C.Structure):
 
 class SnsgState(C.Structure):
     _fields_ = [(n, _vp) for n in ["ux", "uy", "p", "psi", "temp", "temp_alt",
-                                   "srcuxa", "srcuya", "srctempa", "ux_star", "uy_star"]]
+                                   "srcuxa", "srcuya", "srctempa", "ux_star", "uy_star"]] + [
+        ("ld_ux", C.c_long), ("ld_c", C.c_long)]
+
+
+# field ids of snsg_state_upload / snsg_state_download (include/sns_hip.h)
+STATE_FIELDS = {"ux": 0, "uy": 1, "p": 2, "psi": 3, "temp": 4, "ux_star": 5, "uy_star": 6,
+                "srcuxa": 7, "srcuya": 8, "srctempa": 9}
 
 
 class GpuError(RuntimeError):
@@ -79,6 +85,9 @@ def _load_hip():
         L.snsg_param_decide_dt.argtypes = [_vp, _vp, _vp, _dp]
         L.snsg_transpose.argtypes = [_vp, C.c_int, C.c_int, C.c_size_t, _vp, _vp]
         L.snsg_state_alloc.argtypes = [_vp, C.POINTER(SnsgState)]
+        L.snsg_state_upload.argtypes = [_vp, C.POINTER(SnsgState), C.c_int, _vp]
+        L.snsg_state_download.argtypes = [_vp, C.POINTER(SnsgState), C.c_int, _vp]
+        L.snsg_state_decide_dt.argtypes = [_vp, C.POINTER(SnsgState), _dp]
         L.snsg_state_free.argtypes = [_vp, C.POINTER(SnsgState)]
         L.snsg_integrate.argtypes = [_vp, C.POINTER(SnsgState)]
         L.snsg_advance.argtypes = [_vp, C.POINTER(SnsgState), C.c_int]
@@ -169,6 +178,11 @@ FIELD_SHAPES = {
     "srctempa": lambda i, j: (j, i), "srctempb": lambda i, j: (j, i),
     "srctempg": lambda i, j: (j, i),
 }
+
+
+def _state_shape(name):
+    """Reference-layout shape name of a state field."""
+    return {"ux_star": "ux", "uy_star": "uy"}.get(name, name)
 
 
 class Instance:
@@ -347,14 +361,17 @@ class Instance:
         return self.state
 
     def state_upload(self, name, arr):
+        """Copy a reference-layout array into the padded state field."""
         a = np.ascontiguousarray(arr, dtype=np.float64)
-        _check(self.L.snsg_upload(self.ctx, getattr(self.state, name), a.ctypes.data, a.nbytes),
-               "state upload")
+        assert a.shape == self.shape(_state_shape(name)), (name, a.shape)
+        _check(self.L.snsg_state_upload(self.ctx, C.byref(self.state), STATE_FIELDS[name],
+                                        a.ctypes.data), "snsg_state_upload")
 
     def state_download(self, name, shape_name=None):
-        a = np.zeros(self.shape(shape_name or name))
-        _check(self.L.snsg_download(self.ctx, a.ctypes.data, getattr(self.state, name), a.nbytes),
-               "state download")
+        """A state field in the reference layout."""
+        a = np.zeros(self.shape(shape_name or _state_shape(name)))
+        _check(self.L.snsg_state_download(self.ctx, C.byref(self.state), STATE_FIELDS[name],
+                                          a.ctypes.data), "snsg_state_download")
         return a
 
     def integrate(self):
@@ -374,8 +391,8 @@ class Instance:
 
     def state_dt_adv_raw(self):
         v = C.c_double(0.0)
-        _check(self.L.snsg_param_decide_dt(self.ctx, self.state.ux, self.state.uy, C.byref(v)),
-               "decide_dt")
+        _check(self.L.snsg_state_decide_dt(self.ctx, C.byref(self.state), C.byref(v)),
+               "snsg_state_decide_dt")
         return v.value
 
     # ---- timing

@@ -22,7 +22,8 @@ struct snsg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   snsg_config cfg{};
-  Geo geo{};
+  Geo geo{};     // reference layouts (per-function entry points)
+  Geo geo_st{};  // padded row pitches of the fused state (snsg_state)
   Treat treat{};
   double *d_dxf = nullptr, *d_dxc = nullptr;
   double *d_dt = nullptr;
@@ -101,6 +102,12 @@ static int run(snsg_ctx *ctx, const char *name, F fn) {
 
 // ------------------------------------------------------------- creation
 
+// Fused-state rows are padded to a multiple of 16 elements (128 bytes); cell
+// arrays start kStateCOff elements into their allocation so that cell I = 1
+// of every row is 128-byte aligned.
+static int state_pitch(int n) { return (n + 15) / 16 * 16; }
+static constexpr size_t kStateCOff = 15;
+
 static int ctx_alloc(void **p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes ? bytes : 8);
   if (e != hipSuccess) return set_err((int)e, "hipMalloc");
@@ -162,6 +169,8 @@ extern "C" int snsg_create(int device, const snsg_config *cfg, const double *dxf
   Geo &g = c->geo;
   g.itot = itot;
   g.jtot = jtot;
+  g.pu = itot + 1;
+  g.pc = itot + 2;
   g.dy = dy;
   g.dxf = c->d_dxf;
   g.dxc = c->d_dxc;
@@ -172,6 +181,9 @@ extern "C" int snsg_create(int device, const snsg_config *cfg, const double *dxf
   g.two_sqrtRa = 2. * sqrtRa;
   g.two_sqrtPr_sqrtRa = 2. * sqrtPr * sqrtRa;
   g.two_Re = 2. * (sqrtRa / sqrtPr);
+  c->geo_st = g;
+  c->geo_st.pu = state_pitch(itot + 1);
+  c->geo_st.pc = state_pitch(itot + 2);
   c->treat = make_treat(cfg->implicitx, cfg->implicity);
   // eigenvalues of the periodic second difference (src/fluid/compute_potential.c:112-115)
   std::vector<double> eig(jtot / 2 + 1);
@@ -360,20 +372,22 @@ static RK rk_of(const snsg_ctx *c, int rkstep) {
 // treatment: src/temperature/update.c:89-231, src/fluid/update_velocity.c:194-488
 // src/fluid/compute_potential.c:15-168 (three kernel groups: forward
 // transform, x solves, inverse transform)
-static int potential(snsg_ctx *c, double gamma, const double *ux, const double *uy, double *psi) {
+// g: the layout of ux, uy and psi (c->geo or c->geo_st)
+static int potential(snsg_ctx *c, const Geo &g, double gamma, const double *ux, const double *uy,
+                     double *psi) {
   int rc = run(c, "poisson_fwd", [&] {
-    launch_poisson2(c->stream, c->geo, c->tab, c->plan, gamma, c->d_dt, ux, uy, c->d_tw,
-                    c->d_pfac, c->d_Y, c->d_psihat, psi, 0);
+    launch_poisson2(c->stream, g, c->tab, c->plan, gamma, c->d_dt, ux, uy, c->d_tw, c->d_pfac,
+                    c->d_Y, c->d_psihat, psi, 0);
   });
   if (rc) return rc;
   if ((rc = run(c, "poisson_x", [&] {
-         launch_poisson2(c->stream, c->geo, c->tab, c->plan, gamma, c->d_dt, ux, uy, c->d_tw,
+         launch_poisson2(c->stream, g, c->tab, c->plan, gamma, c->d_dt, ux, uy, c->d_tw,
                          c->d_pfac, c->d_Y, c->d_psihat, psi, 1);
        })))
     return rc;
   return run(c, "poisson_inv", [&] {
-    launch_poisson2(c->stream, c->geo, c->tab, c->plan, gamma, c->d_dt, ux, uy, c->d_tw,
-                    c->d_pfac, c->d_Y, c->d_psihat, psi, 2);
+    launch_poisson2(c->stream, g, c->tab, c->plan, gamma, c->d_dt, ux, uy, c->d_tw, c->d_pfac,
+                    c->d_Y, c->d_psihat, psi, 2);
   });
 }
 
@@ -451,7 +465,7 @@ extern "C" int snsg_fluid_compute_potential(snsg_ctx *c, int rkstep, const doubl
                                             const double *uy, double *psi) {
   CHECK_CTX(c, psi);
   if (!ux || !uy || rkstep < 0 || rkstep > 2) return set_err(SNSG_E_ARG, __func__);
-  return potential(c, c->cfg.gamma[rkstep], ux, uy, psi);
+  return potential(c, c->geo, c->cfg.gamma[rkstep], ux, uy, psi);
 }
 
 extern "C" int snsg_fluid_correct_velocity(snsg_ctx *c, int rkstep, const double *psi, double *ux,
@@ -504,10 +518,15 @@ extern "C" int snsg_transpose(snsg_ctx *c, int g_isize, int g_jsize, size_t elem
 // ---------------------------------------------------------- fused step
 
 static size_t bytes_ux(const snsg_ctx *c) {
-  return sizeof(double) * (size_t)(c->geo.itot + 1) * (c->geo.jtot + 2);
+  return sizeof(double) * (size_t)c->geo_st.pu * (c->geo_st.jtot + 2);
 }
 static size_t bytes_c(const snsg_ctx *c) {
-  return sizeof(double) * (size_t)(c->geo.itot + 2) * (c->geo.jtot + 2);
+  return sizeof(double) * ((size_t)c->geo_st.pc * (c->geo_st.jtot + 2) + 16);
+}
+static int alloc_c(const snsg_ctx *c, double **p) {
+  int rc = ctx_alloc((void **)p, bytes_c(c));
+  if (!rc) *p += kStateCOff;
+  return rc;
 }
 
 extern "C" int snsg_state_alloc(snsg_ctx *c, snsg_state *st) {
@@ -515,15 +534,17 @@ extern "C" int snsg_state_alloc(snsg_ctx *c, snsg_state *st) {
   HIPCHECK(hipSetDevice(c->device));
   memset(st, 0, sizeof *st);
   const int itot = c->geo.itot, jtot = c->geo.jtot;
+  st->ld_ux = c->geo_st.pu;
+  st->ld_c = c->geo_st.pc;
   int rc;
   if ((rc = ctx_alloc((void **)&st->ux, bytes_ux(c)))) return rc;
-  if ((rc = ctx_alloc((void **)&st->uy, bytes_c(c)))) return rc;
-  if ((rc = ctx_alloc((void **)&st->p, bytes_c(c)))) return rc;
-  if ((rc = ctx_alloc((void **)&st->psi, bytes_c(c)))) return rc;
-  if ((rc = ctx_alloc((void **)&st->temp, bytes_c(c)))) return rc;
-  if ((rc = ctx_alloc((void **)&st->temp_alt, bytes_c(c)))) return rc;
+  if ((rc = alloc_c(c, &st->uy))) return rc;
+  if ((rc = alloc_c(c, &st->p))) return rc;
+  if ((rc = alloc_c(c, &st->psi))) return rc;
+  if ((rc = alloc_c(c, &st->temp))) return rc;
+  if ((rc = alloc_c(c, &st->temp_alt))) return rc;
   if ((rc = ctx_alloc((void **)&st->ux_star, bytes_ux(c)))) return rc;
-  if ((rc = ctx_alloc((void **)&st->uy_star, bytes_c(c)))) return rc;
+  if ((rc = alloc_c(c, &st->uy_star))) return rc;
   if ((rc = ctx_alloc((void **)&st->srcuxa, sizeof(double) * (size_t)(itot - 1) * jtot))) return rc;
   if ((rc = ctx_alloc((void **)&st->srcuya, sizeof(double) * (size_t)itot * jtot))) return rc;
   if ((rc = ctx_alloc((void **)&st->srctempa, sizeof(double) * (size_t)itot * jtot))) return rc;
@@ -533,11 +554,77 @@ extern "C" int snsg_state_alloc(snsg_ctx *c, snsg_state *st) {
 extern "C" int snsg_state_free(snsg_ctx *c, snsg_state *st) {
   if (!c || !st) return set_err(SNSG_E_ARG, __func__);
   HIPCHECK(hipStreamSynchronize(c->stream));
-  double *ptrs[] = {st->ux, st->uy, st->p, st->psi, st->temp, st->temp_alt, st->ux_star,
-                    st->uy_star, st->srcuxa, st->srcuya, st->srctempa};
-  for (double *p : ptrs)
+  double *plain[] = {st->ux, st->ux_star, st->srcuxa, st->srcuya, st->srctempa};
+  double *cell[] = {st->uy, st->p, st->psi, st->temp, st->temp_alt, st->uy_star};
+  for (double *p : plain)
     if (p) hipFree(p);
+  for (double *p : cell)
+    if (p) hipFree(p - kStateCOff);
   memset(st, 0, sizeof *st);
+  return 0;
+}
+
+// device pointer, row pitch, row length and row count of a state field
+static int state_field(const snsg_ctx *c, const snsg_state *st, int field, double **dev,
+                       size_t *pitch, size_t *width, size_t *rows) {
+  const int itot = c->geo.itot, jtot = c->geo.jtot;
+  *rows = (size_t)jtot + 2;
+  switch (field) {
+    case SNSG_UX: *dev = st->ux; break;
+    case SNSG_UX_STAR: *dev = st->ux_star; break;
+    case SNSG_UY: *dev = st->uy; break;
+    case SNSG_UY_STAR: *dev = st->uy_star; break;
+    case SNSG_P: *dev = st->p; break;
+    case SNSG_PSI: *dev = st->psi; break;
+    case SNSG_TEMP: *dev = st->temp; break;
+    case SNSG_SRCUXA: *dev = st->srcuxa; *pitch = *width = itot - 1; *rows = jtot; return 0;
+    case SNSG_SRCUYA: *dev = st->srcuya; *pitch = *width = itot; *rows = jtot; return 0;
+    case SNSG_SRCTEMPA: *dev = st->srctempa; *pitch = *width = itot; *rows = jtot; return 0;
+    default: return set_err(SNSG_E_ARG, "state field id");
+  }
+  const bool ux = field == SNSG_UX || field == SNSG_UX_STAR;
+  *pitch = ux ? c->geo_st.pu : c->geo_st.pc;
+  *width = ux ? itot + 1 : itot + 2;
+  return 0;
+}
+
+extern "C" int snsg_state_upload(snsg_ctx *c, snsg_state *st, int field, const double *host) {
+  if (!c || !st || !host) return set_err(SNSG_E_ARG, __func__);
+  double *dev;
+  size_t pitch, width, rows;
+  int rc = state_field(c, st, field, &dev, &pitch, &width, &rows);
+  if (rc) return rc;
+  if (!dev) return set_err(SNSG_E_ARG, "snsg_state_upload: state not allocated");
+  HIPCHECK(hipMemcpy2DAsync(dev, pitch * sizeof(double), host, width * sizeof(double),
+                            width * sizeof(double), rows, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int snsg_state_download(snsg_ctx *c, const snsg_state *st, int field, double *host) {
+  if (!c || !st || !host) return set_err(SNSG_E_ARG, __func__);
+  double *dev;
+  size_t pitch, width, rows;
+  int rc = state_field(c, st, field, &dev, &pitch, &width, &rows);
+  if (rc) return rc;
+  if (!dev) return set_err(SNSG_E_ARG, "snsg_state_download: state not allocated");
+  HIPCHECK(hipMemcpy2DAsync(host, width * sizeof(double), dev, pitch * sizeof(double),
+                            width * sizeof(double), rows, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int snsg_state_decide_dt(snsg_ctx *c, const snsg_state *st, double *dt_adv) {
+  if (!c || !st || !st->ux || !dt_adv) return set_err(SNSG_E_ARG, __func__);
+  int rc = run(c, "dt_reduce", [&] {
+    launch_fill_u64(c->stream, c->d_red, 0x7ff0000000000000ull, 1);
+    launch_dt_reduce(c->stream, c->geo_st, st->ux, st->uy, c->d_red);
+  });
+  if (rc) return rc;
+  HIPCHECK(hipMemcpyAsync(c->h_red, c->d_red, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  memcpy(dt_adv, c->h_red, sizeof *dt_adv);
   return 0;
 }
 
@@ -546,7 +633,7 @@ extern "C" int snsg_state_free(snsg_ctx *c, snsg_state *st) {
 // `reduce`, the last correction also folds the advective bound for the next
 // step into d_red.
 static int integrate_fused(snsg_ctx *c, snsg_state *st, bool reduce) {
-  const Geo &g = c->geo;
+  const Geo &g = c->geo_st;
   const int itot = g.itot, jtot = g.jtot;
   int rc;
   const bool impx = c->treat.implicitx, impy = c->treat.implicity;
@@ -612,7 +699,7 @@ static int integrate_fused(snsg_ctx *c, snsg_state *st, bool reduce) {
       }
     }
     const double gamma = c->cfg.gamma[rk];
-    if ((rc = potential(c, gamma, st->ux_star, st->uy_star, st->psi))) return rc;
+    if ((rc = potential(c, c->geo_st, gamma, st->ux_star, st->uy_star, st->psi))) return rc;
     CorrArgs ca;
     ca.g = g;
     ca.tb = c->tab;
@@ -642,7 +729,7 @@ extern "C" int snsg_advance(snsg_ctx *c, snsg_state *st, int nsteps) {
   }
   rc = run(c, "dt_reduce", [&] {
     launch_fill_u64(c->stream, c->d_red, 0x7ff0000000000000ull, 1);
-    launch_dt_reduce(c->stream, c->geo, st->ux, st->uy, c->d_red);
+    launch_dt_reduce(c->stream, c->geo_st, st->ux, st->uy, c->d_red);
   });
   if (rc) return rc;
   for (int n = 0; n < nsteps; n++) {

@@ -2,9 +2,12 @@
 //
 // Layout contract (the reference's include/arrays/{param,fluid,temperature}.h,
 // single rank, jsize == jtot): row-major, x (i) fastest, one halo row on each
-// side in y.
-//   ux            [(J)*(itot+1) + (I-1)]   I = 1..itot+1, J = 0..jtot+1
-//   uy, p, psi, T [(J)*(itot+2) + (I)]     I = 0..itot+1, J = 0..jtot+1
+// side in y.  Row pitches are Geo::pu / Geo::pc: itot+1 / itot+2 for arrays
+// in the reference layout (the per-function entry points), padded to a
+// multiple of 16 for the fused state (first interior cell of every row on a
+// 128-byte boundary).
+//   ux            [(J)*pu + (I-1)]   I = 1..itot+1, J = 0..jtot+1
+//   uy, p, psi, T [(J)*pc + (I)]     I = 0..itot+1, J = 0..jtot+1
 //   srcux*, tempforcex [(J-1)*(itot-1) + (I-2)]   I = 2..itot
 //   srcuy*, srctemp*   [(J-1)*itot + (I-1)]       I = 1..itot
 //   dxf[I-1] = DXF(I) (I = 1..itot), dxc[I-1] = DXC(I) (I = 1..itot+1)
@@ -24,6 +27,7 @@ namespace sns {
 // operations the reference uses.
 struct Geo {
   int itot, jtot;
+  int pu, pc;         // row pitches (elements) of ux-type and cell-type arrays
   double dy;
   const double *dxf;  // device, DXF(I) = dxf[I-1]
   const double *dxc;  // device, DXC(I) = dxc[I-1]
@@ -39,10 +43,10 @@ __device__ __forceinline__ double DXF(const Geo &g, int i) { return g.dxf[i - 1]
 __device__ __forceinline__ double DXC(const Geo &g, int i) { return g.dxc[i - 1]; }
 
 __device__ __forceinline__ size_t ux_idx(const Geo &g, int i, int j) {
-  return (size_t)j * (g.itot + 1) + (i - 1);
+  return (size_t)j * g.pu + (i - 1);
 }
 __device__ __forceinline__ size_t c_idx(const Geo &g, int i, int j) {  // uy, p, psi, temp
-  return (size_t)j * (g.itot + 2) + i;
+  return (size_t)j * g.pc + i;
 }
 __device__ __forceinline__ size_t sx_idx(const Geo &g, int i, int j) {  // srcux*, tempforcex
   return (size_t)(j - 1) * (g.itot - 1) + (i - 2);

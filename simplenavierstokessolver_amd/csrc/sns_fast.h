@@ -61,11 +61,11 @@ struct Rows3 {
   const double *m, *c, *p;
 };
 __device__ __forceinline__ Rows3 ux_rows(const Geo &g, const double *f, int j) {
-  const size_t w = (size_t)(g.itot + 1);
+  const size_t w = (size_t)g.pu;
   return {f + (size_t)(j - 1) * w, f + (size_t)j * w, f + (size_t)(j + 1) * w};
 }
 __device__ __forceinline__ Rows3 c_rows(const Geo &g, const double *f, int j) {
-  const size_t w = (size_t)(g.itot + 2);
+  const size_t w = (size_t)g.pc;
   return {f + (size_t)(j - 1) * w, f + (size_t)j * w, f + (size_t)(j + 1) * w};
 }
 
@@ -210,7 +210,7 @@ __device__ __forceinline__ void uy_src_t(const Geo &g, const Tables &tb, const R
 // Value of ux at face (i, j) (j uniform across the workgroup) with its
 // periodic copy and the wall values of the row.
 __device__ __forceinline__ void put_ux(const Geo &g, double *f, int i, int j, double v) {
-  const size_t w = (size_t)(g.itot + 1);
+  const size_t w = (size_t)g.pu;
   double *row = f + (size_t)j * w;
   double *halo = j == 1 ? f + (size_t)(g.jtot + 1) * w : f;  // used when j is 1 or jtot
   const bool copy = j == 1 || j == g.jtot;
@@ -228,7 +228,7 @@ __device__ __forceinline__ void put_ux(const Geo &g, double *f, int i, int j, do
 // copies (p), 2 temperature walls (rows 1..jtot only).
 template <int WALL>
 __device__ __forceinline__ void put_c(const Geo &g, double *f, int i, int j, double v) {
-  const size_t w = (size_t)(g.itot + 2);
+  const size_t w = (size_t)g.pc;
   double *row = f + (size_t)j * w;
   double *halo = j == 1 ? f + (size_t)(g.jtot + 1) * w : f;
   const bool copy = j == 1 || j == g.jtot;

@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(256) k_corr(CorrArgs a) {
   const double fac = -gdt / g.two_Re;
   double m = 5.e-2;
   if (i <= g.itot) {
-    const size_t wc = (size_t)(g.itot + 2), wu = (size_t)(g.itot + 1);
+    const size_t wc = (size_t)g.pc, wu = (size_t)g.pu;
     double psi_s = ldr(a.psi + (size_t)(j0 - 1) * wc, i);  // psi(i, j-1)
     double psi_c = ldr(a.psi + (size_t)j0 * wc, i);
 #pragma unroll

@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(NT) k_pred_rows(PredArgs a, const double *lu, 
   const double dt = *a.dtp;
   const double wex = a.tr.w_ex_x, wey = a.tr.w_ex_y, wix = a.tr.w_im_x, wiy = a.tr.w_im_y;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const size_t wc = (size_t)(itot + 2), wu = (size_t)(itot + 1);
+  const size_t wc = (size_t)g.pc, wu = (size_t)g.pu;
   const int nk = ((n + kWaveCols - 1) / kWaveCols + NW - 1) / NW;
 
   // ---- stencils and RK increments of both rows into LDS (lane order)
