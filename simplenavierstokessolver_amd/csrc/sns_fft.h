@@ -114,8 +114,11 @@ struct Dft<5, SIGN> {
 // consecutive threads touch consecutive LDS words in every pass.  Twiddles of
 // the length-n transform come from a table of length tw_n (a multiple of n):
 // w_n^m = tw[m * (tw_n / n)].
+// Not inlined: with every radix inlined into the pass loop the register
+// allocation of the kernels grows to the union of the paths (about 200
+// VGPRs); as calls each pass keeps its own (under 80).
 template <int NT, int EMAX, int R, int SIGN>
-__device__ __forceinline__ void fft_pass_il(double2 *buf, int S, int n, int Ns,
+__device__ __noinline__ void fft_pass_il(double2 *buf, int S, int n, int Ns,
                                             const double2 *__restrict__ tw, int twmul) {
   constexpr int BMAX = (EMAX + R - 1) / R;
   const int nb = n / R;
