@@ -245,6 +245,25 @@ __device__ __forceinline__ void put_c(const Geo &g, double *f, int i, int j, dou
   }
 }
 
+// ---------------------------------------------------------- lane shifts
+
+// Value of v in lane l-1 (lane 0: 0).
+__device__ __forceinline__ double from_left(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), 0x138, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// Value of v in lane l+1 (lane 63: 0).
+__device__ __forceinline__ double from_right(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), 0x130, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
 // Block index -> work index so that consecutive work items run on the same
 // XCD (blocks are dispatched round-robin over the 8 XCDs, each with its own
 // L2); neighbouring rows / strips then share that L2.

@@ -137,76 +137,106 @@ void launch_add_out(hipStream_t s, const Geo &g, int kind, const double *in, con
 
 // rows per workgroup of the correction kernel
 constexpr int kCorrRows = 4;
+// output columns per wave: lanes 1..62; lanes 0 and 63 hold the neighbours
+constexpr int kCorrCols = 62;
+constexpr int kCorrThreads = 256;
 
 // src/fluid/correct_velocity.c:6-42 and src/fluid/update_pressure.c:7-60 in
 // one pass: reads ux*, uy*, p and psi, writes ux, uy, p with boundary
-// values.  With red != nullptr it also folds the advective bound of
+// values.  psi(i-1) and psi(i+1) come from the adjacent lanes; psi of the
+// rows above and below is carried in registers; the column metrics are read
+// once per thread.  With red != nullptr it also folds the advective bound of
 // src/param/decide_dt.c:25-45 over the corrected velocities: each workgroup
 // stores the minimum over its cells in red[workgroup] and k_dt reduces them
 // (min is exact, so the result does not depend on the order).
-__global__ void __launch_bounds__(256) k_corr(CorrArgs a) {
+__global__ void __launch_bounds__(kCorrThreads) k_corr(CorrArgs a) {
   const Geo &g = a.g;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const Tables &tb = a.tb;
+  const int itot = g.itot;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = (blockIdx.x * (kCorrThreads / 64) + wave) * kCorrCols + lane;  // lane 0: i-1 halo
+  const bool out = lane >= 1 && lane <= kCorrCols && i <= itot;
+  const int ci = clampi(i, 0, itot + 1);
+  const int ti = clampi(i - 1, 0, itot + 1);  // table entry of column i
   const int j0 = blockIdx.y * kCorrRows + 1;
   const double dt = *a.dtp;
   const double gdt = a.gamma * dt;
   const double fac = -gdt / g.two_Re;
+  const size_t wc = (size_t)g.pc, wu = (size_t)g.pu;
+  const double dxc_i = ldr(tb.dxc, ti), rdxc_i = ldr(tb.rdxc, ti);  // DXC(i), 1./DXC(i)
+  double dxc_p = 0., rdxc_p = 0., dxf_i = 0., rdxf_i = 0.;
+  if (a.tr.implicitx) {
+    dxc_p = from_right(dxc_i);  // DXC(i+1)
+    rdxc_p = from_right(rdxc_i);
+    dxf_i = ldr(tb.dxf, ti);
+    rdxf_i = ldr(tb.rdxf, ti);
+  }
   double m = 5.e-2;
-  if (i <= g.itot) {
-    const size_t wc = (size_t)g.pc, wu = (size_t)g.pu;
-    double psi_s = ldr(a.psi + (size_t)(j0 - 1) * wc, i);  // psi(i, j-1)
-    double psi_c = ldr(a.psi + (size_t)j0 * wc, i);
+  double psi_s = ldr(a.psi + (size_t)(j0 - 1) * wc, ci);  // psi(i, j-1)
+  double psi_c = ldr(a.psi + (size_t)j0 * wc, ci);
 #pragma unroll
-    for (int r = 0; r < kCorrRows; r++) {
-      const int j = j0 + r;
-      if (j > g.jtot) break;
-      const double *psir = a.psi + (size_t)j * wc;
-      const double psi_n = ldr(psir + wc, i);  // psi(i, j+1)
+  for (int r = 0; r < kCorrRows; r++) {
+    const int j = j0 + r;
+    if (j > g.jtot) break;
+    const double psi_n = ldr(a.psi + (size_t)(j + 1) * wc, ci);  // psi(i, j+1)
+    const double psi_w = from_left(psi_c);                        // psi(i-1, j)
+    const double psi_e = from_right(psi_c);                       // psi(i+1, j)
+    const double us = ldr(a.ux_star + (size_t)j * wu, clampi(i - 1, 0, itot));
+    const double vs = ldr(a.uy_star + (size_t)j * wc, ci);
+    const double pj = ldr(a.p + (size_t)j * wc, ci);
+    if (out) {
       if (i >= 2) {
-        const double dxi = ldr(a.tb.rdxc, i - 1);  // 1./DXC(i)
-        const double v = ldr(a.ux_star + (size_t)j * wu, i - 1) - gdt * (psi_c - ldr(psir, i - 1)) * dxi;
+        const double v = us - gdt * (psi_c - psi_w) * rdxc_i;  // (1./DXC(i)) as a product
         put_ux(g, a.ux, i, j, v);
-        if (a.red) m = fmin(m, ldr(a.tb.dxc, i - 1) / fmax(fabs(v), 1.e-8));
+        if (a.red) m = fmin(m, dxc_i / fmax(fabs(v), 1.e-8));
       }
       {
-        const double v = ldr(a.uy_star + (size_t)j * wc, i) - div_cr(gdt * (psi_c - psi_s), a.tb.dy, a.tb.rdy);
+        const double v = vs - div_cr(gdt * (psi_c - psi_s), tb.dy, tb.rdy);
         put_c<0>(g, a.uy, i, j, v);
-        if (a.red) m = fmin(m, a.tb.dy / fmax(fabs(v), 1.e-8));
+        if (a.red) m = fmin(m, tb.dy / fmax(fabs(v), 1.e-8));
       }
-      double pv = ldr(a.p + (size_t)j * wc, i) + psi_c;
+      double pv = pj + psi_c;
       if (a.tr.implicitx) {
-        const double dm = div_cr(-ldr(psir, i - 1) + psi_c, ldr(a.tb.dxc, i - 1), ldr(a.tb.rdxc, i - 1));
-        const double dp = div_cr(-psi_c + ldr(psir, i + 1), ldr(a.tb.dxc, i), ldr(a.tb.rdxc, i));
-        pv += fac * (+div_cr(dp - dm, ldr(a.tb.dxf, i - 1), ldr(a.tb.rdxf, i - 1)));
+        const double dm = div_cr(-psi_w + psi_c, dxc_i, rdxc_i);
+        const double dp = div_cr(-psi_c + psi_e, dxc_p, rdxc_p);
+        pv += fac * (+div_cr(dp - dm, dxf_i, rdxf_i));
       }
       if (a.tr.implicity) {
-        const double dm = div_cr(-psi_s + psi_c, a.tb.dy, a.tb.rdy);
-        const double dp = div_cr(-psi_c + psi_n, a.tb.dy, a.tb.rdy);
-        pv += fac * (+div_cr(dp - dm, a.tb.dy, a.tb.rdy));
+        const double dm = div_cr(-psi_s + psi_c, tb.dy, tb.rdy);
+        const double dp = div_cr(-psi_c + psi_n, tb.dy, tb.rdy);
+        pv += fac * (+div_cr(dp - dm, tb.dy, tb.rdy));
       }
       put_c<1>(g, a.p, i, j, pv);
-      psi_s = psi_c;
-      psi_c = psi_n;
     }
+    psi_s = psi_c;
+    psi_c = psi_n;
   }
   if (a.red) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off));
-    __shared__ double wm[4];
-    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __shared__ double wm[kCorrThreads / 64];
+    if (lane == 0) wm[wave] = m;
     __syncthreads();
-    if (threadIdx.x == 0)
-      a.red[blockIdx.y * gridDim.x + blockIdx.x] = fmin(fmin(wm[0], wm[1]), fmin(wm[2], wm[3]));
+    if (threadIdx.x == 0) {
+      double r = wm[0];
+      for (int w = 1; w < kCorrThreads / 64; w++) r = fmin(r, wm[w]);
+      a.red[blockIdx.y * gridDim.x + blockIdx.x] = r;
+    }
   }
 }
 
+static dim3 corr_grid(const Geo &g) {
+  const int cols = (kCorrThreads / 64) * kCorrCols;
+  return dim3((g.itot + cols - 1) / cols, (g.jtot + kCorrRows - 1) / kCorrRows);
+}
+
 int corr_partials(const Geo &g) {
-  return ((g.itot + 255) / 256) * ((g.jtot + kCorrRows - 1) / kCorrRows);
+  const dim3 gr = corr_grid(g);
+  return (int)(gr.x * gr.y);
 }
 
 void launch_corr(hipStream_t s, const CorrArgs &a) {
-  hipLaunchKernelGGL(k_corr, dim3((a.g.itot + 255) / 256, (a.g.jtot + kCorrRows - 1) / kCorrRows),
-                     dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_corr, corr_grid(a.g), dim3(kCorrThreads), 0, s, a);
 }
 
 // ------------------------------------------------ division self-check
