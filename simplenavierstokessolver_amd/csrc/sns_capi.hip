@@ -40,6 +40,7 @@ struct snsg_ctx {
   double2 *d_Y = nullptr;  // four-step FFT intermediate, jtot rows of (itot+1)/2 pairs
   // per-column tables of the fused kernels (one allocation)
   double *d_tab = nullptr;
+  double2 *d_tab2 = nullptr;           // paired column tables
   Tables tab{};
   // fused-step scratch: x-aligned increments of T, ux, uy when a y solve follows
   double *d_qf[3] = {nullptr, nullptr, nullptr};
@@ -234,6 +235,24 @@ extern "C" int snsg_create(int device, const snsg_config *cfg, const double *dxf
     tb.nT_xm = d + 8 * stride; tb.nT_xp = d + 9 * stride;
     tb.nU_xm = d + 10 * stride; tb.nU_xp = d + 11 * stride;
     tb.pl = d + 12 * stride; tb.pu = d + 13 * stride;
+    // paired copies: (DXC, 1/DXC), (DXF, 1/DXF), (cT_xm, cT_xp), (cU_xm, cU_xp)
+    {
+      std::vector<double2> h2(4 * stride);
+      for (size_t k = 0; k < stride; k++) {
+        h2[0 * stride + k] = make_double2(t[1][k], t[3][k]);
+        h2[1 * stride + k] = make_double2(t[0][k], t[2][k]);
+        h2[2 * stride + k] = make_double2(t[4][k], t[5][k]);
+        h2[3 * stride + k] = make_double2(t[6][k], t[7][k]);
+      }
+      if ((rc = ctx_alloc((void **)&c->d_tab2, sizeof(double2) * h2.size()))) return fail(rc);
+      if (hipMemcpy(c->d_tab2, h2.data(), sizeof(double2) * h2.size(), hipMemcpyHostToDevice) !=
+          hipSuccess)
+        return fail(set_err(SNSG_E_ARG, "upload paired column tables"));
+      tb.dxc2 = c->d_tab2;
+      tb.dxf2 = c->d_tab2 + stride;
+      tb.cT2 = c->d_tab2 + 2 * stride;
+      tb.cU2 = c->d_tab2 + 3 * stride;
+    }
     tb.dy = dy;
     tb.rdy = 1. / dy;
     // fixed diffusive time-step bounds (src/param/decide_dt.c:48-66)
@@ -266,7 +285,7 @@ extern "C" int snsg_destroy(snsg_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *bufs[] = {c->d_dxf, c->d_dxc, c->d_dt, c->d_eigen, c->d_tw, c->d_psihat, c->d_Y, c->d_q,
                   c->d_qy, c->d_red, c->d_tab, c->d_time, c->d_qf[0], c->d_qf[1],
-                  c->d_qf[2], c->d_lu, c->d_part, c->d_pfac};
+                  c->d_qf[2], c->d_lu, c->d_part, c->d_pfac, c->d_tab2};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (c->h_red) hipHostFree(c->h_red);

@@ -40,8 +40,17 @@ struct Tables {
   const double *nT_xm, *nT_xp; // 1./DXF(I)/DXC(I), 1./DXF(I)/DXC(I+1)       (CN of T and uy)
   const double *nU_xm, *nU_xp; // 1./DXC(I)/DXF(I-1), 1./DXC(I)/DXF(I)       (CN of ux)
   const double *pl, *pu;       // 1./DXC(I)/DXF(I), 1./DXC(I+1)/DXF(I)       (Poisson)
+  // the same values in pairs, for one 16-byte load each (entry as above)
+  const double2 *dxc2;         // (DXC(I), 1./DXC(I))
+  const double2 *dxf2;         // (DXF(I), 1./DXF(I))
+  const double2 *cT2, *cU2;    // (cT_xm, cT_xp), (cU_xm, cU_xp)
   double dy, rdy;
 };
+
+__device__ __forceinline__ double2 ldr2(const double2 *row, int k) {
+  return *reinterpret_cast<const double2 *>(reinterpret_cast<const char *>(row) +
+                                            ((unsigned)k << 4));
+}
 
 // Loads and stores through a row base (uniform across the workgroup, so it
 // stays in scalar registers) plus a 32-bit byte offset: the SGPR-base +

@@ -198,12 +198,13 @@ __global__ void __launch_bounds__(NT) k_pred_rows(PredArgs a, const double *lu, 
       const double v_a = ldr(V + ja * wc, ci), v_b = ldr(V + jb * wc, ci);
       const double v_bp = ldr(V + (jb + 1) * wc, ci);
       CellX x;
-      x.dxc = ldr(tb.dxc, ti);
-      x.rdxc = ldr(tb.rdxc, ti);
-      x.dxf = ldr(tb.dxf, ti);
-      x.rdxf = ldr(tb.rdxf, ti);
-      x.c_xm = ldr(tb.cT_xm, ti);
-      x.c_xp = ldr(tb.cT_xp, ti);
+      const double2 c2 = ldr2(tb.dxc2, ti), f2 = ldr2(tb.dxf2, ti), w2 = ldr2(tb.cT2, ti);
+      x.dxc = c2.x;
+      x.rdxc = c2.y;
+      x.dxf = f2.x;
+      x.rdxf = f2.y;
+      x.c_xm = w2.x;
+      x.c_xp = w2.y;
       x.dxc_p = from_right(x.dxc);
       x.rdxc_p = from_right(x.rdxc);
       const double *HA = a.hist_t + (size_t)(ja - 1) * n, *HB = a.hist_t + (size_t)(jb - 1) * n;
@@ -227,12 +228,13 @@ __global__ void __launch_bounds__(NT) k_pred_rows(PredArgs a, const double *lu, 
       const double v_bp = ldr(V + (jb + 1) * wc, ci);
       const double p_a = ldr(P + ja * wc, ci), p_b = ldr(P + jb * wc, ci);
       FaceX x;
-      x.dxf = ldr(tb.dxf, ti);
-      x.rdxf = ldr(tb.rdxf, ti);
-      x.c_xm = ldr(tb.cU_xm, ti);
-      x.c_xp = ldr(tb.cU_xp, ti);
-      x.dxc = ldr(tb.dxc, ti);
-      x.rdxc = ldr(tb.rdxc, ti);
+      const double2 c2 = ldr2(tb.dxc2, ti), f2 = ldr2(tb.dxf2, ti), w2 = ldr2(tb.cU2, ti);
+      x.dxf = f2.x;
+      x.rdxf = f2.y;
+      x.c_xm = w2.x;
+      x.c_xp = w2.y;
+      x.dxc = c2.x;
+      x.rdxc = c2.y;
       x.dxf_m = from_left(x.dxf);
       x.rdxf_m = from_left(x.rdxf);
       double tf_a = 0., tf_b = 0.;
@@ -264,12 +266,13 @@ __global__ void __launch_bounds__(NT) k_pred_rows(PredArgs a, const double *lu, 
       const double p_am = ldr(P + (ja - 1) * wc, ci), p_a = ldr(P + ja * wc, ci);
       const double p_b = ldr(P + jb * wc, ci);
       CellX x;
-      x.dxc = ldr(tb.dxc, ti);
-      x.rdxc = ldr(tb.rdxc, ti);
-      x.dxf = ldr(tb.dxf, ti);
-      x.rdxf = ldr(tb.rdxf, ti);
-      x.c_xm = ldr(tb.cT_xm, ti);
-      x.c_xp = ldr(tb.cT_xp, ti);
+      const double2 c2 = ldr2(tb.dxc2, ti), f2 = ldr2(tb.dxf2, ti), w2 = ldr2(tb.cT2, ti);
+      x.dxc = c2.x;
+      x.rdxc = c2.y;
+      x.dxf = f2.x;
+      x.rdxf = f2.y;
+      x.c_xm = w2.x;
+      x.c_xp = w2.y;
       x.dxc_p = from_right(x.dxc);
       x.rdxc_p = from_right(x.rdxc);
       const double *HA = a.hist_uy + (size_t)(ja - 1) * n, *HB = a.hist_uy + (size_t)(jb - 1) * n;
