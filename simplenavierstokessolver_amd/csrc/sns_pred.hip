@@ -297,6 +297,21 @@ __global__ void __launch_bounds__(NT) k_pred_rows(PredArgs a, const double *lu, 
   }
   __syncthreads();
 
+  // ---- values the increments are added to, fetched now so that their
+  // latency overlaps the line solve (emit order: unknown t + k*NT)
+  const double *oldf = f == 0 ? a.temp : (f == 1 ? a.ux : a.uy);
+  const size_t wf = f == 1 ? wu : wc;
+  const int io = f == 1 ? -1 : 0;  // column i -> element offset in the row
+  double old[2][M];
+#pragma unroll
+  for (int s = 0; s < 2; s++)
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+      const int ii = threadIdx.x + k * NT;
+      const int j = s == 0 ? ja : jb;
+      old[s][k] = (ii < n && !a.q_t) ? ldr(oldf + j * wf, ii + i0 + io) : 0.;
+    }
+
   // ---- Crank-Nicolson in x of both rows (chunk order)
   double dd[2][M];
 #pragma unroll
@@ -319,7 +334,10 @@ __global__ void __launch_bounds__(NT) k_pred_rows(PredArgs a, const double *lu, 
   __syncthreads();
 
   // ---- final values with boundary values, or the increments for a y solve
-  for (int ii = threadIdx.x; ii < n; ii += NT) {
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const int ii = threadIdx.x + k * NT;
+    if (ii >= n) break;
     const int i = ii + i0;
 #pragma unroll
     for (int s = 0; s < 2; s++) {
@@ -328,13 +346,13 @@ __global__ void __launch_bounds__(NT) k_pred_rows(PredArgs a, const double *lu, 
       const double d = L.d[s][pad_idx<M>(ii)];
       if (f == 0) {
         if (a.q_t) str(a.q_t + (size_t)(j - 1) * n, ii, d);
-        else put_c<2>(g, a.out_t, i, j, ldr(a.temp + j * wc, i) + d);
+        else put_c<2>(g, a.out_t, i, j, old[s][k] + d);
       } else if (f == 1) {
         if (a.q_u) str(a.q_u + (size_t)(j - 1) * n, ii, d);
-        else put_ux(g, a.out_u, i, j, ldr(a.ux + j * wu, i - 1) + d);
+        else put_ux(g, a.out_u, i, j, old[s][k] + d);
       } else {
         if (a.q_v) str(a.q_v + (size_t)(j - 1) * n, ii, d);
-        else put_c<0>(g, a.out_v, i, j, ldr(a.uy + j * wc, i) + d);
+        else put_c<0>(g, a.out_v, i, j, old[s][k] + d);
       }
     }
   }
