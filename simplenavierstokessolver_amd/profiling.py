@@ -23,9 +23,9 @@ import sys
 
 # kernel function -> group
 GROUPS = [
-    ("k_pred_row", "pred"), ("k_pred_exp", "pred"),
+    ("k_pred_rows", "pred"), ("k_pred_exp", "pred"),
     ("k_pfwd_a", "poisson_fwd"), ("k_pfwd_b", "poisson_fwd"),
-    ("k_poisson_x2", "poisson_x"),
+    ("k_poisson_x3", "poisson_x"), ("k_poisson_factor", "setup"),
     ("k_pinv_b", "poisson_inv"), ("k_pinv_a", "poisson_inv"),
     ("k_corr", "corr"), ("k_dt_reduce", "dt_reduce"), ("k_fill_u64", "dt_reduce"),
     ("k_dt", "dt"), ("k_lu_factor", "lu_factor"), ("k_copy", "copy"),
