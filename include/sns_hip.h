@@ -147,6 +147,15 @@ int snsg_get_time(snsg_ctx *ctx, double *time, double *dt);
  * (blocking) */
 int snsg_state_upload(snsg_ctx *ctx, snsg_state *st, int field, const double *host);
 int snsg_state_download(snsg_ctx *ctx, const snsg_state *st, int field, double *host);
+/* ---- monitors (src/logging.c:36-272, src/analyses.c:8-121) evaluated on the
+ * device; out[11] = { max|div|, sum div, sum ux dV, sum uy dV, energies of
+ * ux, uy, T, Nusselt numbers from the wall flux, the energy injection, the
+ * kinetic and the thermal dissipation }.  The maximum is exact, the sums are
+ * reduced in a fixed order (reproducible, not in the reference's sequential
+ * order).  temp may be NULL without the temperature equation.  Blocking. */
+int snsg_diagnostics(snsg_ctx *ctx, const double *ux, const double *uy, const double *temp,
+                     double *out);
+int snsg_state_diagnostics(snsg_ctx *ctx, const snsg_state *st, double *out);
 /* advective time-step bound of the state's ux, uy (as snsg_param_decide_dt) */
 int snsg_state_decide_dt(snsg_ctx *ctx, const snsg_state *st, double *dt_adv);
 int snsg_set_time(snsg_ctx *ctx, double time);

@@ -88,6 +88,8 @@ def _load_hip():
         L.snsg_state_upload.argtypes = [_vp, C.POINTER(SnsgState), C.c_int, _vp]
         L.snsg_state_download.argtypes = [_vp, C.POINTER(SnsgState), C.c_int, _vp]
         L.snsg_state_decide_dt.argtypes = [_vp, C.POINTER(SnsgState), _dp]
+        L.snsg_diagnostics.argtypes = [_vp, _vp, _vp, _vp, _dp]
+        L.snsg_state_diagnostics.argtypes = [_vp, C.POINTER(SnsgState), _dp]
         L.snsg_state_free.argtypes = [_vp, C.POINTER(SnsgState)]
         L.snsg_integrate.argtypes = [_vp, C.POINTER(SnsgState)]
         L.snsg_advance.argtypes = [_vp, C.POINTER(SnsgState), C.c_int]
@@ -388,6 +390,21 @@ class Instance:
 
     def set_time(self, t):
         _check(self.L.snsg_set_time(self.ctx, float(t)), "snsg_set_time")
+
+    DIAG_NAMES = ("divmax", "divsum", "mom_x", "mom_y", "energy_ux", "energy_uy", "energy_t",
+                  "nu_wall", "nu_inje", "nu_eps_k", "nu_eps_h")
+
+    def diagnostics(self, state=False):
+        """The monitors of src/logging.c evaluated on the device, as a dict."""
+        out = np.zeros(11)
+        if state:
+            _check(self.L.snsg_state_diagnostics(self.ctx, C.byref(self.state), _p(out)),
+                   "snsg_state_diagnostics")
+        else:
+            b = self.buf
+            _check(self.L.snsg_diagnostics(self.ctx, b["ux"], b["uy"], b.get("temp"), _p(out)),
+                   "snsg_diagnostics")
+        return dict(zip(self.DIAG_NAMES, out))
 
     def state_dt_adv_raw(self):
         v = C.c_double(0.0)

@@ -11,6 +11,7 @@
 #include "sns_launch.h"
 #include "sns_fused.h"
 #include "sns_poisson.h"
+#include "sns_diag.h"
 
 using namespace sns;
 
@@ -35,6 +36,8 @@ struct snsg_ctx {
   unsigned long long *d_red = nullptr;
   unsigned long long *h_red = nullptr;  // pinned
   double *d_part = nullptr;             // per-workgroup minima of the fused correction
+  double *d_diag = nullptr;             // diagnostics: partial slots, then the results
+  double *h_diag = nullptr;             // pinned copy of the results
   int nparts = 0;
   Fft2Plan plan{};
   double2 *d_Y = nullptr;  // four-step FFT intermediate, jtot rows of (itot+1)/2 pairs
@@ -285,10 +288,11 @@ extern "C" int snsg_destroy(snsg_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *bufs[] = {c->d_dxf, c->d_dxc, c->d_dt, c->d_eigen, c->d_tw, c->d_psihat, c->d_Y, c->d_q,
                   c->d_qy, c->d_red, c->d_tab, c->d_time, c->d_qf[0], c->d_qf[1],
-                  c->d_qf[2], c->d_lu, c->d_part, c->d_pfac, c->d_tab2};
+                  c->d_qf[2], c->d_lu, c->d_part, c->d_pfac, c->d_tab2, c->d_diag};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (c->h_red) hipHostFree(c->h_red);
+  if (c->h_diag) hipHostFree(c->h_diag);
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -631,6 +635,38 @@ extern "C" int snsg_state_download(snsg_ctx *c, const snsg_state *st, int field,
                             width * sizeof(double), rows, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   return 0;
+}
+
+// monitors of src/logging.c on the device (see sns_diag.hip); blocking
+static int diagnostics(snsg_ctx *c, const Geo &g, const double *ux, const double *uy,
+                       const double *temp, double *out) {
+  int rc;
+  const size_t np = (size_t)diag_partials(g) * kDiagSlots;
+  if (!c->d_diag) {
+    if ((rc = ctx_alloc((void **)&c->d_diag, sizeof(double) * (np + kDiagSlots)))) return rc;
+    HIPCHECK(hipHostMalloc((void **)&c->h_diag, sizeof(double) * kDiagSlots, hipHostMallocDefault));
+  }
+  DiagArgs a{ux, uy, c->cfg.with_temperature ? temp : nullptr};
+  rc = run(c, "diag",
+           [&] { launch_diag(c->stream, g, a, c->d_diag, 1., c->cfg.ly, c->d_diag + np); });
+  if (rc) return rc;
+  HIPCHECK(hipMemcpyAsync(c->h_diag, c->d_diag + np, sizeof(double) * kDiagSlots,
+                          hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  memcpy(out, c->h_diag, sizeof(double) * kDiagSlots);
+  return 0;
+}
+
+extern "C" int snsg_diagnostics(snsg_ctx *c, const double *ux, const double *uy,
+                                const double *temp, double *out) {
+  if (!c || !ux || !uy || !out || (c->cfg.with_temperature && !temp))
+    return set_err(SNSG_E_ARG, __func__);
+  return diagnostics(c, c->geo, ux, uy, temp, out);
+}
+
+extern "C" int snsg_state_diagnostics(snsg_ctx *c, const snsg_state *st, double *out) {
+  if (!c || !st || !st->ux || !out) return set_err(SNSG_E_ARG, __func__);
+  return diagnostics(c, c->geo_st, st->ux, st->uy, st->temp, out);
 }
 
 extern "C" int snsg_state_decide_dt(snsg_ctx *c, const snsg_state *st, double *dt_adv) {

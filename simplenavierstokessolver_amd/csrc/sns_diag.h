@@ -1,0 +1,20 @@
+// sns_diag.h -- monitoring diagnostics on the device.
+#pragma once
+#include "sns_launch.h"
+
+namespace sns {
+
+// divmax, divsum, momentum x/y, energies ux/uy/T, Nusselt wall/injection/
+// kinetic dissipation/thermal dissipation
+constexpr int kDiagSlots = 11;
+
+struct DiagArgs {
+  const double *ux, *uy, *temp;  // temp may be null (no temperature equation)
+};
+
+int diag_partials(const Geo &g);
+// out (device, kDiagSlots doubles); part: diag_partials(g)*kDiagSlots doubles
+void launch_diag(hipStream_t s, const Geo &g, const DiagArgs &a, double *part, double lx,
+                 double ly, double *out);
+
+}  // namespace sns

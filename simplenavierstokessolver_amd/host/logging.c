@@ -1,8 +1,9 @@
 /*
  * Monitoring (reference: src/logging.c:13-281, src/analyses.c:8-121): the
- * fields are downloaded and the divergence, momentum, quadratic-energy and
- * Nusselt diagnostics are evaluated on the host with the reference's
- * formulas, then appended to output/log/*.dat.
+ * divergence, momentum, quadratic-energy and Nusselt diagnostics are
+ * evaluated on the device (snsg_diagnostics) and appended to output/log/*.dat.
+ * logging_compute evaluates the same quantities on host arrays with the
+ * reference's sequential loops.
  */
 #include <stdio.h>
 #include <math.h>
@@ -137,15 +138,10 @@ static FILE *open_log(const param_t *param, const char *name){
 
 int logging(param_t *param, const parallel_t *parallel, const fluid_t *fluid, const temperature_t *temperature){
   (void)parallel;
-  const int itot = param->itot;
-  const int jsize = param->jtot;
-  double *ux = common_calloc(1, UX_MEMSIZE);
-  double *uy = common_calloc(1, UY_MEMSIZE);
-  double *temp = common_calloc(1, TEMP_MEMSIZE);
-  fluid_download(param, fluid, ux, uy, NULL, NULL);
-  if(param->with_temperature) temperature_download(param, temperature, temp);
   double q[11];
-  logging_compute(param, ux, uy, param->with_temperature ? temp : NULL, q);
+  common_check_gpu(snsg_diagnostics(param->gpu, fluid->ux, fluid->uy,
+                                    param->with_temperature ? temperature->temp : NULL, q),
+                   "logging");
   FILE *fp;
   char scheme[32];
   snprintf(scheme, sizeof scheme, "x: %3s, y: %3s", param->implicitx ? "IMP" : "EXP", param->implicity ? "IMP" : "EXP");
@@ -170,8 +166,5 @@ int logging(param_t *param, const parallel_t *parallel, const fluid_t *fluid, co
     fprintf(fp, "%8.2f % 18.15e % 18.15e % 18.15e % 18.15e\n", param->time, q[7], q[8], q[9], q[10]);
     fileio_fclose(fp);
   }
-  common_free(ux);
-  common_free(uy);
-  common_free(temp);
   return 0;
 }

@@ -400,3 +400,47 @@ def test_device_copy(sns):
     with pytest.raises(sns.GpuError):
         inst.copy(dst, src, 8 * n - 8)
     inst.close()
+
+
+# ------------------------------------------------------------ diagnostics
+
+DIAG_CASES = [
+    # (itot, jtot, stretch, Ra, Pr, imx, imy, seed, amp, steps)
+    (64, 128, 3.0, 1e8, 1.0, 0, 0, 1, 1e-2, 20),
+    (96, 192, 100.0, 1e9, 0.7, 1, 0, 2, 1e-2, 10),
+    (300, 64, 3.0, 1e6, 4.0, 1, 1, 3, 1e-2, 10),
+]
+
+
+@pytest.mark.parametrize("case", DIAG_CASES)
+def test_diagnostics_match_oracle(sns, case):
+    """The monitors of src/logging.c on the device against the oracle's
+    sequential loops: max |div| exactly, energies and Nusselt numbers to
+    1e-12 relative, the sums that cancel (divergence, momentum) to 1e-12 of
+    the magnitude of their terms."""
+    itot, jtot, stretch, Ra, Pr, imx, imy, seed, amp, steps = case
+    o = po.Oracle(itot, jtot, stretch=stretch, Ra=Ra, Pr=Pr)
+    o.set_treatment(imx, imy)
+    o.init_temperature(seed, amp, linear=False)
+    o.apply_boundaries()
+    o.run(steps)
+    inst = sns.Instance(itot, jtot, stretch=stretch, Ra=Ra, Pr=Pr, implicitx=imx, implicity=imy)
+    inst.alloc("ux", "uy", "temp")
+    inst.state_alloc()
+    for n in ("ux", "uy", "temp"):
+        inst.upload(n, o.f[n])
+        inst.state_upload(n, o.f[n])
+    div, mom, ene, nu = o.divergence(), o.momentum(), o.energy(), o.nusselt()
+    ncell = itot * jtot
+    umax = max(np.abs(o.f["ux"]).max(), np.abs(o.f["uy"]).max())
+    for state in (False, True):
+        d = inst.diagnostics(state=state)
+        assert d["divmax"] == div[0]
+        assert abs(d["divsum"] - div[1]) <= 1e-12 * ncell * div[0]
+        for k, ref in (("mom_x", mom[0]), ("mom_y", mom[1])):
+            assert abs(d[k] - ref) <= 1e-12 * umax * o.param.ly, (k, d[k], ref)
+        for k, ref in (("energy_ux", ene[0]), ("energy_uy", ene[1]), ("energy_t", ene[2]),
+                       ("nu_wall", nu[0]), ("nu_inje", nu[1]), ("nu_eps_k", nu[2]),
+                       ("nu_eps_h", nu[3])):
+            assert abs(d[k] - ref) <= 1e-12 * abs(ref), (k, d[k], ref)
+    inst.close()
