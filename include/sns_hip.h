@@ -156,6 +156,17 @@ int snsg_state_download(snsg_ctx *ctx, const snsg_state *st, int field, double *
 int snsg_diagnostics(snsg_ctx *ctx, const double *ux, const double *uy, const double *temp,
                      double *out);
 int snsg_state_diagnostics(snsg_ctx *ctx, const snsg_state *st, double *out);
+/* ---- statistics (src/statistics/collect.c:10-75): ux1 += ux, ux2 += ux^2
+ * (I = 1..itot+1), uy1/uy2 and temp1/temp2 likewise (I = 0..itot+1), rows
+ * 1..jtot; accumulators in the reference layouts (include/arrays/statistics.h
+ * of the reference), fields in the reference layouts (snsg_statistics_collect)
+ * or the state's (snsg_state_statistics_collect).  temp* may be NULL without
+ * the temperature equation. */
+int snsg_statistics_collect(snsg_ctx *ctx, const double *ux, const double *uy, const double *temp,
+                            double *ux1, double *ux2, double *uy1, double *uy2, double *temp1,
+                            double *temp2);
+int snsg_state_statistics_collect(snsg_ctx *ctx, const snsg_state *st, double *ux1, double *ux2,
+                                  double *uy1, double *uy2, double *temp1, double *temp2);
 /* advective time-step bound of the state's ux, uy (as snsg_param_decide_dt) */
 int snsg_state_decide_dt(snsg_ctx *ctx, const snsg_state *st, double *dt_adv);
 int snsg_set_time(snsg_ctx *ctx, double time);

@@ -90,6 +90,8 @@ def _load_hip():
         L.snsg_state_decide_dt.argtypes = [_vp, C.POINTER(SnsgState), _dp]
         L.snsg_diagnostics.argtypes = [_vp, _vp, _vp, _vp, _dp]
         L.snsg_state_diagnostics.argtypes = [_vp, C.POINTER(SnsgState), _dp]
+        L.snsg_statistics_collect.argtypes = [_vp] * 10
+        L.snsg_state_statistics_collect.argtypes = [_vp, C.POINTER(SnsgState)] + [_vp] * 6
         L.snsg_state_free.argtypes = [_vp, C.POINTER(SnsgState)]
         L.snsg_integrate.argtypes = [_vp, C.POINTER(SnsgState)]
         L.snsg_advance.argtypes = [_vp, C.POINTER(SnsgState), C.c_int]
@@ -179,6 +181,9 @@ FIELD_SHAPES = {
     "srcuyb": lambda i, j: (j, i), "srcuyg": lambda i, j: (j, i),
     "srctempa": lambda i, j: (j, i), "srctempb": lambda i, j: (j, i),
     "srctempg": lambda i, j: (j, i),
+    "ux1": lambda i, j: (j + 2, i + 1), "ux2": lambda i, j: (j + 2, i + 1),
+    "uy1": lambda i, j: (j + 2, i + 2), "uy2": lambda i, j: (j + 2, i + 2),
+    "temp1": lambda i, j: (j + 2, i + 2), "temp2": lambda i, j: (j + 2, i + 2),
 }
 
 
@@ -241,7 +246,7 @@ class Instance:
         return FIELD_SHAPES[name](self.itot, self.jtot)
 
     def alloc(self, *names):
-        for n in names or FIELD_SHAPES.keys():
+        for n in names or [k for k in FIELD_SHAPES if k not in self.STAT_NAMES]:
             if n not in self.buf:
                 nbytes = 8 * int(np.prod(self.shape(n)))
                 ptr = _vp()
@@ -405,6 +410,21 @@ class Instance:
             _check(self.L.snsg_diagnostics(self.ctx, b["ux"], b["uy"], b.get("temp"), _p(out)),
                    "snsg_diagnostics")
         return dict(zip(self.DIAG_NAMES, out))
+
+    STAT_NAMES = ("ux1", "ux2", "uy1", "uy2", "temp1", "temp2")
+
+    def statistics_collect(self, state=False):
+        """One sample into the accumulators ux1 .. temp2 (src/statistics/collect.c),
+        from the instance's ux, uy, temp buffers or from the state."""
+        self.alloc(*self.STAT_NAMES)
+        acc = [self.buf[n] for n in self.STAT_NAMES]
+        if state:
+            _check(self.L.snsg_state_statistics_collect(self.ctx, C.byref(self.state), *acc),
+                   "snsg_state_statistics_collect")
+        else:
+            b = self.buf
+            _check(self.L.snsg_statistics_collect(self.ctx, b["ux"], b["uy"], b.get("temp"), *acc),
+                   "snsg_statistics_collect")
 
     def state_dt_adv_raw(self):
         v = C.c_double(0.0)

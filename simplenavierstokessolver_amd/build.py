@@ -23,7 +23,7 @@ LIB = os.path.join(PKG, "lib")
 HIP_SOURCES = ["sns_stencil.hip", "sns_solve.hip", "sns_poisson.hip", "sns_fused.hip", "sns_pred.hip",
                "sns_diag.hip", "sns_capi.hip"]
 HOST_SOURCES = ["common.c", "parallel.c", "param.c", "fluid.c", "temperature.c", "fileio.c",
-                "save.c", "logging.c"]
+                "save.c", "logging.c", "statistics.c"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off: products stay separately rounded as in the reference's

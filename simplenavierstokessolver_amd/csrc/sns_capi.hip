@@ -669,6 +669,28 @@ extern "C" int snsg_state_diagnostics(snsg_ctx *c, const snsg_state *st, double 
   return diagnostics(c, c->geo_st, st->ux, st->uy, st->temp, out);
 }
 
+extern "C" int snsg_statistics_collect(snsg_ctx *c, const double *ux, const double *uy,
+                                       const double *temp, double *ux1, double *ux2, double *uy1,
+                                       double *uy2, double *temp1, double *temp2) {
+  if (!c || !ux || !uy || !ux1 || !ux2 || !uy1 || !uy2) return set_err(SNSG_E_ARG, __func__);
+  const bool t = c->cfg.with_temperature != 0;
+  if (t && (!temp || !temp1 || !temp2)) return set_err(SNSG_E_ARG, __func__);
+  StatArgs a{ux, uy, t ? temp : nullptr, ux1, ux2, uy1, uy2, t ? temp1 : nullptr,
+             t ? temp2 : nullptr};
+  return run(c, "stat", [&] { launch_stat_collect(c->stream, c->geo, a); });
+}
+
+extern "C" int snsg_state_statistics_collect(snsg_ctx *c, const snsg_state *st, double *ux1,
+                                             double *ux2, double *uy1, double *uy2,
+                                             double *temp1, double *temp2) {
+  if (!c || !st || !st->ux || !ux1 || !ux2 || !uy1 || !uy2) return set_err(SNSG_E_ARG, __func__);
+  const bool t = c->cfg.with_temperature != 0;
+  if (t && (!temp1 || !temp2)) return set_err(SNSG_E_ARG, __func__);
+  StatArgs a{st->ux, st->uy, t ? st->temp : nullptr, ux1, ux2, uy1, uy2, t ? temp1 : nullptr,
+             t ? temp2 : nullptr};
+  return run(c, "stat", [&] { launch_stat_collect(c->stream, c->geo_st, a); });
+}
+
 extern "C" int snsg_state_decide_dt(snsg_ctx *c, const snsg_state *st, double *dt_adv) {
   if (!c || !st || !st->ux || !dt_adv) return set_err(SNSG_E_ARG, __func__);
   int rc = run(c, "dt_reduce", [&] {

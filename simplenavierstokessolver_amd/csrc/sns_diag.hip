@@ -181,3 +181,43 @@ void launch_diag(hipStream_t s, const Geo &g, const DiagArgs &a, double *part, d
 }
 
 }  // namespace sns
+
+namespace sns {
+
+// ---------------------------------------------------------- statistics
+
+// src/statistics/collect.c:10-75: ux1 += ux, ux2 += ux^2 over faces
+// I = 1..itot+1, and the same for uy and T over cells I = 0..itot+1, rows
+// J = 1..jtot.  gf: layout of the fields; the accumulators use the reference
+// layouts (row pitches itot+1 and itot+2).  Element-wise, so the sums equal
+// the sequential ones bit for bit.
+__global__ void __launch_bounds__(256)
+    k_stat_collect(Geo gf, StatArgs a) {
+  const int j = blockIdx.y + 1;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0..itot+1
+  const int itot = gf.itot;
+  if (i > itot + 1) return;
+  const size_t ru = (size_t)j * (itot + 1), rc = (size_t)j * (itot + 2);
+  if (i >= 1) {
+    const double v = a.ux[(size_t)j * gf.pu + (i - 1)];
+    a.ux1[ru + (i - 1)] += v;
+    a.ux2[ru + (i - 1)] += v * v;
+  }
+  {
+    const double v = a.uy[(size_t)j * gf.pc + i];
+    a.uy1[rc + i] += v;
+    a.uy2[rc + i] += v * v;
+  }
+  if (a.temp) {
+    const double v = a.temp[(size_t)j * gf.pc + i];
+    a.temp1[rc + i] += v;
+    a.temp2[rc + i] += v * v;
+  }
+}
+
+void launch_stat_collect(hipStream_t s, const Geo &gf, const StatArgs &a) {
+  hipLaunchKernelGGL(k_stat_collect, dim3((gf.itot + 2 + 255) / 256, gf.jtot), dim3(256), 0, s,
+                     gf, a);
+}
+
+}  // namespace sns

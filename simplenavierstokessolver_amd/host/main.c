@@ -1,8 +1,8 @@
 /*
  * Driver (reference: src/main.c).  Same time loop: decide dt, integrate three
- * RK substages through the reference-named functions, log / save on their
- * schedules, stop at timemax or the wall-time limit, save at the end.
- * Statistics collection (src/statistics/) is not part of this build.
+ * RK substages through the reference-named functions, log / save / collect
+ * statistics on their schedules, stop at timemax or the wall-time limit,
+ * save the fields and the statistics at the end.
  */
 #include <stdio.h>
 #include <sys/stat.h>
@@ -12,6 +12,7 @@
 #include "temperature.h"
 #include "save.h"
 #include "logging.h"
+#include "statistics.h"
 #include "fileio.h"
 
 /* src/main.c:79-101 */
@@ -39,10 +40,12 @@ int main(void){
   mkdir("output", 0777);
   mkdir(FILEIO_SAVE, 0777);
   mkdir(FILEIO_LOG, 0777);
+  mkdir(FILEIO_STAT, 0777);
   param_t       *param       = param_init();
   parallel_t    *parallel    = parallel_init();
   fluid_t       *fluid       = fluid_init(param, parallel);
   temperature_t *temperature = temperature_init(param, parallel);
+  statistics_t  *statistics  = statistics_init(param, parallel);
   param_estimate_cost(param, parallel);
   for(;;){
     param_decide_dt(param, parallel, fluid);
@@ -57,6 +60,10 @@ int main(void){
       save(param, parallel, fluid, temperature);
       param->save.next += param->save.rate;
     }
+    if(param->stat.next < param->time){
+      statistics_collect(param, parallel, fluid, temperature, statistics);
+      param->stat.next += param->stat.rate;
+    }
     if(param->time > param->timemax){
       break;
     }
@@ -67,6 +74,8 @@ int main(void){
   }
   printf("elapsed: %.2f [s]\n", wtimes[1]-wtimes[0]);
   save(param, parallel, fluid, temperature);
+  statistics_output(param, parallel, statistics);
+  statistics_finalise(statistics);
   temperature_finalise(temperature);
   fluid_finalise(fluid);
   parallel_finalise(parallel);

@@ -444,3 +444,34 @@ def test_diagnostics_match_oracle(sns, case):
                        ("nu_eps_h", nu[3])):
             assert abs(d[k] - ref) <= 1e-12 * abs(ref), (k, d[k], ref)
     inst.close()
+
+
+# ------------------------------------------------------------- statistics
+
+@pytest.mark.parametrize("itot,jtot", [(40, 48), (300, 45)])
+@pytest.mark.parametrize("state", [False, True])
+def test_statistics_collect_bitwise(sns, itot, jtot, state):
+    """src/statistics/collect.c: two samples accumulate into ux1 .. temp2
+    exactly as the sequential sums (rows 1..jtot; faces 1..itot+1, cells
+    0..itot+1), from the reference-layout buffers and from the padded state;
+    the halo rows stay zero."""
+    rng = np.random.default_rng(itot + jtot)
+    inst = sns.Instance(itot, jtot)
+    inst.alloc("ux", "uy", "temp")
+    if state:
+        inst.state_alloc()
+    ref = {n: np.zeros(inst.shape(n)) for n in inst.STAT_NAMES}
+    for sample in range(2):
+        for n in ("ux", "uy", "temp"):
+            f = rng.standard_normal(inst.shape(n))
+            if state:
+                inst.state_upload(n, f)
+            else:
+                inst.upload(n, f)
+            ref[n + "1"][1:-1] += f[1:-1]
+            ref[n + "2"][1:-1] += f[1:-1] ** 2
+        inst.statistics_collect(state=state)
+    for n in inst.STAT_NAMES:
+        got = inst.download(n)
+        assert np.array_equal(got.view(np.uint64), ref[n].view(np.uint64)), n
+    inst.close()

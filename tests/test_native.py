@@ -47,7 +47,8 @@ def test_hip_library_exports_every_c_abi_symbol(sns):
 
 
 @pytest.mark.parametrize("header", ["param.h", "fluid.h", "temperature.h", "parallel.h",
-                                    "fileio.h", "save.h", "logging.h", "common.h"])
+                                    "fileio.h", "save.h", "logging.h", "common.h",
+                                    "statistics.h"])
 def test_host_library_exports_reference_call_surface(sns, header):
     names = declared(header, pattern=r"^\s*extern\s+[\w\s\*]+?\b(\w+)\s*\(")
     assert names
