@@ -167,6 +167,28 @@ int snsg_statistics_collect(snsg_ctx *ctx, const double *ux, const double *uy, c
                             double *temp2);
 int snsg_state_statistics_collect(snsg_ctx *ctx, const snsg_state *st, double *ux1, double *ux2,
                                   double *uy1, double *uy2, double *temp1, double *temp2);
+/* ---- batches (no counterpart in the reference, which runs one instance per
+ * process).  Many instances advance together: each kernel of a step is
+ * launched once per shape class (the members with the same itot, jtot,
+ * treatment and temperature equation) for all of the class's members, each
+ * class on a stream of its own, its step replayed from a HIP graph.  Each
+ * member's fields, dt and time are bitwise those of snsg_advance on the
+ * instance alone.  The batch keeps the ctx and state pointers it is given:
+ * the instances must outlive it, and while it exists their states are
+ * advanced through snsg_batch_advance only (a member advanced on its own
+ * must go an even number of steps, or the batch refuses to advance).
+ * Other calls on a member (uploads, downloads, diagnostics) are ordered
+ * with the batch's work through the member's own stream.  Asynchronous like
+ * snsg_advance; snsg_batch_synchronize waits for the batch's work. */
+typedef struct snsg_batch snsg_batch;
+#define SNSG_BATCH_EAGER 1 /* flags: launch every step's kernels instead of replaying graphs */
+int snsg_batch_create(int n, snsg_ctx *const *ctx, snsg_state *const *st, int flags,
+                      snsg_batch **out);
+/* nsteps iterations of the time loop body (as snsg_advance) for every member */
+int snsg_batch_advance(snsg_batch *b, int nsteps);
+int snsg_batch_synchronize(snsg_batch *b);
+int snsg_batch_classes(const snsg_batch *b, int *nclasses);
+int snsg_batch_destroy(snsg_batch *b);
 /* advective time-step bound of the state's ux, uy (as snsg_param_decide_dt) */
 int snsg_state_decide_dt(snsg_ctx *ctx, const snsg_state *st, double *dt_adv);
 int snsg_set_time(snsg_ctx *ctx, double time);

@@ -91,6 +91,12 @@ def _load_hip():
         L.snsg_diagnostics.argtypes = [_vp, _vp, _vp, _vp, _dp]
         L.snsg_state_diagnostics.argtypes = [_vp, C.POINTER(SnsgState), _dp]
         L.snsg_statistics_collect.argtypes = [_vp] * 10
+        L.snsg_batch_create.argtypes = [C.c_int, C.POINTER(_vp), C.POINTER(C.POINTER(SnsgState)),
+                                        C.c_int, C.POINTER(_vp)]
+        L.snsg_batch_advance.argtypes = [_vp, C.c_int]
+        L.snsg_batch_synchronize.argtypes = [_vp]
+        L.snsg_batch_classes.argtypes = [_vp, C.POINTER(C.c_int)]
+        L.snsg_batch_destroy.argtypes = [_vp]
         L.snsg_state_statistics_collect.argtypes = [_vp, C.POINTER(SnsgState)] + [_vp] * 6
         L.snsg_state_free.argtypes = [_vp, C.POINTER(SnsgState)]
         L.snsg_integrate.argtypes = [_vp, C.POINTER(SnsgState)]
@@ -467,3 +473,42 @@ def decide_dt(g, Ra, Pr, dt_adv_raw, implicitx, implicity):
     if implicity:
         return min(dt_adv, dt_dif_x)
     return min(dt_adv, min(dt_dif_x, dt_dif_y))
+
+
+class Batch:
+    """Instances advanced together (snsg_batch_*): each kernel launched once
+    per shape class for all its members, one stream and one replayed HIP
+    graph per class.  The instances must have their states allocated and
+    must outlive the batch."""
+
+    def __init__(self, instances, eager=False):
+        self.L = _load_hip()
+        self.instances = list(instances)
+        n = len(self.instances)
+        ctxs = (_vp * n)(*[i.ctx for i in self.instances])
+        sts = (C.POINTER(SnsgState) * n)(*[C.pointer(i.state) for i in self.instances])
+        self.b = _vp()
+        _check(self.L.snsg_batch_create(n, ctxs, sts, 1 if eager else 0, C.byref(self.b)),
+               "snsg_batch_create")
+
+    def advance(self, nsteps):
+        _check(self.L.snsg_batch_advance(self.b, int(nsteps)), "snsg_batch_advance")
+
+    def sync(self):
+        _check(self.L.snsg_batch_synchronize(self.b), "snsg_batch_synchronize")
+
+    def classes(self):
+        n = C.c_int(0)
+        _check(self.L.snsg_batch_classes(self.b, C.byref(n)), "snsg_batch_classes")
+        return n.value
+
+    def close(self):
+        if self.b:
+            self.L.snsg_batch_destroy(self.b)
+            self.b = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -112,10 +112,15 @@ static int run(snsg_ctx *ctx, const char *name, F fn) {
 static int state_pitch(int n) { return (n + 15) / 16 * 16; }
 static constexpr size_t kStateCOff = 15;
 
+// Zero-filled device allocation.  The fill runs on the null stream, which
+// is not ordered with the instances' non-blocking streams, so it is waited
+// for before the buffer is handed out (a kernel on another stream must not
+// run before it, or the fill could land on the kernel's output).
 static int ctx_alloc(void **p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes ? bytes : 8);
   if (e != hipSuccess) return set_err((int)e, "hipMalloc");
-  e = hipMemset(*p, 0, bytes ? bytes : 8);
+  e = hipMemsetAsync(*p, 0, bytes ? bytes : 8, nullptr);
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
   if (e != hipSuccess) return set_err((int)e, "hipMemset");
   return 0;
 }
@@ -231,13 +236,8 @@ extern "C" int snsg_create(int device, const snsg_config *cfg, const double *dxf
       return fail(set_err(SNSG_E_ARG, "upload column tables"));
     const double *d = c->d_tab;
     Tables &tb = c->tab;
-    tb.dxf = d + 0 * stride; tb.dxc = d + 1 * stride;
-    tb.rdxf = d + 2 * stride; tb.rdxc = d + 3 * stride;
-    tb.cT_xm = d + 4 * stride; tb.cT_xp = d + 5 * stride;
-    tb.cU_xm = d + 6 * stride; tb.cU_xp = d + 7 * stride;
-    tb.nT_xm = d + 8 * stride; tb.nT_xp = d + 9 * stride;
-    tb.nU_xm = d + 10 * stride; tb.nU_xp = d + 11 * stride;
-    tb.pl = d + 12 * stride; tb.pu = d + 13 * stride;
+    tb.t = d;
+    tb.stride = (long)stride;
     // paired copies: (DXC, 1/DXC), (DXF, 1/DXF), (cT_xm, cT_xp), (cU_xm, cU_xp)
     {
       std::vector<double2> h2(4 * stride);
@@ -251,10 +251,7 @@ extern "C" int snsg_create(int device, const snsg_config *cfg, const double *dxf
       if (hipMemcpy(c->d_tab2, h2.data(), sizeof(double2) * h2.size(), hipMemcpyHostToDevice) !=
           hipSuccess)
         return fail(set_err(SNSG_E_ARG, "upload paired column tables"));
-      tb.dxc2 = c->d_tab2;
-      tb.dxf2 = c->d_tab2 + stride;
-      tb.cT2 = c->d_tab2 + 2 * stride;
-      tb.cU2 = c->d_tab2 + 3 * stride;
+      tb.t2 = c->d_tab2;
     }
     tb.dy = dy;
     tb.rdy = 1. / dy;
@@ -396,22 +393,31 @@ static RK rk_of(const snsg_ctx *c, int rkstep) {
 // src/fluid/compute_potential.c:15-168 (three kernel groups: forward
 // transform, x solves, inverse transform)
 // g: the layout of ux, uy and psi (c->geo or c->geo_st)
+static PoisArgs pois_args(const snsg_ctx *c, const Geo &g, double gamma, const double *ux,
+                          const double *uy, double *psi) {
+  PoisArgs a;
+  a.g = g;
+  a.tb = c->tab;
+  a.pl = c->plan;
+  a.gamma = gamma;
+  a.dtp = c->d_dt;
+  a.ux = ux;
+  a.uy = uy;
+  a.tw = c->d_tw;
+  a.fac = c->d_pfac;
+  a.Y = c->d_Y;
+  a.psihat = c->d_psihat;
+  a.psi = psi;
+  return a;
+}
+
 static int potential(snsg_ctx *c, const Geo &g, double gamma, const double *ux, const double *uy,
                      double *psi) {
-  int rc = run(c, "poisson_fwd", [&] {
-    launch_poisson2(c->stream, g, c->tab, c->plan, gamma, c->d_dt, ux, uy, c->d_tw, c->d_pfac,
-                    c->d_Y, c->d_psihat, psi, 0);
-  });
+  const PoisArgs a = pois_args(c, g, gamma, ux, uy, psi);
+  int rc = run(c, "poisson_fwd", [&] { launch_poisson(c->stream, a, 0); });
   if (rc) return rc;
-  if ((rc = run(c, "poisson_x", [&] {
-         launch_poisson2(c->stream, g, c->tab, c->plan, gamma, c->d_dt, ux, uy, c->d_tw,
-                         c->d_pfac, c->d_Y, c->d_psihat, psi, 1);
-       })))
-    return rc;
-  return run(c, "poisson_inv", [&] {
-    launch_poisson2(c->stream, g, c->tab, c->plan, gamma, c->d_dt, ux, uy, c->d_tw, c->d_pfac,
-                    c->d_Y, c->d_psihat, psi, 2);
-  });
+  if ((rc = run(c, "poisson_x", [&] { launch_poisson(c->stream, a, 1); }))) return rc;
+  return run(c, "poisson_inv", [&] { launch_poisson(c->stream, a, 2); });
 }
 
 static int update_field(snsg_ctx *c, int kind, int rkstep, const double *srca, const double *srcb,
@@ -705,47 +711,80 @@ extern "C" int snsg_state_decide_dt(snsg_ctx *c, const snsg_state *st, double *d
   return 0;
 }
 
-// One RK3 step (src/main.c:79-101) with the fused kernels; dt is read from
-// the device slot (snsg_set_dt or the device time-step kernel).  With
-// `reduce`, the last correction also folds the advective bound for the next
-// step into d_red.
-static int integrate_fused(snsg_ctx *c, snsg_state *st, bool reduce) {
+// Argument blocks of one fused RK3 step (src/main.c:79-101) of instance c on
+// state st, with st->temp and st->temp_alt as they are at the start of the
+// step (T and T' trade places after every substage).  `reduce`: the last
+// correction stores the advective-bound minima for the next step in d_part.
+struct StepArgs {
+  LuFactorArgs lu;
+  DtArgs dt;
+  CorrArgs bound;  // advective bound of the state as it is (d_part)
+  PredArgs pred[3];
+  YArgs y[3][3];   // [substage][T, ux, uy]
+  PoisArgs pois[3];
+  CorrArgs corr[3];
+};
+
+static const int kYKinds[3] = {KIND_TEMP, KIND_UX, KIND_UY};
+
+// Scratch of the fused step, allocated on first use
+static int step_prepare(snsg_ctx *c) {
   const Geo &g = c->geo_st;
   const int itot = g.itot, jtot = g.jtot;
   int rc;
-  const bool impx = c->treat.implicitx, impy = c->treat.implicity;
-  if (impx && !pred_impx_supported(itot))
+  if (c->treat.implicitx && !pred_impx_supported(itot))
     return set_err(SNSG_E_SIZE, "fused step: implicit x needs itot <= 8192");
-  if (jtot < 2) return set_err(SNSG_E_SIZE, "fused step: jtot must be at least 2");
-  if (impy && !c->d_qf[0]) {
+  if (c->treat.implicity && !c->d_qf[0]) {
     const size_t nb[3] = {(size_t)itot * jtot, (size_t)(itot - 1) * jtot, (size_t)itot * jtot};
     for (int k = 0; k < 3; k++)
       if ((rc = ctx_alloc((void **)&c->d_qf[k], sizeof(double) * nb[k]))) return rc;
   }
-  const bool with_temp = c->cfg.with_temperature != 0;
-  if (impx) {
-    if (!c->d_lu) {
-      c->lu_ntm = lu_table_entries(itot);
-      if ((rc = ctx_alloc((void **)&c->d_lu, sizeof(double) * 27 * (size_t)c->lu_ntm))) return rc;
-    }
-    // the CN-x operators of all three substages depend on dt only
-    LuFactorArgs la;
-    la.g = g;
-    la.tb = c->tab;
-    la.dtp = c->d_dt;
-    for (int k = 0; k < 3; k++) la.gamma[k] = c->cfg.gamma[k];
-    la.out = c->d_lu;
-    la.ntm = c->lu_ntm;
-    if ((rc = run(c, "lu_factor", [&] { launch_lu_factor(c->stream, la); }))) return rc;
+  if (c->treat.implicitx && !c->d_lu) {
+    c->lu_ntm = lu_table_entries(itot);
+    if ((rc = ctx_alloc((void **)&c->d_lu, sizeof(double) * 27 * (size_t)c->lu_ntm))) return rc;
   }
+  if (!c->d_part) {
+    c->nparts = corr_partials(g);
+    if ((rc = ctx_alloc((void **)&c->d_part, sizeof(double) * (size_t)c->nparts))) return rc;
+  }
+  return 0;
+}
+
+static void step_args(const snsg_ctx *c, const snsg_state *st, bool reduce, StepArgs &sa) {
+  const Geo &g = c->geo_st;
+  const bool impx = c->treat.implicitx, impy = c->treat.implicity;
+  const bool with_temp = c->cfg.with_temperature != 0;
+  sa = StepArgs{};
+  // the CN-x operators of all three substages depend on dt only
+  sa.lu.g = g;
+  sa.lu.tb = c->tab;
+  sa.lu.dtp = c->d_dt;
+  for (int k = 0; k < 3; k++) sa.lu.gamma[k] = c->cfg.gamma[k];
+  sa.lu.out = c->d_lu;
+  sa.lu.ntm = c->lu_ntm;
+  sa.dt.part = c->d_part;
+  sa.dt.n = c->nparts;
+  sa.dt.dt = c->d_dt;
+  sa.dt.time = c->d_time;
+  sa.dt.dt_dif_x = c->dt_dif_x;
+  sa.dt.dt_dif_y = c->dt_dif_y;
+  sa.dt.implicitx = impx;
+  sa.dt.implicity = impy;
+  sa.bound.g = g;
+  sa.bound.tb = c->tab;
+  sa.bound.tr = c->treat;
+  sa.bound.ux = st->ux;
+  sa.bound.uy = st->uy;
+  sa.bound.red = c->d_part;
+  double *temp = st->temp, *temp_alt = st->temp_alt;
   for (int rk = 0; rk < 3; rk++) {
-    PredArgs pa;
+    PredArgs &pa = sa.pred[rk];
     pa.g = g;
     pa.tb = c->tab;
     pa.tr = c->treat;
-    pa.ux = st->ux; pa.uy = st->uy; pa.p = st->p; pa.temp = st->temp;
+    pa.ux = st->ux; pa.uy = st->uy; pa.p = st->p; pa.temp = temp;
     pa.hist_ux = st->srcuxa; pa.hist_uy = st->srcuya; pa.hist_t = st->srctempa;
-    pa.out_u = st->ux_star; pa.out_v = st->uy_star; pa.out_t = st->temp_alt;
+    pa.out_u = st->ux_star; pa.out_v = st->uy_star; pa.out_t = temp_alt;
     pa.q_u = impy ? c->d_qf[1] : nullptr;
     pa.q_v = impy ? c->d_qf[2] : nullptr;
     pa.q_t = impy ? c->d_qf[0] : nullptr;
@@ -755,38 +794,60 @@ static int integrate_fused(snsg_ctx *c, snsg_state *st, bool reduce) {
     pa.write_hist = rk < 2;
     pa.with_temp = with_temp;
     pa.forcing = with_temp && c->cfg.with_thermal_forcing;
-    const double *lu = c->d_lu ? c->d_lu + (size_t)(rk * 3 * 3) * c->lu_ntm : nullptr;
-    if ((rc = run(c, "pred", [&] { launch_pred(c->stream, pa, impx, lu, c->lu_ntm); })))
-      return rc;
-    if (impy) {
-      const RK rkc = rk_of(c, rk);
-      const int kinds[3] = {KIND_TEMP, KIND_UX, KIND_UY};
-      const double *ins[3] = {st->temp, st->ux, st->uy};
-      double *outs[3] = {st->temp_alt, st->ux_star, st->uy_star};
-      for (int f = 0; f < 3; f++) {
-        if (f == 0 && !with_temp) continue;
-        const int n = kinds[f] == KIND_UX ? itot - 1 : itot;
-        rc = run(c, "ysolve", [&] {
-          launch_transpose(c->stream, n, jtot, 8, c->d_qf[f], c->d_qy);
-          launch_cny_rows(c->stream, g, kinds[f], rkc, c->d_dt, n, c->d_qy);
-          launch_transpose(c->stream, jtot, n, 8, c->d_qy, c->d_qf[f]);
-          launch_add_out(c->stream, g, kinds[f], ins[f], c->d_qf[f], outs[f]);
-        });
-        if (rc) return rc;
-      }
+    pa.lu = c->d_lu ? c->d_lu + (size_t)(rk * 3 * 3) * c->lu_ntm : nullptr;
+    pa.ntm = c->lu_ntm;
+    const double *ins[3] = {temp, st->ux, st->uy};
+    double *outs[3] = {temp_alt, st->ux_star, st->uy_star};
+    for (int f = 0; f < 3; f++) {
+      YArgs &y = sa.y[rk][f];
+      y.g = g;
+      y.rk = rk_of(c, rk);
+      y.dtp = c->d_dt;
+      y.n = kYKinds[f] == KIND_UX ? g.itot - 1 : g.itot;
+      y.in = ins[f];
+      y.qf = c->d_qf[f];
+      y.qy = c->d_qy;
+      y.out = outs[f];
     }
-    const double gamma = c->cfg.gamma[rk];
-    if ((rc = potential(c, c->geo_st, gamma, st->ux_star, st->uy_star, st->psi))) return rc;
-    CorrArgs ca;
+    sa.pois[rk] = pois_args(c, g, c->cfg.gamma[rk], st->ux_star, st->uy_star, st->psi);
+    CorrArgs &ca = sa.corr[rk];
     ca.g = g;
     ca.tb = c->tab;
     ca.tr = c->treat;
     ca.ux_star = st->ux_star; ca.uy_star = st->uy_star; ca.psi = st->psi;
     ca.ux = st->ux; ca.uy = st->uy; ca.p = st->p;
     ca.dtp = c->d_dt;
-    ca.gamma = gamma;
+    ca.gamma = c->cfg.gamma[rk];
     ca.red = (reduce && rk == 2) ? c->d_part : nullptr;
-    if ((rc = run(c, "corr", [&] { launch_corr(c->stream, ca); }))) return rc;
+    if (with_temp) std::swap(temp, temp_alt);
+  }
+}
+
+// One RK3 step with the fused kernels; dt is read from the device slot
+// (snsg_set_dt or the device time-step kernel).
+static int integrate_fused(snsg_ctx *c, snsg_state *st, bool reduce) {
+  int rc;
+  if (c->geo_st.jtot < 2) return set_err(SNSG_E_SIZE, "fused step: jtot must be at least 2");
+  if ((rc = step_prepare(c))) return rc;
+  StepArgs sa;
+  step_args(c, st, reduce, sa);
+  const bool impx = c->treat.implicitx, impy = c->treat.implicity;
+  const bool with_temp = c->cfg.with_temperature != 0;
+  if (impx && (rc = run(c, "lu_factor", [&] { launch_lu_factor(c->stream, sa.lu); }))) return rc;
+  for (int rk = 0; rk < 3; rk++) {
+    if ((rc = run(c, "pred", [&] { launch_pred(c->stream, sa.pred[rk], impx); }))) return rc;
+    if (impy) {
+      for (int f = 0; f < 3; f++) {
+        if (f == 0 && !with_temp) continue;
+        rc = run(c, "ysolve", [&] { launch_ysolve(c->stream, kYKinds[f], sa.y[rk][f]); });
+        if (rc) return rc;
+      }
+    }
+    const PoisArgs &pa = sa.pois[rk];
+    if ((rc = run(c, "poisson_fwd", [&] { launch_poisson(c->stream, pa, 0); }))) return rc;
+    if ((rc = run(c, "poisson_x", [&] { launch_poisson(c->stream, pa, 1); }))) return rc;
+    if ((rc = run(c, "poisson_inv", [&] { launch_poisson(c->stream, pa, 2); }))) return rc;
+    if ((rc = run(c, "corr", [&] { launch_corr(c->stream, sa.corr[rk]); }))) return rc;
     if (with_temp) std::swap(st->temp, st->temp_alt);
   }
   return 0;
@@ -797,28 +858,18 @@ extern "C" int snsg_integrate(snsg_ctx *c, snsg_state *st) {
   return integrate_fused(c, st, false);
 }
 
+// Each step decides dt from the partial minima of the advective bound: the
+// first from those of the state as passed in (k_adv_bound), later ones from
+// those the previous step's last correction stored.
 extern "C" int snsg_advance(snsg_ctx *c, snsg_state *st, int nsteps) {
   if (!c || !st || !st->ux || nsteps < 0) return set_err(SNSG_E_ARG, __func__);
   int rc;
-  if (!c->d_part) {
-    c->nparts = corr_partials(c->geo);
-    if ((rc = ctx_alloc((void **)&c->d_part, sizeof(double) * (size_t)c->nparts))) return rc;
-  }
-  rc = run(c, "dt_reduce", [&] {
-    launch_fill_u64(c->stream, c->d_red, 0x7ff0000000000000ull, 1);
-    launch_dt_reduce(c->stream, c->geo_st, st->ux, st->uy, c->d_red);
-  });
-  if (rc) return rc;
+  if ((rc = step_prepare(c))) return rc;
+  StepArgs sa;
+  step_args(c, st, true, sa);
+  if ((rc = run(c, "dt_reduce", [&] { launch_adv_bound(c->stream, sa.bound); }))) return rc;
   for (int n = 0; n < nsteps; n++) {
-    // step 0 starts from the full reduction (one positive double in d_red),
-    // later steps from the minima the previous step's last correction stored
-    const double *part = n == 0 ? reinterpret_cast<const double *>(c->d_red) : c->d_part;
-    const int np = n == 0 ? 1 : c->nparts;
-    rc = run(c, "dt", [&] {
-      launch_dt(c->stream, part, np, c->d_dt, c->d_time, c->dt_dif_x, c->dt_dif_y,
-                c->treat.implicitx, c->treat.implicity);
-    });
-    if (rc) return rc;
+    if ((rc = run(c, "dt", [&] { launch_dt(c->stream, sa.dt); }))) return rc;
     if ((rc = integrate_fused(c, st, true))) return rc;
   }
   return 0;
@@ -873,5 +924,312 @@ extern "C" int snsg_timing_read(snsg_ctx *c, int *launches, double *total_ms) {
   *launches = (int)(c->ev_used / 2);
   *total_ms = tot;
   c->ev_used = 0;
+  return 0;
+}
+
+// ------------------------------------------------------------ batches
+
+// Instances of one shape class: same itot, jtot, treatment and temperature
+// equation, hence the same launch configuration for every kernel.  Each
+// kernel of a step is launched once for the class, with grid.z = number of
+// members and the members' argument blocks in device tables.  Only the
+// predictor and the y solve depend on which of the two temperature buffers
+// holds T (the parity), so they have a table per parity.
+struct BatchClass {
+  std::vector<int> members;
+  int impx = 0, impy = 0, with_temp = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  StepArgs shape;  // argument blocks of the first member (launch configuration)
+  char *d_tab = nullptr;
+  LuFactorArgs *lu = nullptr;
+  DtArgs *dt = nullptr;
+  CorrArgs *bound = nullptr;
+  PredArgs *pred[2][3] = {};
+  YArgs *y[2][3][3] = {};
+  PoisArgs *pois[3] = {};
+  CorrArgs *corr[3] = {};
+  hipGraph_t graph[2] = {};
+  hipGraphExec_t exec[2] = {};
+};
+
+struct snsg_batch {
+  int device = 0;
+  int flags = 0;
+  std::vector<snsg_ctx *> ctx;
+  std::vector<snsg_state *> st;
+  std::vector<double *> temp0;  // st->temp of parity 0
+  std::vector<BatchClass> cls;
+  std::vector<hipEvent_t> ready;  // one per member stream
+};
+
+static void batch_free(snsg_batch *b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  for (BatchClass &k : b->cls) {
+    if (k.stream) (void)hipStreamSynchronize(k.stream);
+    for (int p = 0; p < 2; p++) {
+      if (k.exec[p]) (void)hipGraphExecDestroy(k.exec[p]);
+      if (k.graph[p]) (void)hipGraphDestroy(k.graph[p]);
+    }
+    if (k.d_tab) (void)hipFree(k.d_tab);
+    if (k.done) (void)hipEventDestroy(k.done);
+    if (k.stream) (void)hipStreamDestroy(k.stream);
+  }
+  for (hipEvent_t e : b->ready) (void)hipEventDestroy(e);
+  delete b;
+}
+
+// Lays out the class's argument tables in one device allocation.
+static int batch_tables(snsg_batch *b, BatchClass &k) {
+  const size_t nz = k.members.size();
+  std::vector<char> host;
+  auto reserve = [&](size_t bytes) {
+    const size_t off = (host.size() + 255) / 256 * 256;
+    host.resize(off + bytes);
+    return off;
+  };
+  std::vector<StepArgs> sa[2];
+  for (int p = 0; p < 2; p++) {
+    sa[p].resize(nz);
+    for (size_t m = 0; m < nz; m++) {
+      const int id = k.members[m];
+      snsg_state s = *b->st[id];
+      if (p == 1 && k.with_temp) std::swap(s.temp, s.temp_alt);
+      step_args(b->ctx[id], &s, true, sa[p][m]);
+    }
+  }
+  k.shape = sa[0][0];
+  // offsets first, then pointers into the device allocation
+  const size_t o_lu = reserve(nz * sizeof(LuFactorArgs));
+  const size_t o_dt = reserve(nz * sizeof(DtArgs));
+  const size_t o_bound = reserve(nz * sizeof(CorrArgs));
+  size_t o_pred[2][3], o_y[2][3][3], o_pois[3], o_corr[3];
+  for (int p = 0; p < 2; p++)
+    for (int rk = 0; rk < 3; rk++) {
+      o_pred[p][rk] = reserve(nz * sizeof(PredArgs));
+      for (int f = 0; f < 3; f++) o_y[p][rk][f] = reserve(nz * sizeof(YArgs));
+    }
+  for (int rk = 0; rk < 3; rk++) {
+    o_pois[rk] = reserve(nz * sizeof(PoisArgs));
+    o_corr[rk] = reserve(nz * sizeof(CorrArgs));
+  }
+  auto put = [&](size_t off, size_t m, const void *src, size_t sz) {
+    memcpy(host.data() + off + m * sz, src, sz);
+  };
+  for (size_t m = 0; m < nz; m++) {
+    put(o_lu, m, &sa[0][m].lu, sizeof(LuFactorArgs));
+    put(o_dt, m, &sa[0][m].dt, sizeof(DtArgs));
+    put(o_bound, m, &sa[0][m].bound, sizeof(CorrArgs));
+    for (int p = 0; p < 2; p++)
+      for (int rk = 0; rk < 3; rk++) {
+        put(o_pred[p][rk], m, &sa[p][m].pred[rk], sizeof(PredArgs));
+        for (int f = 0; f < 3; f++) put(o_y[p][rk][f], m, &sa[p][m].y[rk][f], sizeof(YArgs));
+      }
+    for (int rk = 0; rk < 3; rk++) {
+      put(o_pois[rk], m, &sa[0][m].pois[rk], sizeof(PoisArgs));
+      put(o_corr[rk], m, &sa[0][m].corr[rk], sizeof(CorrArgs));
+    }
+  }
+  int rc;
+  if ((rc = ctx_alloc((void **)&k.d_tab, host.size()))) return rc;
+  HIPCHECK(hipMemcpy(k.d_tab, host.data(), host.size(), hipMemcpyHostToDevice));
+  char *d = k.d_tab;
+  k.lu = (LuFactorArgs *)(d + o_lu);
+  k.dt = (DtArgs *)(d + o_dt);
+  k.bound = (CorrArgs *)(d + o_bound);
+  for (int p = 0; p < 2; p++)
+    for (int rk = 0; rk < 3; rk++) {
+      k.pred[p][rk] = (PredArgs *)(d + o_pred[p][rk]);
+      for (int f = 0; f < 3; f++) k.y[p][rk][f] = (YArgs *)(d + o_y[p][rk][f]);
+    }
+  for (int rk = 0; rk < 3; rk++) {
+    k.pois[rk] = (PoisArgs *)(d + o_pois[rk]);
+    k.corr[rk] = (CorrArgs *)(d + o_corr[rk]);
+  }
+  return 0;
+}
+
+// One time-loop iteration (dt decision + RK3 step) of every member of class
+// k, whose T is in the parity-p buffer, enqueued on the class stream.
+static void batch_step_launch(const BatchClass &k, int p) {
+  const int nz = (int)k.members.size();
+  const hipStream_t s = k.stream;
+  const StepArgs &sh = k.shape;
+  launch_dt_batch(s, k.dt, nz);
+  if (k.impx) launch_lu_factor_batch(s, sh.lu, k.lu, nz);
+  for (int rk = 0; rk < 3; rk++) {
+    launch_pred_batch(s, sh.pred[rk], k.impx, k.pred[p][rk], nz);
+    if (k.impy)
+      for (int f = 0; f < 3; f++) {
+        if (f == 0 && !k.with_temp) continue;
+        launch_ysolve_batch(s, kYKinds[f], sh.y[rk][f], k.y[p][rk][f], nz);
+      }
+    for (int stage = 0; stage < 3; stage++)
+      launch_poisson_batch(s, sh.pois[rk], k.pois[rk], nz, stage);
+    launch_corr_batch(s, sh.corr[rk], k.corr[rk], nz);
+  }
+}
+
+static int batch_capture(BatchClass &k, int p) {
+  HIPCHECK(hipStreamBeginCapture(k.stream, hipStreamCaptureModeThreadLocal));
+  batch_step_launch(k, p);
+  const hipError_t le = hipGetLastError();
+  hipGraph_t g = nullptr;
+  const hipError_t ce = hipStreamEndCapture(k.stream, &g);
+  if (le != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return set_err((int)le, "batch: launch during graph capture");
+  }
+  if (ce != hipSuccess) return set_err((int)ce, "hipStreamEndCapture");
+  k.graph[p] = g;
+  HIPCHECK(hipGraphInstantiate(&k.exec[p], g, nullptr, nullptr, 0));
+  return 0;
+}
+
+extern "C" int snsg_batch_create(int n, snsg_ctx *const *ctx, snsg_state *const *st, int flags,
+                                 snsg_batch **out) {
+  if (n < 1 || !ctx || !st || !out) return set_err(SNSG_E_ARG, "snsg_batch_create: arguments");
+  for (int m = 0; m < n; m++) {
+    if (!ctx[m] || !st[m] || !st[m]->ux)
+      return set_err(SNSG_E_ARG, "snsg_batch_create: null instance or unallocated state");
+    if (ctx[m]->device != ctx[0]->device)
+      return set_err(SNSG_E_ARG, "snsg_batch_create: instances on different devices");
+    for (int q = 0; q < m; q++)
+      if (st[q] == st[m] || st[q]->ux == st[m]->ux || ctx[q] == ctx[m])
+        return set_err(SNSG_E_ARG, "snsg_batch_create: an instance or state appears twice");
+  }
+  HIPCHECK(hipSetDevice(ctx[0]->device));
+  snsg_batch *b = new snsg_batch();
+  b->device = ctx[0]->device;
+  b->flags = flags;
+  b->ctx.assign(ctx, ctx + n);
+  b->st.assign(st, st + n);
+  int rc = 0;
+  auto fail = [&](int code) {
+    batch_free(b);
+    return code;
+  };
+  for (int m = 0; m < n; m++) {
+    snsg_ctx *c = ctx[m];
+    if (c->geo_st.jtot < 2) return fail(set_err(SNSG_E_SIZE, "snsg_batch_create: jtot < 2"));
+    if ((rc = step_prepare(c))) return fail(rc);
+    b->temp0.push_back(st[m]->temp);
+    // shape class
+    const int impx = c->treat.implicitx, impy = c->treat.implicity;
+    const int wt = c->cfg.with_temperature != 0;
+    int found = -1;
+    for (size_t k = 0; k < b->cls.size(); k++) {
+      const snsg_ctx *r = ctx[b->cls[k].members[0]];
+      if (r->geo.itot == c->geo.itot && r->geo.jtot == c->geo.jtot && b->cls[k].impx == impx &&
+          b->cls[k].impy == impy && b->cls[k].with_temp == wt)
+        found = (int)k;
+    }
+    if (found < 0) {
+      b->cls.emplace_back();
+      BatchClass &k = b->cls.back();
+      k.impx = impx;
+      k.impy = impy;
+      k.with_temp = wt;
+      found = (int)b->cls.size() - 1;
+    }
+    b->cls[found].members.push_back(m);
+  }
+  for (BatchClass &k : b->cls) {
+    if (hipStreamCreateWithFlags(&k.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&k.done, hipEventDisableTiming) != hipSuccess)
+      return fail(set_err(SNSG_E_ARG, "snsg_batch_create: stream or event"));
+    if ((rc = batch_tables(b, k))) return fail(rc);
+  }
+  b->ready.resize(n, nullptr);
+  for (int m = 0; m < n; m++)
+    if (hipEventCreateWithFlags(&b->ready[m], hipEventDisableTiming) != hipSuccess)
+      return fail(set_err(SNSG_E_ARG, "snsg_batch_create: event"));
+  *out = b;
+  return 0;
+}
+
+// The parity of the members of class k (0: T in the buffer it was in at
+// creation), or -1 if they differ (some member was advanced on its own an
+// odd number of steps).
+static int batch_parity(const snsg_batch *b, const BatchClass &k) {
+  int par = -2;
+  for (int id : k.members) {
+    const int p = b->st[id]->temp == b->temp0[id] ? 0 : 1;
+    if (par == -2) par = p;
+    else if (p != par) return -1;
+  }
+  return k.with_temp ? par : 0;
+}
+
+extern "C" int snsg_batch_advance(snsg_batch *b, int nsteps) {
+  if (!b || nsteps < 0) return set_err(SNSG_E_ARG, __func__);
+  if (nsteps == 0) return 0;
+  HIPCHECK(hipSetDevice(b->device));
+  std::vector<int> par(b->cls.size());
+  for (size_t q = 0; q < b->cls.size(); q++) {
+    par[q] = batch_parity(b, b->cls[q]);
+    if (par[q] < 0)
+      return set_err(SNSG_E_ARG,
+                     "snsg_batch_advance: members of a class were advanced apart by an odd "
+                     "number of steps");
+  }
+  // the class streams start after the work already queued on the members' streams
+  for (size_t m = 0; m < b->ctx.size(); m++)
+    HIPCHECK(hipEventRecord(b->ready[m], b->ctx[m]->stream));
+  for (BatchClass &k : b->cls) {
+    for (int id : k.members) HIPCHECK(hipStreamWaitEvent(k.stream, b->ready[id], 0));
+    // first dt decision from the state as it is
+    launch_adv_bound_batch(k.stream, k.shape.bound, k.bound, (int)k.members.size());
+    int rc = check_launch("snsg_batch_advance: advective bound");
+    if (rc) return rc;
+  }
+  const bool graphs = !(b->flags & SNSG_BATCH_EAGER);
+  for (int n = 0; n < nsteps; n++) {
+    for (size_t q = 0; q < b->cls.size(); q++) {
+      BatchClass &k = b->cls[q];
+      const int p = par[q];
+      if (graphs && k.exec[p]) {
+        HIPCHECK(hipGraphLaunch(k.exec[p], k.stream));
+      } else {
+        // eager launches (also the first step of a graph-replayed class, which
+        // sets up the kernels before they are recorded)
+        batch_step_launch(k, p);
+        int rc = check_launch("snsg_batch_advance");
+        if (rc) return rc;
+        if (graphs && !k.exec[p]) {
+          if ((rc = batch_capture(k, p))) return rc;
+        }
+      }
+      if (k.with_temp) par[q] ^= 1;
+    }
+  }
+  for (BatchClass &k : b->cls) {
+    HIPCHECK(hipEventRecord(k.done, k.stream));
+    for (int id : k.members) HIPCHECK(hipStreamWaitEvent(b->ctx[id]->stream, k.done, 0));
+  }
+  // T and T' trade places once per step
+  if (nsteps & 1)
+    for (BatchClass &k : b->cls)
+      if (k.with_temp)
+        for (int id : k.members) std::swap(b->st[id]->temp, b->st[id]->temp_alt);
+  return 0;
+}
+
+extern "C" int snsg_batch_synchronize(snsg_batch *b) {
+  if (!b) return set_err(SNSG_E_ARG, __func__);
+  HIPCHECK(hipSetDevice(b->device));
+  for (BatchClass &k : b->cls) HIPCHECK(hipStreamSynchronize(k.stream));
+  return 0;
+}
+
+extern "C" int snsg_batch_classes(const snsg_batch *b, int *nclasses) {
+  if (!b || !nclasses) return set_err(SNSG_E_ARG, __func__);
+  *nclasses = (int)b->cls.size();
+  return 0;
+}
+
+extern "C" int snsg_batch_destroy(snsg_batch *b) {
+  batch_free(b);
   return 0;
 }

@@ -31,20 +31,39 @@ __device__ __forceinline__ double div_cr(double a, double b, double r) {
 }
 
 // Per-column tables, all indexed like the reference arrays they derive from
-// (entry I-1 holds the value for column I unless noted).
+// (entry I-1 holds the value for column I unless noted).  They live in one
+// allocation of 14 double tables and one of 4 double2 tables, each table
+// `stride` entries long, so an instance's tables cost two pointers.
 struct Tables {
-  const double *dxf, *dxc;     // DXF(I), DXC(I)
-  const double *rdxf, *rdxc;   // 1./DXF(I), 1./DXC(I)
-  const double *cT_xm, *cT_xp; // DXC(I)/(2.*DXF(I)), DXC(I+1)/(2.*DXF(I))   I = 1..itot
-  const double *cU_xm, *cU_xp; // DXF(I-1)/(2.*DXC(I)), DXF(I)/(2.*DXC(I))   I = 2..itot (entry I-1)
-  const double *nT_xm, *nT_xp; // 1./DXF(I)/DXC(I), 1./DXF(I)/DXC(I+1)       (CN of T and uy)
-  const double *nU_xm, *nU_xp; // 1./DXC(I)/DXF(I-1), 1./DXC(I)/DXF(I)       (CN of ux)
-  const double *pl, *pu;       // 1./DXC(I)/DXF(I), 1./DXC(I+1)/DXF(I)       (Poisson)
-  // the same values in pairs, for one 16-byte load each (entry as above)
-  const double2 *dxc2;         // (DXC(I), 1./DXC(I))
-  const double2 *dxf2;         // (DXF(I), 1./DXF(I))
-  const double2 *cT2, *cU2;    // (cT_xm, cT_xp), (cU_xm, cU_xp)
+  const double *t;             // 14 tables, in the order of the accessors below
+  const double2 *t2;           // 4 paired tables
+  long stride;
   double dy, rdy;
+  __host__ __device__ const double *tab(int k) const { return t + k * stride; }
+  __host__ __device__ const double *dxf() const { return tab(0); }    // DXF(I)
+  __host__ __device__ const double *dxc() const { return tab(1); }    // DXC(I)
+  __host__ __device__ const double *rdxf() const { return tab(2); }   // 1./DXF(I)
+  __host__ __device__ const double *rdxc() const { return tab(3); }   // 1./DXC(I)
+  // DXC(I)/(2.*DXF(I)), DXC(I+1)/(2.*DXF(I))   I = 1..itot
+  __host__ __device__ const double *cT_xm() const { return tab(4); }
+  __host__ __device__ const double *cT_xp() const { return tab(5); }
+  // DXF(I-1)/(2.*DXC(I)), DXF(I)/(2.*DXC(I))   I = 2..itot (entry I-1)
+  __host__ __device__ const double *cU_xm() const { return tab(6); }
+  __host__ __device__ const double *cU_xp() const { return tab(7); }
+  // 1./DXF(I)/DXC(I), 1./DXF(I)/DXC(I+1)       (CN of T and uy)
+  __host__ __device__ const double *nT_xm() const { return tab(8); }
+  __host__ __device__ const double *nT_xp() const { return tab(9); }
+  // 1./DXC(I)/DXF(I-1), 1./DXC(I)/DXF(I)       (CN of ux)
+  __host__ __device__ const double *nU_xm() const { return tab(10); }
+  __host__ __device__ const double *nU_xp() const { return tab(11); }
+  // 1./DXC(I)/DXF(I), 1./DXC(I+1)/DXF(I)       (Poisson)
+  __host__ __device__ const double *pl() const { return tab(12); }
+  __host__ __device__ const double *pu() const { return tab(13); }
+  // the same values in pairs, for one 16-byte load each (entry as above)
+  __host__ __device__ const double2 *dxc2() const { return t2; }               // (DXC, 1./DXC)
+  __host__ __device__ const double2 *dxf2() const { return t2 + stride; }      // (DXF, 1./DXF)
+  __host__ __device__ const double2 *cT2() const { return t2 + 2 * stride; }   // (cT_xm, cT_xp)
+  __host__ __device__ const double2 *cU2() const { return t2 + 3 * stride; }   // (cU_xm, cU_xp)
 };
 
 __device__ __forceinline__ double2 ldr2(const double2 *row, int k) {
@@ -176,12 +195,12 @@ __device__ __forceinline__ void uy_src_v(const Geo &g, const Tables &tb, const C
 // ------------------------------------------------ loaders (pointer form)
 
 __device__ __forceinline__ CellX cell_x(const Tables &tb, int i) {
-  return {ldr(tb.dxc, i - 1), ldr(tb.rdxc, i - 1), ldr(tb.dxc, i),   ldr(tb.rdxc, i),
-          ldr(tb.dxf, i - 1), ldr(tb.rdxf, i - 1), ldr(tb.cT_xm, i - 1), ldr(tb.cT_xp, i - 1)};
+  return {ldr(tb.dxc(), i - 1), ldr(tb.rdxc(), i - 1), ldr(tb.dxc(), i),   ldr(tb.rdxc(), i),
+          ldr(tb.dxf(), i - 1), ldr(tb.rdxf(), i - 1), ldr(tb.cT_xm(), i - 1), ldr(tb.cT_xp(), i - 1)};
 }
 __device__ __forceinline__ FaceX face_x(const Tables &tb, int i) {
-  return {ldr(tb.dxf, i - 2),   ldr(tb.rdxf, i - 2),  ldr(tb.dxf, i - 1), ldr(tb.rdxf, i - 1),
-          ldr(tb.cU_xm, i - 1), ldr(tb.cU_xp, i - 1), ldr(tb.dxc, i - 1), ldr(tb.rdxc, i - 1)};
+  return {ldr(tb.dxf(), i - 2),   ldr(tb.rdxf(), i - 2),  ldr(tb.dxf(), i - 1), ldr(tb.rdxf(), i - 1),
+          ldr(tb.cU_xm(), i - 1), ldr(tb.cU_xp(), i - 1), ldr(tb.dxc(), i - 1), ldr(tb.rdxc(), i - 1)};
 }
 
 __device__ __forceinline__ void temp_src_t(const Geo &g, const Tables &tb, const Rows3 &ux,

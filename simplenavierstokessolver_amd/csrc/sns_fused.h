@@ -18,6 +18,8 @@ struct PredArgs {
   int read_hist, write_hist;
   int with_temp;                          // temperature equation integrated
   int forcing;                            // buoyancy term from T
+  const double *lu;                       // implicit x: LU tables of the substage (3 fields)
+  int ntm;                                // entries per LU table
 };
 
 struct CorrArgs {
@@ -29,6 +31,16 @@ struct CorrArgs {
   const double *dtp;
   double gamma;
   double *red;  // per-workgroup advective-bound minima (corr_partials entries), or nullptr
+};
+
+// Time-step decision of one instance from the partial minima of the
+// advective bound (k_corr / k_adv_bound)
+struct DtArgs {
+  const double *part;
+  int n;
+  double *dt, *time;
+  double dt_dif_x, dt_dif_y;
+  int implicitx, implicity;
 };
 
 // Thomas factorisation tables of the three CN-x operators for the three
@@ -46,16 +58,34 @@ struct LuFactorArgs {
 bool pred_impx_supported(int itot);
 int lu_table_entries(int itot);
 void launch_lu_factor(hipStream_t s, const LuFactorArgs &a);
-void launch_pred(hipStream_t s, const PredArgs &a, int implicitx, const double *lu, int ntm);
-// implicit-x predictor (sns_pred.hip); lu: the 3 field groups of one substage
-void launch_pred_rows(hipStream_t s, const PredArgs &a, const double *lu, int ntm);
-void launch_add_out(hipStream_t s, const Geo &g, int kind, const double *in, const double *q,
-                    double *out);
+// predictor of one substage; a.lu, a.ntm: the LU tables when implicitx
+void launch_pred(hipStream_t s, const PredArgs &a, int implicitx);
+// implicit-x predictor (sns_pred.hip)
+void launch_pred_rows(hipStream_t s, const PredArgs &a);
+void launch_pred_rows_batch(hipStream_t s, const PredArgs &shape, const PredArgs *dev, int nz);
+// y solve of one field (kind) after the predictor: transpose the x-aligned
+// increments, periodic Crank-Nicolson solve per column, transpose back,
+// out = in + increment with boundary values
+void launch_ysolve(hipStream_t s, int kind, const YArgs &a);
 int corr_partials(const Geo &g);
 void launch_corr(hipStream_t s, const CorrArgs &a);
-void launch_dt(hipStream_t s, const double *part, int n, double *dt, double *time,
-               double dt_dif_x, double dt_dif_y, int implicitx, int implicity);
+// the advective bound of ux, uy over the workgroups of k_corr into a.red
+// (the same partial minima the last correction of a step leaves)
+void launch_adv_bound(hipStream_t s, const CorrArgs &a);
+void launch_dt(hipStream_t s, const DtArgs &a);
 void launch_check_div(hipStream_t s, int n, const double *a, const double *b, const double *r,
                       double *q_cr, double *q_ieee);
+
+// The same kernels for nz instances of one shape (itot, jtot, treatment,
+// temperature equation): `shape` gives the launch configuration, dev the
+// per-instance arguments (a device array of nz blocks, instance = blockIdx.z).
+void launch_lu_factor_batch(hipStream_t s, const LuFactorArgs &shape, const LuFactorArgs *dev,
+                            int nz);
+void launch_pred_batch(hipStream_t s, const PredArgs &shape, int implicitx, const PredArgs *dev,
+                       int nz);
+void launch_ysolve_batch(hipStream_t s, int kind, const YArgs &shape, const YArgs *dev, int nz);
+void launch_corr_batch(hipStream_t s, const CorrArgs &shape, const CorrArgs *dev, int nz);
+void launch_adv_bound_batch(hipStream_t s, const CorrArgs &shape, const CorrArgs *dev, int nz);
+void launch_dt_batch(hipStream_t s, const DtArgs *dev, int nz);
 
 }  // namespace sns

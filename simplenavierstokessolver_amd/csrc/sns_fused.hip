@@ -87,7 +87,9 @@ __device__ __forceinline__ void emit_field(const PredArgs &a, const PredRows &R,
 }
 
 // explicit diffusion in x: point-wise, the three fields per thread
-__global__ void __launch_bounds__(256) k_pred_exp(PredArgs a) {
+template <class S>
+__global__ void __launch_bounds__(256) k_pred_exp(S src) {
+  const PredArgs a = src.get();
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = xcd_remap(blockIdx.y, gridDim.y) + 1;
   if (i > a.g.itot) return;
@@ -98,39 +100,20 @@ __global__ void __launch_bounds__(256) k_pred_exp(PredArgs a) {
   emit_field(a, R, 2, i, j, pred_field(a, R, 2, i, dt));
 }
 
-void launch_pred(hipStream_t s, const PredArgs &a, int implicitx, const double *lu, int ntm) {
-  if (implicitx) {
-    launch_pred_rows(s, a, lu, ntm);
-    return;
-  }
-  hipLaunchKernelGGL(k_pred_exp, dim3((a.g.itot + 255) / 256, a.g.jtot), dim3(256), 0, s, a);
+template <class S>
+static void pred_exp_launch(hipStream_t s, const Geo &g, S src, int nz) {
+  hipLaunchKernelGGL(k_pred_exp<S>, dim3((g.itot + 255) / 256, g.jtot, nz), dim3(256), 0, s, src);
 }
 
-// ----------------------------------------------------- y-solve completion
-
-// out = in + q (x-aligned increment after the y solve), with boundary values
-template <int KIND>
-__global__ void k_add_out(Geo g, const double *__restrict__ in, const double *__restrict__ q,
-                          double *__restrict__ out) {
-  const int n = KIND == KIND_UX ? g.itot - 1 : g.itot;
-  const int i0 = KIND == KIND_UX ? 2 : 1;
-  const int ii = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y + 1;
-  if (ii >= n) return;
-  const int i = ii + i0;
-  const double d = q[(size_t)(j - 1) * n + ii];
-  if (KIND == KIND_UX) put_ux(g, out, i, j, in[ux_idx(g, i, j)] + d);
-  else if (KIND == KIND_UY) put_c<0>(g, out, i, j, in[c_idx(g, i, j)] + d);
-  else put_c<2>(g, out, i, j, in[c_idx(g, i, j)] + d);
+void launch_pred(hipStream_t s, const PredArgs &a, int implicitx) {
+  if (implicitx) launch_pred_rows(s, a);
+  else pred_exp_launch(s, a.g, ArgOne<PredArgs>{a}, 1);
 }
 
-void launch_add_out(hipStream_t s, const Geo &g, int kind, const double *in, const double *q,
-                    double *out) {
-  const int n = kind == KIND_UX ? g.itot - 1 : g.itot;
-  const dim3 gr((n + 255) / 256, g.jtot);
-  if (kind == KIND_UX) hipLaunchKernelGGL(k_add_out<KIND_UX>, gr, dim3(256), 0, s, g, in, q, out);
-  else if (kind == KIND_UY) hipLaunchKernelGGL(k_add_out<KIND_UY>, gr, dim3(256), 0, s, g, in, q, out);
-  else hipLaunchKernelGGL(k_add_out<KIND_TEMP>, gr, dim3(256), 0, s, g, in, q, out);
+void launch_pred_batch(hipStream_t s, const PredArgs &shape, int implicitx, const PredArgs *dev,
+                       int nz) {
+  if (implicitx) launch_pred_rows_batch(s, shape, dev, nz);
+  else pred_exp_launch(s, shape.g, arg_batch<PredArgs>(dev), nz);
 }
 
 // ------------------------------------------------- correction + pressure
@@ -149,7 +132,9 @@ constexpr int kCorrThreads = 256;
 // src/param/decide_dt.c:25-45 over the corrected velocities: each workgroup
 // stores the minimum over its cells in red[workgroup] and k_dt reduces them
 // (min is exact, so the result does not depend on the order).
-__global__ void __launch_bounds__(kCorrThreads) k_corr(CorrArgs a) {
+template <class S>
+__global__ void __launch_bounds__(kCorrThreads) k_corr(S src) {
+  const CorrArgs a = src.get();
   const Geo &g = a.g;
   const Tables &tb = a.tb;
   const int itot = g.itot;
@@ -163,13 +148,13 @@ __global__ void __launch_bounds__(kCorrThreads) k_corr(CorrArgs a) {
   const double gdt = a.gamma * dt;
   const double fac = -gdt / g.two_Re;
   const size_t wc = (size_t)g.pc, wu = (size_t)g.pu;
-  const double dxc_i = ldr(tb.dxc, ti), rdxc_i = ldr(tb.rdxc, ti);  // DXC(i), 1./DXC(i)
+  const double dxc_i = ldr(tb.dxc(), ti), rdxc_i = ldr(tb.rdxc(), ti);  // DXC(i), 1./DXC(i)
   double dxc_p = 0., rdxc_p = 0., dxf_i = 0., rdxf_i = 0.;
   if (a.tr.implicitx) {
     dxc_p = from_right(dxc_i);  // DXC(i+1)
     rdxc_p = from_right(rdxc_i);
-    dxf_i = ldr(tb.dxf, ti);
-    rdxf_i = ldr(tb.rdxf, ti);
+    dxf_i = ldr(tb.dxf(), ti);
+    rdxf_i = ldr(tb.rdxf(), ti);
   }
   double m = 5.e-2;
   double psi_s = ldr(a.psi + (size_t)(j0 - 1) * wc, ci);  // psi(i, j-1)
@@ -236,7 +221,68 @@ int corr_partials(const Geo &g) {
 }
 
 void launch_corr(hipStream_t s, const CorrArgs &a) {
-  hipLaunchKernelGGL(k_corr, corr_grid(a.g), dim3(kCorrThreads), 0, s, a);
+  hipLaunchKernelGGL(k_corr<ArgOne<CorrArgs>>, corr_grid(a.g), dim3(kCorrThreads), 0, s,
+                     ArgOne<CorrArgs>{a});
+}
+
+static dim3 with_z(dim3 d, int nz) { return dim3(d.x, d.y, nz); }
+
+void launch_corr_batch(hipStream_t s, const CorrArgs &shape, const CorrArgs *dev, int nz) {
+  hipLaunchKernelGGL(k_corr<ArgBatch<CorrArgs>>, with_z(corr_grid(shape.g), nz),
+                     dim3(kCorrThreads), 0, s, arg_batch<CorrArgs>(dev));
+}
+
+// The advective bound of src/param/decide_dt.c:25-45 over ux, uy as they
+// are, with the tiling and the expressions of the reduction in k_corr: the
+// minimum over the cells of workgroup w goes to red[w], so a time-step
+// decision from these partials equals one from the partials the previous
+// step's last correction stored.
+template <class S>
+__global__ void __launch_bounds__(kCorrThreads) k_adv_bound(S src) {
+  const CorrArgs a = src.get();
+  const Geo &g = a.g;
+  const Tables &tb = a.tb;
+  const int itot = g.itot;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = (blockIdx.x * (kCorrThreads / 64) + wave) * kCorrCols + lane;
+  const bool out = lane >= 1 && lane <= kCorrCols && i <= itot;
+  const int ci = clampi(i, 0, itot + 1);
+  const int ti = clampi(i - 1, 0, itot + 1);
+  const int j0 = blockIdx.y * kCorrRows + 1;
+  const size_t wc = (size_t)g.pc, wu = (size_t)g.pu;
+  const double dxc_i = ldr(tb.dxc(), ti);
+  double m = 5.e-2;
+#pragma unroll
+  for (int r = 0; r < kCorrRows; r++) {
+    const int j = j0 + r;
+    if (j > g.jtot) break;
+    const double u = ldr(a.ux + (size_t)j * wu, clampi(i - 1, 0, itot));
+    const double v = ldr(a.uy + (size_t)j * wc, ci);
+    if (out) {
+      if (i >= 2) m = fmin(m, dxc_i / fmax(fabs(u), 1.e-8));
+      m = fmin(m, tb.dy / fmax(fabs(v), 1.e-8));
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off));
+  __shared__ double wm[kCorrThreads / 64];
+  if (lane == 0) wm[wave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = wm[0];
+    for (int w = 1; w < kCorrThreads / 64; w++) r = fmin(r, wm[w]);
+    a.red[blockIdx.y * gridDim.x + blockIdx.x] = r;
+  }
+}
+
+void launch_adv_bound(hipStream_t s, const CorrArgs &a) {
+  hipLaunchKernelGGL(k_adv_bound<ArgOne<CorrArgs>>, corr_grid(a.g), dim3(kCorrThreads), 0, s,
+                     ArgOne<CorrArgs>{a});
+}
+
+void launch_adv_bound_batch(hipStream_t s, const CorrArgs &shape, const CorrArgs *dev, int nz) {
+  hipLaunchKernelGGL(k_adv_bound<ArgBatch<CorrArgs>>, with_z(corr_grid(shape.g), nz),
+                     dim3(kCorrThreads), 0, s, arg_batch<CorrArgs>(dev));
 }
 
 // ------------------------------------------------ division self-check
@@ -264,11 +310,12 @@ void launch_check_div(hipStream_t s, int n, const double *a, const double *b, co
 // kDtThreads threads.
 constexpr int kDtThreads = 1024;
 
-__global__ void __launch_bounds__(kDtThreads) k_dt(const double *part, int n, double *dt,
-                                                   double *time, double dt_dif_x, double dt_dif_y,
-                                                   int implicitx, int implicity) {
+template <class S>
+__global__ void __launch_bounds__(kDtThreads) k_dt(S src) {
+  const DtArgs a = src.get();
+  const double *part = a.part;
   double m = part[0];
-  for (int k = threadIdx.x; k < n; k += kDtThreads) m = fmin(m, part[k]);
+  for (int k = threadIdx.x; k < a.n; k += kDtThreads) m = fmin(m, part[k]);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off));
   __shared__ double wm[kDtThreads / 64];
@@ -278,17 +325,20 @@ __global__ void __launch_bounds__(kDtThreads) k_dt(const double *part, int n, do
   for (int w = 1; w < kDtThreads / 64; w++) m = fmin(m, wm[w]);
   m = fmin(m, wm[0]);
   const double dt_adv = m * 0.7;
-  const double d = implicitx ? implicity ? dt_adv : fmin(dt_adv, dt_dif_y)
-                             : implicity ? fmin(dt_adv, dt_dif_x)
-                                         : fmin(dt_adv, fmin(dt_dif_x, dt_dif_y));
-  *dt = d;
-  *time += d;
+  const double dx = a.dt_dif_x, dy = a.dt_dif_y;
+  const double d = a.implicitx ? a.implicity ? dt_adv : fmin(dt_adv, dy)
+                               : a.implicity ? fmin(dt_adv, dx) : fmin(dt_adv, fmin(dx, dy));
+  *a.dt = d;
+  *a.time += d;
 }
 
-void launch_dt(hipStream_t s, const double *part, int n, double *dt, double *time,
-               double dt_dif_x, double dt_dif_y, int implicitx, int implicity) {
-  hipLaunchKernelGGL(k_dt, dim3(1), dim3(kDtThreads), 0, s, part, n, dt, time, dt_dif_x, dt_dif_y,
-                     implicitx, implicity);
+void launch_dt(hipStream_t s, const DtArgs &a) {
+  hipLaunchKernelGGL(k_dt<ArgOne<DtArgs>>, dim3(1), dim3(kDtThreads), 0, s, ArgOne<DtArgs>{a});
+}
+
+void launch_dt_batch(hipStream_t s, const DtArgs *dev, int nz) {
+  hipLaunchKernelGGL(k_dt<ArgBatch<DtArgs>>, dim3(1, 1, nz), dim3(kDtThreads), 0, s,
+                     arg_batch<DtArgs>(dev));
 }
 
 }  // namespace sns

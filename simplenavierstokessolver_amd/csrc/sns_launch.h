@@ -28,6 +28,49 @@ struct RK {
   double alpha, beta, gamma;
 };
 
+// Where a kernel finds its argument block.  ArgOne: passed by value, one
+// instance.  ArgBatch: a device array of blocks, one per z slice of the grid,
+// for a batch of instances of the same shape (itot, jtot, treatment) that one
+// launch advances together.  The array is read through the constant address
+// space, as kernel arguments are: the loads are scalar and invariant, and the
+// pointers in the blocks are known to address global memory, so the batched
+// kernels compile to the same code and register use as the single ones.
+template <class A>
+struct ArgOne {
+  A a;
+  __device__ __forceinline__ A get() const { return a; }
+};
+template <class A>
+struct ArgBatch {
+  const __attribute__((address_space(4))) A *p;
+  __device__ __forceinline__ A get() const {
+    A r;
+    __builtin_memcpy(&r, (const A *)(p + blockIdx.z), sizeof(A));
+    return r;
+  }
+};
+template <class A>
+inline ArgBatch<A> arg_batch(const A *dev) {
+  ArgBatch<A> b;
+  b.p = (const __attribute__((address_space(4))) A *)(uintptr_t)dev;
+  return b;
+}
+
+// Crank-Nicolson solve in y of one field of one instance (after the
+// predictor): the x-aligned increments qf (jtot rows of n) are transposed
+// into qy (n rows of jtot), solved per row, transposed back, and
+// out = in + increment with boundary values.  The per-function path uses
+// the solve on qy alone (in, qf, out unused).
+struct YArgs {
+  Geo g;
+  RK rk;
+  const double *dtp;
+  int n;             // unknowns per x row: itot-1 (ux) or itot
+  const double *in;  // substage-start field
+  double *qf, *qy;
+  double *out;
+};
+
 // ---- boundary values (src/fluid/boundary_conditions.c, src/temperature/boundary_conditions.c)
 void launch_bc_ux(hipStream_t s, const Geo &g, double *ux);
 void launch_bc_uy(hipStream_t s, const Geo &g, double *uy);

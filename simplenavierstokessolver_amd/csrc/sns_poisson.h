@@ -23,13 +23,30 @@ int poisson_factor_entries(int itot);
 void launch_poisson_factor(hipStream_t s, const Geo &g, const Tables &tb, const double *eigen,
                            double *fac);
 
+// Arguments of the Poisson kernels for one instance: ux*, uy* in the layout
+// of g, the spectrum psihat (x-aligned, jtot/2+1 rows of itot), the four-step
+// intermediate Y, the Thomas factors fac and psi with boundary values.
+struct PoisArgs {
+  Geo g;
+  Tables tb;
+  Fft2Plan pl;
+  double gamma;
+  const double *dtp;
+  const double *ux, *uy;
+  const double2 *tw;
+  const double *fac;
+  double2 *Y, *psihat;
+  double *psi;
+};
+
 // stage 0: RHS + forward transform -> psihat (x-aligned spectrum)
 // stage 1: tri-diagonal solves along x, one per wavenumber (factors in fac)
 // stage 2: inverse transform -> psi with boundary values
 // stage < 0: all three
-void launch_poisson2(hipStream_t s, const Geo &g, const Tables &tb, const Fft2Plan &pl,
-                     double gamma, const double *dtp, const double *ux, const double *uy,
-                     const double2 *tw, const double *fac, double2 *Y, double2 *psihat,
-                     double *psi, int stage);
+void launch_poisson(hipStream_t s, const PoisArgs &a, int stage);
+// the same for nz instances of the shape of `shape` (same itot, jtot), their
+// arguments in the device array dev
+void launch_poisson_batch(hipStream_t s, const PoisArgs &shape, const PoisArgs *dev, int nz,
+                          int stage);
 
 }  // namespace sns

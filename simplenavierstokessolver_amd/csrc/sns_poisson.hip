@@ -31,8 +31,8 @@ static constexpr int EMAXF = 8;  // complex values per thread and pass
 __device__ __forceinline__ double poisson_rhs2(const Geo &g, const Tables &tb, const double *ux,
                                                const double *uy, double inv_gdt, int i, int j) {
   // 1./(gamma*dt)*((ux_xp-ux_xm)/DXF(i) + (uy_yp-uy_ym)/dy)
-  const double ax = div_cr(ux[ux_idx(g, i + 1, j)] - ux[ux_idx(g, i, j)], tb.dxf[i - 1],
-                           tb.rdxf[i - 1]);
+  const double ax = div_cr(ux[ux_idx(g, i + 1, j)] - ux[ux_idx(g, i, j)], tb.dxf()[i - 1],
+                           tb.rdxf()[i - 1]);
   const double ay = div_cr(uy[c_idx(g, i, j + 1)] - uy[c_idx(g, i, j)], tb.dy, tb.rdy);
   return inv_gdt * (+ax + ay);
 }
@@ -52,10 +52,11 @@ __device__ __forceinline__ void put_psi(const Geo &g, double *f, int i, int j, d
 
 // ------------------------------------------------------------ forward A
 
-__global__ void __launch_bounds__(NTF)
-    k_pfwd_a(Geo g, Tables tb, Fft2Plan pl, double gamma, const double *__restrict__ dtp,
-             const double *__restrict__ ux, const double *__restrict__ uy,
-             const double2 *__restrict__ tw, double2 *__restrict__ Y) {
+__device__ __forceinline__ void pfwd_a(const Geo &g, const Tables &tb, const Fft2Plan &pl,
+                                       double gamma, const double *__restrict__ dtp,
+                                       const double *__restrict__ ux,
+                                       const double *__restrict__ uy,
+                                       const double2 *__restrict__ tw, double2 *__restrict__ Y) {
   __shared__ double2 buf[EMAXF * NTF];
   const int tile = blockIdx.x, j20 = blockIdx.y * pl.G;
   const int S = pl.G * TILE, n1 = pl.n1, n2 = pl.n2;
@@ -87,9 +88,10 @@ __global__ void __launch_bounds__(NTF)
 
 // ------------------------------------------------------------ forward B
 
-__global__ void __launch_bounds__(NTF)
-    k_pfwd_b(Geo g, Fft2Plan pl, const double2 *__restrict__ tw, const double2 *__restrict__ Y,
-             double2 *__restrict__ psihat) {
+__device__ __forceinline__ void pfwd_b(const Geo &g, const Fft2Plan &pl,
+                                       const double2 *__restrict__ tw,
+                                       const double2 *__restrict__ Y,
+                                       double2 *__restrict__ psihat) {
   __shared__ double2 buf[EMAXF * NTF];
   const int tile = blockIdx.x;
   const int n1 = pl.n1, n2 = pl.n2, n = pl.n;
@@ -138,9 +140,10 @@ __global__ void __launch_bounds__(NTF)
 
 // ------------------------------------------------------------ inverse B'
 
-__global__ void __launch_bounds__(NTF)
-    k_pinv_b(Geo g, Fft2Plan pl, const double2 *__restrict__ tw,
-             const double2 *__restrict__ psihat, double2 *__restrict__ Y) {
+__device__ __forceinline__ void pinv_b(const Geo &g, const Fft2Plan &pl,
+                                       const double2 *__restrict__ tw,
+                                       const double2 *__restrict__ psihat,
+                                       double2 *__restrict__ Y) {
   __shared__ double2 buf[EMAXF * NTF];
   const int tile = blockIdx.x;
   const int n1 = pl.n1, n2 = pl.n2, n = pl.n;
@@ -185,9 +188,9 @@ __global__ void __launch_bounds__(NTF)
 
 // ------------------------------------------------------------ inverse A'
 
-__global__ void __launch_bounds__(NTF)
-    k_pinv_a(Geo g, Fft2Plan pl, const double2 *__restrict__ tw, const double2 *__restrict__ Y,
-             double *__restrict__ psi) {
+__device__ __forceinline__ void pinv_a(const Geo &g, const Fft2Plan &pl,
+                                       const double2 *__restrict__ tw,
+                                       const double2 *__restrict__ Y, double *__restrict__ psi) {
   __shared__ double2 buf[EMAXF * NTF];
   const int tile = blockIdx.x, j20 = blockIdx.y * pl.G;
   const int S = pl.G * TILE, n1 = pl.n1, n2 = pl.n2;
@@ -237,8 +240,8 @@ __global__ void __launch_bounds__(64)
   const double lam = eigen[J];
   double *R = fac + (size_t)J * 3 * ntm, *AL = R + ntm, *BE = R + 2 * ntm;
   auto coef = [&](int i, double &l, double &c, double &u) {
-    l = tb.pl[i];
-    u = tb.pu[i];
+    l = tb.pl()[i];
+    u = tb.pu()[i];
     c = -l - u + lam;
     if (i == 0) c += l;
     if (i == n - 1) c += u;
@@ -275,8 +278,8 @@ struct PxLds {
 // One workgroup per wavenumber row; the row is staged through LDS (coalesced
 // global access, chunk order for the substitutions).
 template <int NT, int M>
-__global__ void __launch_bounds__(NT)
-    k_poisson_x3(Geo g, const double *__restrict__ fac, int ntm, double2 *__restrict__ psihat) {
+__device__ __forceinline__ void poisson_x3(const Geo &g, const double *__restrict__ fac, int ntm,
+                                           double2 *__restrict__ psihat) {
   extern __shared__ double lds_raw[];
   PxLds<NT, M> &L = *reinterpret_cast<PxLds<NT, M> *>(lds_raw);
   const int J = xcd_remap(blockIdx.x, gridDim.x);
@@ -304,6 +307,40 @@ __global__ void __launch_bounds__(NT)
   for (int r = threadIdx.x; r < n; r += NT) row[r] = L.row[r + r / M];
 }
 
+// ---------------------------------------------------------------- kernels
+// Argument source S: ArgOne<PoisArgs> (one instance) or ArgBatch<PoisArgs>
+// (instance blockIdx.z of a batch).
+
+template <class S>
+__global__ void __launch_bounds__(NTF) k_pfwd_a(S src) {
+  const PoisArgs a = src.get();
+  pfwd_a(a.g, a.tb, a.pl, a.gamma, a.dtp, a.ux, a.uy, a.tw, a.Y);
+}
+
+template <class S>
+__global__ void __launch_bounds__(NTF) k_pfwd_b(S src) {
+  const PoisArgs a = src.get();
+  pfwd_b(a.g, a.pl, a.tw, a.Y, a.psihat);
+}
+
+template <int NT, int M, class S>
+__global__ void __launch_bounds__(NT) k_poisson_x3(S src) {
+  const PoisArgs a = src.get();
+  poisson_x3<NT, M>(a.g, a.fac, NT * M, a.psihat);
+}
+
+template <class S>
+__global__ void __launch_bounds__(NTF) k_pinv_b(S src) {
+  const PoisArgs a = src.get();
+  pinv_b(a.g, a.pl, a.tw, a.psihat, a.Y);
+}
+
+template <class S>
+__global__ void __launch_bounds__(NTF) k_pinv_a(S src) {
+  const PoisArgs a = src.get();
+  pinv_a(a.g, a.pl, a.tw, a.Y, a.psi);
+}
+
 static constexpr int kPxM = 8;
 // threads of the x solve for n = itot unknowns (NT*kPxM >= n)
 static int px_threads(int n) {
@@ -324,17 +361,16 @@ void launch_poisson_factor(hipStream_t s, const Geo &g, const Tables &tb, const 
                      kPxM, ntm, fac);
 }
 
-template <int NT, int M>
-static void px_launch(hipStream_t s, const Geo &g, const double *fac, double2 *psihat) {
+template <int NT, int M, class S>
+static void px_launch(hipStream_t s, const Geo &g, S src, int nz) {
   const size_t lds = sizeof(PxLds<NT, M>);
   static bool attr = false;
   if (!attr) {
-    hipFuncSetAttribute((const void *)k_poisson_x3<NT, M>,
+    hipFuncSetAttribute((const void *)k_poisson_x3<NT, M, S>,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr = true;
   }
-  hipLaunchKernelGGL((k_poisson_x3<NT, M>), dim3(g.jtot / 2 + 1), dim3(NT), lds, s, g, fac,
-                     NT * M, psihat);
+  hipLaunchKernelGGL((k_poisson_x3<NT, M, S>), dim3(g.jtot / 2 + 1, 1, nz), dim3(NT), lds, s, src);
 }
 
 // ------------------------------------------------------------- plan, launch
@@ -387,27 +423,37 @@ bool fft2_plan(int n, int itot, Fft2Plan &pl) {
 static dim3 grid_a(const Fft2Plan &pl) { return dim3((pl.npair + TILE - 1) / TILE, pl.n2 / pl.G); }
 static dim3 grid_b(const Fft2Plan &pl) { return dim3((pl.npair + TILE - 1) / TILE, pl.n1 / 2 + 1); }
 
-void launch_poisson2(hipStream_t s, const Geo &g, const Tables &tb, const Fft2Plan &pl,
-                     double gamma, const double *dtp, const double *ux, const double *uy,
-                     const double2 *tw, const double *fac, double2 *Y, double2 *psihat,
-                     double *psi, int stage) {
+static dim3 with_z(dim3 d, int nz) { return dim3(d.x, d.y, nz); }
+
+template <class S>
+static void poisson_stage(hipStream_t s, const Geo &g, const Fft2Plan &pl, S src, int nz,
+                          int stage) {
   if (stage == 0 || stage < 0) {
-    hipLaunchKernelGGL(k_pfwd_a, grid_a(pl), dim3(NTF), 0, s, g, tb, pl, gamma, dtp, ux, uy, tw, Y);
-    hipLaunchKernelGGL(k_pfwd_b, grid_b(pl), dim3(NTF), 0, s, g, pl, tw, Y, psihat);
+    hipLaunchKernelGGL(k_pfwd_a<S>, with_z(grid_a(pl), nz), dim3(NTF), 0, s, src);
+    hipLaunchKernelGGL(k_pfwd_b<S>, with_z(grid_b(pl), nz), dim3(NTF), 0, s, src);
   }
   if (stage == 1 || stage < 0) {
     switch (px_threads(g.itot)) {
-      case 64: px_launch<64, kPxM>(s, g, fac, psihat); break;
-      case 128: px_launch<128, kPxM>(s, g, fac, psihat); break;
-      case 256: px_launch<256, kPxM>(s, g, fac, psihat); break;
-      case 512: px_launch<512, kPxM>(s, g, fac, psihat); break;
-      default: px_launch<1024, kPxM>(s, g, fac, psihat); break;
+      case 64: px_launch<64, kPxM>(s, g, src, nz); break;
+      case 128: px_launch<128, kPxM>(s, g, src, nz); break;
+      case 256: px_launch<256, kPxM>(s, g, src, nz); break;
+      case 512: px_launch<512, kPxM>(s, g, src, nz); break;
+      default: px_launch<1024, kPxM>(s, g, src, nz); break;
     }
   }
   if (stage == 2 || stage < 0) {
-    hipLaunchKernelGGL(k_pinv_b, grid_b(pl), dim3(NTF), 0, s, g, pl, tw, psihat, Y);
-    hipLaunchKernelGGL(k_pinv_a, grid_a(pl), dim3(NTF), 0, s, g, pl, tw, Y, psi);
+    hipLaunchKernelGGL(k_pinv_b<S>, with_z(grid_b(pl), nz), dim3(NTF), 0, s, src);
+    hipLaunchKernelGGL(k_pinv_a<S>, with_z(grid_a(pl), nz), dim3(NTF), 0, s, src);
   }
+}
+
+void launch_poisson(hipStream_t s, const PoisArgs &a, int stage) {
+  poisson_stage(s, a.g, a.pl, ArgOne<PoisArgs>{a}, 1, stage);
+}
+
+void launch_poisson_batch(hipStream_t s, const PoisArgs &shape, const PoisArgs *dev, int nz,
+                          int stage) {
+  poisson_stage(s, shape.g, shape.pl, arg_batch<PoisArgs>(dev), nz, stage);
 }
 
 }  // namespace sns

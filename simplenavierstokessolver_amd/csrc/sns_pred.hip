@@ -29,8 +29,8 @@ namespace sns {
 __device__ __forceinline__ void cnx_row(const Geo &g, const Tables &tb, int f, double pf, int r,
                                         double &l, double &c, double &u) {
   const int i = r + (f == 1 ? 2 : 1);
-  const double coef_xm = f == 1 ? tb.nU_xm[i - 1] : tb.nT_xm[i - 1];
-  const double coef_xp = f == 1 ? tb.nU_xp[i - 1] : tb.nT_xp[i - 1];
+  const double coef_xm = f == 1 ? tb.nU_xm()[i - 1] : tb.nT_xm()[i - 1];
+  const double coef_xp = f == 1 ? tb.nU_xp()[i - 1] : tb.nT_xp()[i - 1];
   const double coef_x0 = -coef_xm - coef_xp;
   l = -pf * coef_xm;
   u = -pf * coef_xp;
@@ -65,14 +65,17 @@ __device__ __forceinline__ M22 mshfl_up(const M22 &m, int off) {
 // vector v_i = [[0, u_i], [-l_i, c_i]] v_{i-1}, v_0 = (u_0, c_0), with
 // u'_i = p_i/q_i and 1/pivot_i = q_{i-1}/q_i; the products of these 2x2
 // matrices are scanned across the workgroup.
-template <int NT, int M>
-__global__ void __launch_bounds__(NT) k_lu_factor(LuFactorArgs a) {
+template <int NT, int M, class S>
+__global__ void __launch_bounds__(NT) k_lu_factor(S src) {
+  const LuFactorArgs a = src.get();
   constexpr int NW = NT / 64;
   __shared__ double wtot[NW][4];
   const Geo &g = a.g;
   const int f = blockIdx.x, s = blockIdx.y;
   const int n = f == 1 ? g.itot - 1 : g.itot;
-  const double pf = cn_pf(g, f, a.gamma[s] * (*a.dtp));
+  // substage gamma selected without indexing (keeps the argument block in registers)
+  const double gam = s == 0 ? a.gamma[0] : s == 1 ? a.gamma[1] : a.gamma[2];
+  const double pf = cn_pf(g, f, gam * (*a.dtp));
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   // chunk product of the row matrices (rows >= 1)
   M22 T = {1., 0., 0., 1.};
@@ -159,8 +162,11 @@ constexpr int kWaveCols = 62;  // output columns per wave (lanes 1..62)
 // Workgroup w: field w % nfld (+1 without the temperature equation) of rows
 // a = 2q+1 and b = a+1, q = w / nfld.  For odd jtot the last pair is shifted
 // down to (jtot-1, jtot) and writes row b only.  Consecutive w share an XCD.
-template <int NT, int M>
-__global__ void __launch_bounds__(NT) k_pred_rows(PredArgs a, const double *lu, int ntm, int nfld) {
+template <int NT, int M, class S>
+__global__ void __launch_bounds__(NT) k_pred_rows(S src, int nfld) {
+  const PredArgs a = src.get();
+  const double *lu = a.lu;
+  const int ntm = a.ntm;
   extern __shared__ double lds_raw[];
   PredRowsLds<NT, M> &L = *reinterpret_cast<PredRowsLds<NT, M> *>(lds_raw);
   constexpr int NW = NT / 64;
@@ -198,7 +204,7 @@ __global__ void __launch_bounds__(NT) k_pred_rows(PredArgs a, const double *lu, 
       const double v_a = ldr(V + ja * wc, ci), v_b = ldr(V + jb * wc, ci);
       const double v_bp = ldr(V + (jb + 1) * wc, ci);
       CellX x;
-      const double2 c2 = ldr2(tb.dxc2, ti), f2 = ldr2(tb.dxf2, ti), w2 = ldr2(tb.cT2, ti);
+      const double2 c2 = ldr2(tb.dxc2(), ti), f2 = ldr2(tb.dxf2(), ti), w2 = ldr2(tb.cT2(), ti);
       x.dxc = c2.x;
       x.rdxc = c2.y;
       x.dxf = f2.x;
@@ -228,7 +234,7 @@ __global__ void __launch_bounds__(NT) k_pred_rows(PredArgs a, const double *lu, 
       const double v_bp = ldr(V + (jb + 1) * wc, ci);
       const double p_a = ldr(P + ja * wc, ci), p_b = ldr(P + jb * wc, ci);
       FaceX x;
-      const double2 c2 = ldr2(tb.dxc2, ti), f2 = ldr2(tb.dxf2, ti), w2 = ldr2(tb.cU2, ti);
+      const double2 c2 = ldr2(tb.dxc2(), ti), f2 = ldr2(tb.dxf2(), ti), w2 = ldr2(tb.cU2(), ti);
       x.dxf = f2.x;
       x.rdxf = f2.y;
       x.c_xm = w2.x;
@@ -266,7 +272,7 @@ __global__ void __launch_bounds__(NT) k_pred_rows(PredArgs a, const double *lu, 
       const double p_am = ldr(P + (ja - 1) * wc, ci), p_a = ldr(P + ja * wc, ci);
       const double p_b = ldr(P + jb * wc, ci);
       CellX x;
-      const double2 c2 = ldr2(tb.dxc2, ti), f2 = ldr2(tb.dxf2, ti), w2 = ldr2(tb.cT2, ti);
+      const double2 c2 = ldr2(tb.dxc2(), ti), f2 = ldr2(tb.dxf2(), ti), w2 = ldr2(tb.cT2(), ti);
       x.dxc = c2.x;
       x.rdxc = c2.y;
       x.dxf = f2.x;
@@ -360,23 +366,24 @@ __global__ void __launch_bounds__(NT) k_pred_rows(PredArgs a, const double *lu, 
 
 // ---------------------------------------------------------------- launch
 
-template <int NT, int M>
-static void pred_rows_launch(hipStream_t s, const PredArgs &a, const double *lu, int ntm) {
+template <int NT, int M, class S>
+static void pred_rows_launch(hipStream_t s, const PredArgs &shape, S src, int nz) {
   const size_t lds = sizeof(PredRowsLds<NT, M>);
   static bool attr = false;
   if (!attr) {
-    hipFuncSetAttribute((const void *)k_pred_rows<NT, M>,
+    hipFuncSetAttribute((const void *)k_pred_rows<NT, M, S>,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr = true;
   }
-  const int nfld = a.with_temp ? 3 : 2;
-  const int npair = (a.g.jtot + 1) / 2;
-  hipLaunchKernelGGL((k_pred_rows<NT, M>), dim3(npair * nfld), dim3(NT), lds, s, a, lu, ntm, nfld);
+  const int nfld = shape.with_temp ? 3 : 2;
+  const int npair = (shape.g.jtot + 1) / 2;
+  hipLaunchKernelGGL((k_pred_rows<NT, M, S>), dim3(npair * nfld, 1, nz), dim3(NT), lds, s, src,
+                     nfld);
 }
 
-template <int NT, int M>
-static void lu_factor_launch(hipStream_t s, const LuFactorArgs &a) {
-  hipLaunchKernelGGL((k_lu_factor<NT, M>), dim3(3, 3), dim3(NT), 0, s, a);
+template <int NT, int M, class S>
+static void lu_factor_launch(hipStream_t s, S src, int nz) {
+  hipLaunchKernelGGL((k_lu_factor<NT, M, S>), dim3(3, 3, nz), dim3(NT), 0, s, src);
 }
 
 // (NT, M) of the row kernels for a given itot; ntm = NT*M >= itot
@@ -402,28 +409,47 @@ int lu_table_entries(int itot) {
   return row_config(itot, NT, M);
 }
 
-void launch_lu_factor(hipStream_t s, const LuFactorArgs &a) {
+template <class S>
+static void lu_factor_dispatch(hipStream_t s, int itot, S src, int nz) {
   int NT, M;
-  row_config(a.g.itot, NT, M);
+  row_config(itot, NT, M);
   switch (NT) {
-    case 64: lu_factor_launch<64, kRowM>(s, a); break;
-    case 128: lu_factor_launch<128, kRowM>(s, a); break;
-    case 256: lu_factor_launch<256, kRowM>(s, a); break;
-    case 512: lu_factor_launch<512, kRowM>(s, a); break;
-    default: lu_factor_launch<1024, kRowM>(s, a); break;
+    case 64: lu_factor_launch<64, kRowM>(s, src, nz); break;
+    case 128: lu_factor_launch<128, kRowM>(s, src, nz); break;
+    case 256: lu_factor_launch<256, kRowM>(s, src, nz); break;
+    case 512: lu_factor_launch<512, kRowM>(s, src, nz); break;
+    default: lu_factor_launch<1024, kRowM>(s, src, nz); break;
   }
 }
 
-void launch_pred_rows(hipStream_t s, const PredArgs &a, const double *lu, int ntm) {
+void launch_lu_factor(hipStream_t s, const LuFactorArgs &a) {
+  lu_factor_dispatch(s, a.g.itot, ArgOne<LuFactorArgs>{a}, 1);
+}
+
+void launch_lu_factor_batch(hipStream_t s, const LuFactorArgs &shape, const LuFactorArgs *dev,
+                            int nz) {
+  lu_factor_dispatch(s, shape.g.itot, arg_batch<LuFactorArgs>(dev), nz);
+}
+
+template <class S>
+static void pred_rows_dispatch(hipStream_t s, const PredArgs &shape, S src, int nz) {
   int NT, M;
-  row_config(a.g.itot, NT, M);
+  row_config(shape.g.itot, NT, M);
   switch (NT) {
-    case 64: pred_rows_launch<64, kRowM>(s, a, lu, ntm); break;
-    case 128: pred_rows_launch<128, kRowM>(s, a, lu, ntm); break;
-    case 256: pred_rows_launch<256, kRowM>(s, a, lu, ntm); break;
-    case 512: pred_rows_launch<512, kRowM>(s, a, lu, ntm); break;
-    default: pred_rows_launch<1024, kRowM>(s, a, lu, ntm); break;
+    case 64: pred_rows_launch<64, kRowM>(s, shape, src, nz); break;
+    case 128: pred_rows_launch<128, kRowM>(s, shape, src, nz); break;
+    case 256: pred_rows_launch<256, kRowM>(s, shape, src, nz); break;
+    case 512: pred_rows_launch<512, kRowM>(s, shape, src, nz); break;
+    default: pred_rows_launch<1024, kRowM>(s, shape, src, nz); break;
   }
+}
+
+void launch_pred_rows(hipStream_t s, const PredArgs &a) {
+  pred_rows_dispatch(s, a, ArgOne<PredArgs>{a}, 1);
+}
+
+void launch_pred_rows_batch(hipStream_t s, const PredArgs &shape, const PredArgs *dev, int nz) {
+  pred_rows_dispatch(s, shape, arg_batch<PredArgs>(dev), nz);
 }
 
 }  // namespace sns

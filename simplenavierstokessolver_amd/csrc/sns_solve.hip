@@ -1,6 +1,8 @@
 // sns_solve.hip -- line solvers, transposes and the Poisson chain (gfx950).
 #include "sns_launch.h"
 #include "sns_tridiag.h"
+#include "sns_fast.h"
+#include "sns_fused.h"
 
 namespace sns {
 
@@ -160,14 +162,16 @@ void launch_cnx(hipStream_t s, const Geo &g, int kind, RK rk, const double *dtp,
 // row's own right-hand side and the matrix-only right-hand side
 // [-l, 0, .., 0, -u] share one elimination (NR = 2).
 // Coefficients: src/temperature/update.c:188-197, src/fluid/update_velocity.c:302-310
-template <int KIND, int NT, int M>
-__global__ void __launch_bounds__(NT)
-    k_cny_rows(Geo g, RK rk, const double *__restrict__ dtp, double *__restrict__ qy) {
+template <int KIND, int NT, int M, class S>
+__global__ void __launch_bounds__(NT) k_cny_rows(S src) {
+  const YArgs ya = src.get();
+  const Geo &g = ya.g;
+  const RK rk = ya.rk;
   __shared__ TridiagLds<NT, 2> lds;
   __shared__ double ends[4];
   const int n = g.jtot;
-  double *q = qy + (size_t)blockIdx.x * n;
-  const double dt = *dtp;
+  double *q = ya.qy + (size_t)blockIdx.x * n;
+  const double dt = *ya.dtp;
   const double pf = cn_prefactor<KIND>(g, rk.gamma, dt);
   const double coef_ym = 1. / g.dy / g.dy;
   const double coef_yp = 1. / g.dy / g.dy;
@@ -209,29 +213,38 @@ __global__ void __launch_bounds__(NT)
   if (threadIdx.x == 0) q[n - 1] = xl;
 }
 
-template <int KIND, int NT, int M>
-static void cny_launch(hipStream_t s, const Geo &g, RK rk, const double *dtp, int nrows,
-                       double *qy) {
-  hipLaunchKernelGGL((k_cny_rows<KIND, NT, M>), dim3(nrows), dim3(NT), 0, s, g, rk, dtp, qy);
+template <int KIND, int NT, int M, class S>
+static void cny_launch(hipStream_t s, S src, int nrows, int nz) {
+  hipLaunchKernelGGL((k_cny_rows<KIND, NT, M, S>), dim3(nrows, 1, nz), dim3(NT), 0, s, src);
 }
 
-template <int KIND>
-static void cny_dispatch(hipStream_t s, const Geo &g, RK rk, const double *dtp, int nrows,
-                         double *qy) {
+template <int KIND, class S>
+static void cny_dispatch(hipStream_t s, const Geo &g, S src, int nrows, int nz) {
   const int m = g.jtot - 1;
-  if (m <= 64 * 4) cny_launch<KIND, 64, 4>(s, g, rk, dtp, nrows, qy);
-  else if (m <= 256 * 4) cny_launch<KIND, 256, 4>(s, g, rk, dtp, nrows, qy);
-  else if (m <= 256 * 8) cny_launch<KIND, 256, 8>(s, g, rk, dtp, nrows, qy);
-  else if (m <= 256 * 16) cny_launch<KIND, 256, 16>(s, g, rk, dtp, nrows, qy);
-  else if (m <= 256 * 32) cny_launch<KIND, 256, 32>(s, g, rk, dtp, nrows, qy);
-  else cny_launch<KIND, 512, 32>(s, g, rk, dtp, nrows, qy);  // jtot <= 16384 (snsg_create)
+  if (m <= 64 * 4) cny_launch<KIND, 64, 4>(s, src, nrows, nz);
+  else if (m <= 256 * 4) cny_launch<KIND, 256, 4>(s, src, nrows, nz);
+  else if (m <= 256 * 8) cny_launch<KIND, 256, 8>(s, src, nrows, nz);
+  else if (m <= 256 * 16) cny_launch<KIND, 256, 16>(s, src, nrows, nz);
+  else if (m <= 256 * 32) cny_launch<KIND, 256, 32>(s, src, nrows, nz);
+  else cny_launch<KIND, 512, 32>(s, src, nrows, nz);  // jtot <= 16384 (snsg_create)
+}
+
+template <class S>
+static void cny_kind(hipStream_t s, const Geo &g, int kind, S src, int nrows, int nz) {
+  if (kind == KIND_UX) cny_dispatch<KIND_UX>(s, g, src, nrows, nz);
+  else if (kind == KIND_UY) cny_dispatch<KIND_UY>(s, g, src, nrows, nz);
+  else cny_dispatch<KIND_TEMP>(s, g, src, nrows, nz);
 }
 
 void launch_cny_rows(hipStream_t s, const Geo &g, int kind, RK rk, const double *dtp, int nrows,
                      double *qy) {
-  if (kind == KIND_UX) cny_dispatch<KIND_UX>(s, g, rk, dtp, nrows, qy);
-  else if (kind == KIND_UY) cny_dispatch<KIND_UY>(s, g, rk, dtp, nrows, qy);
-  else cny_dispatch<KIND_TEMP>(s, g, rk, dtp, nrows, qy);
+  YArgs a{};
+  a.g = g;
+  a.rk = rk;
+  a.dtp = dtp;
+  a.n = nrows;
+  a.qy = qy;
+  cny_kind(s, g, kind, ArgOne<YArgs>{a}, nrows, 1);
 }
 
 // ------------------------------------------------------------ transpose
@@ -268,6 +281,69 @@ void launch_transpose(hipStream_t s, int g_isize, int g_jsize, size_t elemsize, 
   else
     hipLaunchKernelGGL(k_transpose<uint4>, gr, dim3(256), 0, s, g_isize, g_jsize,
                        (const uint4 *)send, (uint4 *)recv);
+}
+
+// ------------------------------------------- y solve after the predictor
+
+// DIR 0: qy[i*jtot + j] = qf[j*n + i]; DIR 1: qf[j*n + i] = qy[i*jtot + j]
+// (k_transpose's tiling, for instance blockIdx.z)
+template <int DIR, class S>
+__global__ void __launch_bounds__(256) k_ytrans(S src) {
+  const YArgs a = src.get();
+  __shared__ double tile[32][33];
+  const int isz = DIR == 0 ? a.n : a.g.jtot, jsz = DIR == 0 ? a.g.jtot : a.n;
+  const double *__restrict__ send = DIR == 0 ? a.qf : a.qy;
+  double *__restrict__ recv = DIR == 0 ? a.qy : a.qf;
+  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + tx, j = j0 + r;
+    if (i < isz && j < jsz) tile[r][tx] = send[(size_t)j * isz + i];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int j = j0 + tx, i = i0 + r;
+    if (i < isz && j < jsz) recv[(size_t)i * jsz + j] = tile[tx][r];
+  }
+}
+
+// out = in + q (x-aligned increment after the y solve), with boundary values
+template <int KIND, class S>
+__global__ void k_add_out(S src) {
+  const YArgs a = src.get();
+  const Geo &g = a.g;
+  const int n = a.n;
+  const int i0 = KIND == KIND_UX ? 2 : 1;
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y + 1;
+  if (ii >= n) return;
+  const int i = ii + i0;
+  const double d = a.qf[(size_t)(j - 1) * n + ii];
+  if (KIND == KIND_UX) put_ux(g, a.out, i, j, a.in[ux_idx(g, i, j)] + d);
+  else if (KIND == KIND_UY) put_c<0>(g, a.out, i, j, a.in[c_idx(g, i, j)] + d);
+  else put_c<2>(g, a.out, i, j, a.in[c_idx(g, i, j)] + d);
+}
+
+template <class S>
+static void ysolve(hipStream_t s, int kind, const YArgs &shape, S src, int nz) {
+  const Geo &g = shape.g;
+  const int n = shape.n;
+  const dim3 t0((n + 31) / 32, (g.jtot + 31) / 32, nz), t1((g.jtot + 31) / 32, (n + 31) / 32, nz);
+  hipLaunchKernelGGL((k_ytrans<0, S>), t0, dim3(256), 0, s, src);
+  cny_kind(s, g, kind, src, n, nz);
+  hipLaunchKernelGGL((k_ytrans<1, S>), t1, dim3(256), 0, s, src);
+  const dim3 gr((n + 255) / 256, g.jtot, nz);
+  if (kind == KIND_UX) hipLaunchKernelGGL((k_add_out<KIND_UX, S>), gr, dim3(256), 0, s, src);
+  else if (kind == KIND_UY) hipLaunchKernelGGL((k_add_out<KIND_UY, S>), gr, dim3(256), 0, s, src);
+  else hipLaunchKernelGGL((k_add_out<KIND_TEMP, S>), gr, dim3(256), 0, s, src);
+}
+
+void launch_ysolve(hipStream_t s, int kind, const YArgs &a) {
+  ysolve(s, kind, a, ArgOne<YArgs>{a}, 1);
+}
+
+void launch_ysolve_batch(hipStream_t s, int kind, const YArgs &shape, const YArgs *dev, int nz) {
+  ysolve(s, kind, shape, arg_batch<YArgs>(dev), nz);
 }
 
 }  // namespace sns
