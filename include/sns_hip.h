@@ -32,6 +32,7 @@
 #define SNS_HIP_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -40,6 +41,7 @@ extern "C" {
 #define SNSG_E_ARG      (-1)   /* invalid argument */
 #define SNSG_E_SIZE     (-2)   /* grid size not supported by the kernels */
 #define SNSG_E_NOMEM    (-3)   /* device allocation failed */
+#define SNSG_E_TIMEOUT  (-4)   /* a wait ended before its job */
 
 typedef struct snsg_ctx snsg_ctx;
 
@@ -167,6 +169,18 @@ int snsg_statistics_collect(snsg_ctx *ctx, const double *ux, const double *uy, c
                             double *temp2);
 int snsg_state_statistics_collect(snsg_ctx *ctx, const snsg_state *st, double *ux1, double *ux2,
                                   double *uy1, double *uy2, double *temp1, double *temp2);
+/* Seeded initial temperature of the state on the device: interior
+ * T = A*(2U-1) with U from the same counter hash as the host layer's initial
+ * condition (host/temperature.c: sns_initial_temperature, non-linear
+ * profile; it replaces the rand() field of src/temperature/init.c:39-47),
+ * then the boundary values of src/temperature/boundary_conditions.c. */
+int snsg_state_init_temperature(snsg_ctx *ctx, snsg_state *st, uint64_t seed, double amplitude);
+/* Device bytes that an instance of cfg's size and treatment holds with its
+ * state once it has stepped (snsg_create + snsg_state_alloc + the fused
+ * step's scratch), for admission against a memory budget. */
+int snsg_instance_bytes(const snsg_config *cfg, size_t *bytes);
+/* Device bytes an instance holds now (its state and scratch included) */
+int snsg_memory(const snsg_ctx *ctx, size_t *bytes);
 /* ---- batches (no counterpart in the reference, which runs one instance per
  * process).  Many instances advance together: each kernel of a step is
  * launched once per shape class (the members with the same itot, jtot,
@@ -189,6 +203,52 @@ int snsg_batch_advance(snsg_batch *b, int nsteps);
 int snsg_batch_synchronize(snsg_batch *b);
 int snsg_batch_classes(const snsg_batch *b, int *nclasses);
 int snsg_batch_destroy(snsg_batch *b);
+/* ---- serving engine (no counterpart in the reference).  Simulation jobs
+ * of any size arrive from any thread (snsg_server_submit); an engine thread
+ * admits them in arrival order while their device memory
+ * (snsg_instance_bytes) fits the budget, keeps every admitted job's fields
+ * resident and advances all admitted jobs continuously: the jobs of one shape
+ * class form a lane (as in a batch) that starts its next step as soon as the
+ * previous one is done, jobs joining and leaving it between steps.  Each
+ * job's fields, dt and time are bitwise those of snsg_advance(nsteps) on the
+ * job alone.  Instances of finished jobs are reused by later jobs of the same
+ * size and treatment, and freed when the budget needs their memory.  The
+ * per-step latency of a job-step is the duration of its lane's step on the
+ * device (HIP events). */
+typedef struct snsg_server snsg_server;
+typedef struct {
+  snsg_config cfg;           /* size, parameters, treatment */
+  const double *dxf, *dxc;   /* grid (as for snsg_create), copied at submission */
+  uint64_t seed;             /* initial temperature (snsg_state_init_temperature) */
+  double amplitude;
+  int nsteps;                /* time-loop iterations (as snsg_advance) */
+  int keep;                  /* keep the result for snsg_server_download until released */
+} snsg_job;
+typedef struct {
+  long submitted, admitted, completed, active, queued;
+  long steps;                /* job-steps completed since the last reset */
+  double cell_steps;         /* their itot*jtot summed */
+  double busy_s;             /* from the first admission (or the reset) to the last step */
+  double p50_ms, p99_ms, max_ms;  /* per-step latency over those job-steps */
+  size_t mem_used, mem_peak; /* device bytes of the instances held (active, kept, reusable) */
+  int lanes;
+} snsg_server_stats;
+#define SNSG_JOB_QUEUED 0
+#define SNSG_JOB_ACTIVE 1
+#define SNSG_JOB_DONE 2
+#define SNSG_JOB_RELEASED 3
+int snsg_server_create(int device, size_t mem_budget, snsg_server **out);
+int snsg_server_submit(snsg_server *s, const snsg_job *job, int *id);
+int snsg_server_wait(snsg_server *s, int id, double timeout_s);
+int snsg_server_drain(snsg_server *s, double timeout_s);
+int snsg_server_status(snsg_server *s, int id, int *state, int *steps_done);
+int snsg_server_download(snsg_server *s, int id, int field, double *host);
+int snsg_server_job_time(snsg_server *s, int id, double *time, double *dt);
+int snsg_server_release(snsg_server *s, int id);
+int snsg_server_stats_get(snsg_server *s, snsg_server_stats *out);
+int snsg_server_reset_stats(snsg_server *s);
+/* stops the engine (after the steps in flight) and frees every job */
+int snsg_server_destroy(snsg_server *s);
 /* advective time-step bound of the state's ux, uy (as snsg_param_decide_dt) */
 int snsg_state_decide_dt(snsg_ctx *ctx, const snsg_state *st, double *dt_adv);
 int snsg_set_time(snsg_ctx *ctx, double time);

@@ -41,6 +41,19 @@ class SnsgState(C.Structure):
         ("ld_ux", C.c_long), ("ld_c", C.c_long)]
 
 
+class SnsgJob(C.Structure):
+    _fields_ = [("cfg", SnsgConfig), ("dxf", _dp), ("dxc", _dp), ("seed", C.c_uint64),
+                ("amplitude", C.c_double), ("nsteps", C.c_int), ("keep", C.c_int)]
+
+
+class SnsgServerStats(C.Structure):
+    _fields_ = [(n, C.c_long) for n in ("submitted", "admitted", "completed", "active",
+                                        "queued", "steps")] + [
+        ("cell_steps", C.c_double), ("busy_s", C.c_double), ("p50_ms", C.c_double),
+        ("p99_ms", C.c_double), ("max_ms", C.c_double), ("mem_used", C.c_size_t),
+        ("mem_peak", C.c_size_t), ("lanes", C.c_int)]
+
+
 # field ids of snsg_state_upload / snsg_state_download (include/sns_hip.h)
 STATE_FIELDS = {"ux": 0, "uy": 1, "p": 2, "psi": 3, "temp": 4, "ux_star": 5, "uy_star": 6,
                 "srcuxa": 7, "srcuya": 8, "srctempa": 9}
@@ -91,6 +104,21 @@ def _load_hip():
         L.snsg_diagnostics.argtypes = [_vp, _vp, _vp, _vp, _dp]
         L.snsg_state_diagnostics.argtypes = [_vp, C.POINTER(SnsgState), _dp]
         L.snsg_statistics_collect.argtypes = [_vp] * 10
+        L.snsg_state_init_temperature.argtypes = [_vp, C.POINTER(SnsgState), C.c_uint64,
+                                                  C.c_double]
+        L.snsg_instance_bytes.argtypes = [C.POINTER(SnsgConfig), C.POINTER(C.c_size_t)]
+        L.snsg_memory.argtypes = [_vp, C.POINTER(C.c_size_t)]
+        L.snsg_server_create.argtypes = [C.c_int, C.c_size_t, C.POINTER(_vp)]
+        L.snsg_server_submit.argtypes = [_vp, C.POINTER(SnsgJob), C.POINTER(C.c_int)]
+        L.snsg_server_wait.argtypes = [_vp, C.c_int, C.c_double]
+        L.snsg_server_drain.argtypes = [_vp, C.c_double]
+        L.snsg_server_status.argtypes = [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.snsg_server_download.argtypes = [_vp, C.c_int, C.c_int, _vp]
+        L.snsg_server_job_time.argtypes = [_vp, C.c_int, _dp, _dp]
+        L.snsg_server_release.argtypes = [_vp, C.c_int]
+        L.snsg_server_stats_get.argtypes = [_vp, C.POINTER(SnsgServerStats)]
+        L.snsg_server_reset_stats.argtypes = [_vp]
+        L.snsg_server_destroy.argtypes = [_vp]
         L.snsg_batch_create.argtypes = [C.c_int, C.POINTER(_vp), C.POINTER(C.POINTER(SnsgState)),
                                         C.c_int, C.POINTER(_vp)]
         L.snsg_batch_advance.argtypes = [_vp, C.c_int]
@@ -193,6 +221,27 @@ FIELD_SHAPES = {
 }
 
 
+def make_config(itot, jtot, ly=2.0, Ra=1e8, Pr=1.0, implicitx=0, implicity=0,
+                with_temperature=True, with_thermal_forcing=True):
+    """snsg_config of an instance (RK coefficients of src/param/init.c:197-213)."""
+    alpha, beta, gamma = rk_coefs()
+    cfg = SnsgConfig()
+    cfg.itot, cfg.jtot, cfg.ly, cfg.Ra, cfg.Pr = itot, jtot, ly, Ra, Pr
+    cfg.with_temperature = int(with_temperature)
+    cfg.with_thermal_forcing = int(with_thermal_forcing)
+    cfg.implicitx, cfg.implicity = int(implicitx), int(implicity)
+    for k in range(3):
+        cfg.alpha[k], cfg.beta[k], cfg.gamma[k] = alpha[k], beta[k], gamma[k]
+    return cfg
+
+
+def instance_bytes(cfg):
+    """Device bytes of an instance of this configuration (snsg_instance_bytes)."""
+    n = C.c_size_t(0)
+    _check(_load_hip().snsg_instance_bytes(C.byref(cfg), C.byref(n)), "snsg_instance_bytes")
+    return n.value
+
+
 def _state_shape(name):
     """Reference-layout shape name of a state field."""
     return {"ux_star": "ux", "uy_star": "uy"}.get(name, name)
@@ -208,13 +257,8 @@ class Instance:
         self.itot, self.jtot = itot, jtot
         self.g = grid(itot, jtot, ly, stretch)
         self.alpha, self.beta, self.gamma = rk_coefs()
-        cfg = SnsgConfig()
-        cfg.itot, cfg.jtot, cfg.ly, cfg.Ra, cfg.Pr = itot, jtot, ly, Ra, Pr
-        cfg.with_temperature = int(with_temperature)
-        cfg.with_thermal_forcing = int(with_thermal_forcing)
-        cfg.implicitx, cfg.implicity = int(implicitx), int(implicity)
-        for k in range(3):
-            cfg.alpha[k], cfg.beta[k], cfg.gamma[k] = self.alpha[k], self.beta[k], self.gamma[k]
+        cfg = make_config(itot, jtot, ly, Ra, Pr, implicitx, implicity, with_temperature,
+                          with_thermal_forcing)
         self.cfg = cfg
         self.Ra, self.Pr, self.ly = Ra, Pr, ly
         self.ctx = _vp()
@@ -380,6 +424,17 @@ class Instance:
         _check(self.L.snsg_state_upload(self.ctx, C.byref(self.state), STATE_FIELDS[name],
                                         a.ctypes.data), "snsg_state_upload")
 
+    def state_init_temperature(self, seed, amplitude):
+        """Seeded initial temperature on the device (snsg_state_init_temperature)."""
+        _check(self.L.snsg_state_init_temperature(self.ctx, C.byref(self.state), int(seed),
+                                                  float(amplitude)),
+               "snsg_state_init_temperature")
+
+    def memory(self):
+        n = C.c_size_t(0)
+        _check(self.L.snsg_memory(self.ctx, C.byref(n)), "snsg_memory")
+        return n.value
+
     def state_download(self, name, shape_name=None):
         """A state field in the reference layout."""
         a = np.zeros(self.shape(shape_name or _state_shape(name)))
@@ -506,6 +561,79 @@ class Batch:
         if self.b:
             self.L.snsg_batch_destroy(self.b)
             self.b = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The serving engine (snsg_server_*): jobs submitted at any time are
+    admitted against a device-memory budget and advanced continuously, one
+    lane per shape class."""
+
+    def __init__(self, mem_budget, device=0):
+        self.L = _load_hip()
+        self.s = _vp()
+        _check(self.L.snsg_server_create(int(device), int(mem_budget), C.byref(self.s)),
+               "snsg_server_create")
+        self.sizes = {}
+
+    def submit(self, itot, jtot, nsteps, Ra=1e8, Pr=1.0, implicitx=0, implicity=0, seed=1,
+               amplitude=1e-3, ly=2.0, stretch=None, keep=False, with_temperature=True):
+        g = grid(itot, jtot, ly, float(itot) if stretch is None else stretch)
+        job = SnsgJob()
+        job.cfg = make_config(itot, jtot, ly, Ra, Pr, implicitx, implicity, with_temperature)
+        job.dxf, job.dxc = _p(g["dxf"]), _p(g["dxc"])
+        job.seed, job.amplitude = int(seed), float(amplitude)
+        job.nsteps, job.keep = int(nsteps), int(keep)
+        jid = C.c_int(0)
+        _check(self.L.snsg_server_submit(self.s, C.byref(job), C.byref(jid)), "snsg_server_submit")
+        self.sizes[jid.value] = (itot, jtot)
+        return jid.value
+
+    def wait(self, jid, timeout=600.0):
+        _check(self.L.snsg_server_wait(self.s, int(jid), float(timeout)), "snsg_server_wait")
+
+    def drain(self, timeout=600.0):
+        _check(self.L.snsg_server_drain(self.s, float(timeout)), "snsg_server_drain")
+
+    def status(self, jid):
+        st, n = C.c_int(0), C.c_int(0)
+        _check(self.L.snsg_server_status(self.s, int(jid), C.byref(st), C.byref(n)),
+               "snsg_server_status")
+        return st.value, n.value
+
+    def download(self, jid, name):
+        itot, jtot = self.sizes[jid]
+        a = np.zeros(FIELD_SHAPES[_state_shape(name)](itot, jtot))
+        _check(self.L.snsg_server_download(self.s, int(jid), STATE_FIELDS[name], a.ctypes.data),
+               "snsg_server_download")
+        return a
+
+    def job_time(self, jid):
+        t, dt = C.c_double(0.0), C.c_double(0.0)
+        _check(self.L.snsg_server_job_time(self.s, int(jid), C.byref(t), C.byref(dt)),
+               "snsg_server_job_time")
+        return t.value, dt.value
+
+    def release(self, jid):
+        _check(self.L.snsg_server_release(self.s, int(jid)), "snsg_server_release")
+
+    def stats(self):
+        st = SnsgServerStats()
+        _check(self.L.snsg_server_stats_get(self.s, C.byref(st)), "snsg_server_stats_get")
+        return {n: getattr(st, n) for n, _ in SnsgServerStats._fields_}
+
+    def reset_stats(self):
+        _check(self.L.snsg_server_reset_stats(self.s), "snsg_server_reset_stats")
+
+    def close(self):
+        if self.s:
+            self.L.snsg_server_destroy(self.s)
+            self.s = _vp()
 
     def __del__(self):
         try:

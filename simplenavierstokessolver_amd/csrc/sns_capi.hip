@@ -7,61 +7,20 @@
 #include <string>
 #include <vector>
 
-#include "sns_hip.h"
-#include "sns_launch.h"
-#include "sns_fused.h"
-#include "sns_poisson.h"
-#include "sns_diag.h"
+#include <map>
+#include <mutex>
 
-using namespace sns;
+#include "sns_ctx.h"
+#include "sns_diag.h"
 
 #ifndef M_PI
 #define M_PI 3.1415926535897932
 #endif
 
-struct snsg_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  snsg_config cfg{};
-  Geo geo{};     // reference layouts (per-function entry points)
-  Geo geo_st{};  // padded row pitches of the fused state (snsg_state)
-  Treat treat{};
-  double *d_dxf = nullptr, *d_dxc = nullptr;
-  double *d_dt = nullptr;
-  double *d_eigen = nullptr;
-  double *d_pfac = nullptr;             // Thomas factors of the Poisson x systems
-  double2 *d_tw = nullptr;
-  double2 *d_psihat = nullptr;
-  double *d_q = nullptr, *d_qy = nullptr;
-  unsigned long long *d_red = nullptr;
-  unsigned long long *h_red = nullptr;  // pinned
-  double *d_part = nullptr;             // per-workgroup minima of the fused correction
-  double *d_diag = nullptr;             // diagnostics: partial slots, then the results
-  double *h_diag = nullptr;             // pinned copy of the results
-  int nparts = 0;
-  Fft2Plan plan{};
-  double2 *d_Y = nullptr;  // four-step FFT intermediate, jtot rows of (itot+1)/2 pairs
-  // per-column tables of the fused kernels (one allocation)
-  double *d_tab = nullptr;
-  double2 *d_tab2 = nullptr;           // paired column tables
-  Tables tab{};
-  // fused-step scratch: x-aligned increments of T, ux, uy when a y solve follows
-  double *d_qf[3] = {nullptr, nullptr, nullptr};
-  // LU tables of the CN-x operators (3 substages x 3 fields x 5 tables)
-  double *d_lu = nullptr;
-  int lu_ntm = 0;
-  double *d_time = nullptr;
-  double dt_dif_x = 0., dt_dif_y = 0.;
-  // timing
-  bool timing = false;
-  std::string filter;
-  std::vector<hipEvent_t> ev_pool;
-  size_t ev_used = 0;
-};
 
 static thread_local std::string g_last_error;
 
-static int set_err(int code, const char *what) {
+int set_err(int code, const char *what) {
   char buf[512];
   snprintf(buf, sizeof buf, "%s (code %d%s%s)", what, code, code > 0 ? ": " : "",
            code > 0 ? hipGetErrorString((hipError_t)code) : "");
@@ -69,13 +28,8 @@ static int set_err(int code, const char *what) {
   return code;
 }
 
-#define HIPCHECK(expr)                                          \
-  do {                                                          \
-    hipError_t e_ = (expr);                                     \
-    if (e_ != hipSuccess) return set_err((int)e_, #expr);       \
-  } while (0)
 
-static int check_launch(const char *what) {
+int check_launch(const char *what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err((int)e, what);
   return 0;
@@ -112,16 +66,39 @@ static int run(snsg_ctx *ctx, const char *name, F fn) {
 static int state_pitch(int n) { return (n + 15) / 16 * 16; }
 static constexpr size_t kStateCOff = 15;
 
-// Zero-filled device allocation.  The fill runs on the null stream, which
-// is not ordered with the instances' non-blocking streams, so it is waited
-// for before the buffer is handed out (a kernel on another stream must not
-// run before it, or the fill could land on the kernel's output).
-static int ctx_alloc(void **p, size_t bytes) {
+// A non-blocking stream per device for the library's own fills.  The
+// library never uses the null stream: its implicit synchronisation is not
+// ordered with non-blocking streams, and it is refused while another
+// thread records a graph.
+static hipError_t util_stream(hipStream_t *out) {
+  static std::mutex mu;
+  static std::map<int, hipStream_t> streams;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = streams.find(dev);
+  if (it == streams.end()) {
+    hipStream_t s = nullptr;
+    e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
+    it = streams.emplace(dev, s).first;
+  }
+  *out = it->second;
+  return hipSuccess;
+}
+
+// Zero-filled device allocation: the fill is complete on return, so the
+// buffer is ready for any stream.
+int ctx_alloc(void **p, size_t bytes, size_t *count) {
   hipError_t e = hipMalloc(p, bytes ? bytes : 8);
   if (e != hipSuccess) return set_err((int)e, "hipMalloc");
-  e = hipMemsetAsync(*p, 0, bytes ? bytes : 8, nullptr);
-  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  hipStream_t u = nullptr;
+  e = util_stream(&u);
+  if (e == hipSuccess) e = hipMemsetAsync(*p, 0, bytes ? bytes : 8, u);
+  if (e == hipSuccess) e = hipStreamSynchronize(u);
   if (e != hipSuccess) return set_err((int)e, "hipMemset");
+  if (count) *count += bytes ? bytes : 8;
   return 0;
 }
 
@@ -132,46 +109,31 @@ extern "C" int snsg_device_count(int *count) {
 
 extern "C" const char *snsg_last_error(void) { return g_last_error.c_str(); }
 
-extern "C" int snsg_create(int device, const snsg_config *cfg, const double *dxf,
-                           const double *dxc, snsg_ctx **out) {
-  if (!cfg || !dxf || !dxc || !out) return set_err(SNSG_E_ARG, "snsg_create: null argument");
+__global__ void k_set_f64(double *p, double v) { *p = v; }
+
+static int check_sizes(const snsg_config *cfg, Fft2Plan &pl) {
   const int itot = cfg->itot, jtot = cfg->jtot;
   if (itot < 4 || jtot < 4) return set_err(SNSG_E_SIZE, "snsg_create: itot and jtot must be >= 4");
   if (itot > 8192) return set_err(SNSG_E_SIZE, "snsg_create: itot > 8192 is not supported");
-  Fft2Plan pl;
   if (jtot > 16384 || !fft2_plan(jtot, itot, pl))
     return set_err(SNSG_E_SIZE,
                    "snsg_create: jtot must be <= 16384, not prime, with prime factors 2, 3 "
                    "and 5 only");
-  HIPCHECK(hipSetDevice(device));
-  snsg_ctx *c = new snsg_ctx();
-  c->device = device;
+  return 0;
+}
+
+// Everything of an instance that depends on its parameters and grid (the
+// buffers exist): the grid and the column tables, the constants of Geo, the
+// Poisson eigenvalues, twiddles and Thomas factors, the diffusive time-step
+// bounds; time and dt restart from 0.  Used by snsg_create and to reuse the
+// buffers of a finished instance of the same size for a new one.
+int ctx_configure(snsg_ctx *c, const snsg_config *cfg, const double *dxf, const double *dxc) {
+  const int itot = cfg->itot, jtot = cfg->jtot;
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipStreamSynchronize(c->stream));
   c->cfg = *cfg;
-  c->plan = pl;
-  int rc = 0;
-  auto fail = [&](int code) {
-    snsg_destroy(c);
-    return code;
-  };
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
-    return fail(set_err(SNSG_E_ARG, "hipStreamCreate"));
-  if ((rc = ctx_alloc((void **)&c->d_dxf, sizeof(double) * itot))) return fail(rc);
-  if ((rc = ctx_alloc((void **)&c->d_dxc, sizeof(double) * (itot + 1)))) return fail(rc);
-  if ((rc = ctx_alloc((void **)&c->d_dt, sizeof(double)))) return fail(rc);
-  if ((rc = ctx_alloc((void **)&c->d_eigen, sizeof(double) * (jtot / 2 + 1)))) return fail(rc);
-  if ((rc = ctx_alloc((void **)&c->d_tw, sizeof(double2) * jtot))) return fail(rc);
-  if ((rc = ctx_alloc((void **)&c->d_psihat, sizeof(double2) * (size_t)itot * (jtot / 2 + 1))))
-    return fail(rc);
-  if ((rc = ctx_alloc((void **)&c->d_Y, sizeof(double2) * (size_t)pl.npair * jtot))) return fail(rc);
-  if ((rc = ctx_alloc((void **)&c->d_q, sizeof(double) * (size_t)(itot + 1) * jtot))) return fail(rc);
-  if ((rc = ctx_alloc((void **)&c->d_qy, sizeof(double) * (size_t)(itot + 1) * jtot))) return fail(rc);
-  if ((rc = ctx_alloc((void **)&c->d_red, sizeof(unsigned long long)))) return fail(rc);
-  if (hipHostMalloc((void **)&c->h_red, sizeof(unsigned long long), hipHostMallocDefault) !=
-      hipSuccess)
-    return fail(set_err(SNSG_E_NOMEM, "hipHostMalloc"));
-  if (hipMemcpy(c->d_dxf, dxf, sizeof(double) * itot, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->d_dxc, dxc, sizeof(double) * (itot + 1), hipMemcpyHostToDevice) != hipSuccess)
-    return fail(set_err(SNSG_E_ARG, "upload grid"));
+  HIPCHECK(hipMemcpyAsync(c->d_dxf, dxf, sizeof(double) * itot, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->d_dxc, dxc, sizeof(double) * (itot + 1), hipMemcpyHostToDevice, c->stream));
   // host-evaluated constants, with the reference's operations
   const double sqrtPr = sqrt(cfg->Pr), sqrtRa = sqrt(cfg->Ra);
   const double dy = cfg->ly / jtot;  // src/param/init.c:187
@@ -203,78 +165,110 @@ extern "C" int snsg_create(int device, const snsg_config *cfg, const double *dxf
     const double a = 2. * M_PI * (double)m / (double)jtot;
     tw[m] = make_double2(cos(a), -sin(a));
   }
+  HIPCHECK(hipMemcpyAsync(c->d_eigen, eig.data(), sizeof(double) * eig.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->d_tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice, c->stream));
   // per-column tables (see sns_fast.h), each evaluated as in the reference
-  {
-    const int nt = 14;
-    const size_t stride = (size_t)itot + 2;
-    if ((rc = ctx_alloc((void **)&c->d_tab, sizeof(double) * nt * stride))) return fail(rc);
-    std::vector<double> h(nt * stride, 0.);
-    auto DXF = [&](int i) { return dxf[i - 1]; };
-    auto DXC = [&](int i) { return dxc[i - 1]; };
-    double *t[nt];
-    for (int k = 0; k < nt; k++) t[k] = h.data() + k * stride;
-    for (int i = 1; i <= itot; i++) t[0][i - 1] = DXF(i);
-    for (int i = 1; i <= itot + 1; i++) t[1][i - 1] = DXC(i);
-    for (int i = 1; i <= itot; i++) t[2][i - 1] = 1. / DXF(i);
-    for (int i = 1; i <= itot + 1; i++) t[3][i - 1] = 1. / DXC(i);
-    for (int i = 1; i <= itot; i++) {
-      t[4][i - 1] = DXC(i) / (2. * DXF(i));
-      t[5][i - 1] = DXC(i + 1) / (2. * DXF(i));
-      t[8][i - 1] = 1. / DXF(i) / DXC(i);
-      t[9][i - 1] = 1. / DXF(i) / DXC(i + 1);
-      t[12][i - 1] = 1. / DXC(i) / DXF(i);
-      t[13][i - 1] = 1. / DXC(i + 1) / DXF(i);
-    }
-    for (int i = 2; i <= itot; i++) {
-      t[6][i - 1] = DXF(i - 1) / (2. * DXC(i));
-      t[7][i - 1] = DXF(i) / (2. * DXC(i));
-      t[10][i - 1] = 1. / DXC(i) / DXF(i - 1);
-      t[11][i - 1] = 1. / DXC(i) / DXF(i);
-    }
-    if (hipMemcpy(c->d_tab, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice) !=
-        hipSuccess)
-      return fail(set_err(SNSG_E_ARG, "upload column tables"));
-    const double *d = c->d_tab;
-    Tables &tb = c->tab;
-    tb.t = d;
-    tb.stride = (long)stride;
-    // paired copies: (DXC, 1/DXC), (DXF, 1/DXF), (cT_xm, cT_xp), (cU_xm, cU_xp)
-    {
-      std::vector<double2> h2(4 * stride);
-      for (size_t k = 0; k < stride; k++) {
-        h2[0 * stride + k] = make_double2(t[1][k], t[3][k]);
-        h2[1 * stride + k] = make_double2(t[0][k], t[2][k]);
-        h2[2 * stride + k] = make_double2(t[4][k], t[5][k]);
-        h2[3 * stride + k] = make_double2(t[6][k], t[7][k]);
-      }
-      if ((rc = ctx_alloc((void **)&c->d_tab2, sizeof(double2) * h2.size()))) return fail(rc);
-      if (hipMemcpy(c->d_tab2, h2.data(), sizeof(double2) * h2.size(), hipMemcpyHostToDevice) !=
-          hipSuccess)
-        return fail(set_err(SNSG_E_ARG, "upload paired column tables"));
-      tb.t2 = c->d_tab2;
-    }
-    tb.dy = dy;
-    tb.rdy = 1. / dy;
-    // fixed diffusive time-step bounds (src/param/decide_dt.c:48-66)
-    const double diffusivity = fmax(sqrtPr / sqrtRa, 1. / sqrtPr / sqrtRa);
-    double dx = 1.;
-    for (int i = 2; i <= itot; i++) dx = fmin(dx, DXC(i));
-    c->dt_dif_x = 0.25 / diffusivity * pow(dx, 2.) * 0.9;
-    c->dt_dif_y = 0.25 / diffusivity * pow(dy, 2.) * 0.9;
+  const int nt = kTables;
+  const size_t stride = (size_t)itot + 2;
+  std::vector<double> h(nt * stride, 0.);
+  auto DXF = [&](int i) { return dxf[i - 1]; };
+  auto DXC = [&](int i) { return dxc[i - 1]; };
+  double *t[nt];
+  for (int k = 0; k < nt; k++) t[k] = h.data() + k * stride;
+  for (int i = 1; i <= itot; i++) t[0][i - 1] = DXF(i);
+  for (int i = 1; i <= itot + 1; i++) t[1][i - 1] = DXC(i);
+  for (int i = 1; i <= itot; i++) t[2][i - 1] = 1. / DXF(i);
+  for (int i = 1; i <= itot + 1; i++) t[3][i - 1] = 1. / DXC(i);
+  for (int i = 1; i <= itot; i++) {
+    t[4][i - 1] = DXC(i) / (2. * DXF(i));
+    t[5][i - 1] = DXC(i + 1) / (2. * DXF(i));
+    t[8][i - 1] = 1. / DXF(i) / DXC(i);
+    t[9][i - 1] = 1. / DXF(i) / DXC(i + 1);
+    t[12][i - 1] = 1. / DXC(i) / DXF(i);
+    t[13][i - 1] = 1. / DXC(i + 1) / DXF(i);
   }
-  if ((rc = ctx_alloc((void **)&c->d_time, sizeof(double)))) return fail(rc);
-  if (hipMemcpy(c->d_eigen, eig.data(), sizeof(double) * eig.size(), hipMemcpyHostToDevice) !=
-          hipSuccess ||
-      hipMemcpy(c->d_tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice) !=
-          hipSuccess)
-    return fail(set_err(SNSG_E_ARG, "upload tables"));
-  {
-    const size_t nfac = (size_t)(jtot / 2 + 1) * 3 * poisson_factor_entries(itot);
-    if ((rc = ctx_alloc((void **)&c->d_pfac, sizeof(double) * nfac))) return fail(rc);
-    launch_poisson_factor(c->stream, c->geo, c->tab, c->d_eigen, c->d_pfac);
-    if (hipStreamSynchronize(c->stream) != hipSuccess)
-      return fail(set_err(SNSG_E_ARG, "Poisson factorisation"));
+  for (int i = 2; i <= itot; i++) {
+    t[6][i - 1] = DXF(i - 1) / (2. * DXC(i));
+    t[7][i - 1] = DXF(i) / (2. * DXC(i));
+    t[10][i - 1] = 1. / DXC(i) / DXF(i - 1);
+    t[11][i - 1] = 1. / DXC(i) / DXF(i);
   }
+  HIPCHECK(hipMemcpyAsync(c->d_tab, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+  // paired copies: (DXC, 1/DXC), (DXF, 1/DXF), (cT_xm, cT_xp), (cU_xm, cU_xp)
+  std::vector<double2> h2(kTables2 * stride);
+  for (size_t k = 0; k < stride; k++) {
+    h2[0 * stride + k] = make_double2(t[1][k], t[3][k]);
+    h2[1 * stride + k] = make_double2(t[0][k], t[2][k]);
+    h2[2 * stride + k] = make_double2(t[4][k], t[5][k]);
+    h2[3 * stride + k] = make_double2(t[6][k], t[7][k]);
+  }
+  HIPCHECK(hipMemcpyAsync(c->d_tab2, h2.data(), sizeof(double2) * h2.size(), hipMemcpyHostToDevice, c->stream));
+  Tables &tb = c->tab;
+  tb.t = c->d_tab;
+  tb.t2 = c->d_tab2;
+  tb.stride = (long)stride;
+  tb.dy = dy;
+  tb.rdy = 1. / dy;
+  // fixed diffusive time-step bounds (src/param/decide_dt.c:48-66)
+  const double diffusivity = fmax(sqrtPr / sqrtRa, 1. / sqrtPr / sqrtRa);
+  double dx = 1.;
+  for (int i = 2; i <= itot; i++) dx = fmin(dx, DXC(i));
+  c->dt_dif_x = 0.25 / diffusivity * pow(dx, 2.) * 0.9;
+  c->dt_dif_y = 0.25 / diffusivity * pow(dy, 2.) * 0.9;
+  launch_poisson_factor(c->stream, c->geo, c->tab, c->d_eigen, c->d_pfac);
+  hipLaunchKernelGGL(k_set_f64, dim3(1), dim3(1), 0, c->stream, c->d_time, 0.);
+  hipLaunchKernelGGL(k_set_f64, dim3(1), dim3(1), 0, c->stream, c->d_dt, 0.);
+  int rc = check_launch("instance set-up");
+  if (rc) return rc;
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int snsg_create(int device, const snsg_config *cfg, const double *dxf,
+                           const double *dxc, snsg_ctx **out) {
+  if (!cfg || !dxf || !dxc || !out) return set_err(SNSG_E_ARG, "snsg_create: null argument");
+  const int itot = cfg->itot, jtot = cfg->jtot;
+  Fft2Plan pl;
+  int rc = check_sizes(cfg, pl);
+  if (rc) return rc;
+  HIPCHECK(hipSetDevice(device));
+  snsg_ctx *c = new snsg_ctx();
+  c->device = device;
+  c->cfg = *cfg;
+  c->plan = pl;
+  auto fail = [&](int code) {
+    snsg_destroy(c);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(set_err(SNSG_E_ARG, "hipStreamCreate"));
+  const size_t stride = (size_t)itot + 2;
+  const size_t nfac = (size_t)(jtot / 2 + 1) * 3 * poisson_factor_entries(itot);
+  struct {
+    void **p;
+    size_t bytes;
+  } bufs[] = {
+      {(void **)&c->d_dxf, sizeof(double) * itot},
+      {(void **)&c->d_dxc, sizeof(double) * (itot + 1)},
+      {(void **)&c->d_dt, sizeof(double)},
+      {(void **)&c->d_time, sizeof(double)},
+      {(void **)&c->d_eigen, sizeof(double) * (jtot / 2 + 1)},
+      {(void **)&c->d_tw, sizeof(double2) * jtot},
+      {(void **)&c->d_psihat, sizeof(double2) * (size_t)itot * (jtot / 2 + 1)},
+      {(void **)&c->d_Y, sizeof(double2) * (size_t)pl.npair * jtot},
+      {(void **)&c->d_q, sizeof(double) * (size_t)(itot + 1) * jtot},
+      {(void **)&c->d_qy, sizeof(double) * (size_t)(itot + 1) * jtot},
+      {(void **)&c->d_red, sizeof(unsigned long long)},
+      {(void **)&c->d_tab, sizeof(double) * kTables * stride},
+      {(void **)&c->d_tab2, sizeof(double2) * kTables2 * stride},
+      {(void **)&c->d_pfac, sizeof(double) * nfac},
+  };
+  for (auto &b : bufs)
+    if ((rc = ctx_alloc(b.p, b.bytes, &c->mem))) return fail(rc);
+  if (hipHostMalloc((void **)&c->h_red, sizeof(unsigned long long), hipHostMallocDefault) !=
+      hipSuccess)
+    return fail(set_err(SNSG_E_NOMEM, "hipHostMalloc"));
+  if ((rc = ctx_configure(c, cfg, dxf, dxc))) return fail(rc);
   *out = c;
   return 0;
 }
@@ -304,7 +298,6 @@ extern "C" int snsg_set_treatment(snsg_ctx *c, int implicitx, int implicity) {
   return 0;
 }
 
-__global__ void k_set_f64(double *p, double v) { *p = v; }
 
 extern "C" int snsg_set_dt(snsg_ctx *c, double dt) {
   if (!c) return set_err(SNSG_E_ARG, "null ctx");
@@ -380,7 +373,7 @@ extern "C" int snsg_temperature_update_boundaries_temp(snsg_ctx *c, double *temp
 
 // ------------------------------------------------- predictor internals
 
-static RK rk_of(const snsg_ctx *c, int rkstep) {
+RK rk_of(const snsg_ctx *c, int rkstep) {
   RK r;
   r.alpha = c->cfg.alpha[rkstep];
   r.beta = c->cfg.beta[rkstep];
@@ -393,7 +386,7 @@ static RK rk_of(const snsg_ctx *c, int rkstep) {
 // src/fluid/compute_potential.c:15-168 (three kernel groups: forward
 // transform, x solves, inverse transform)
 // g: the layout of ux, uy and psi (c->geo or c->geo_st)
-static PoisArgs pois_args(const snsg_ctx *c, const Geo &g, double gamma, const double *ux,
+PoisArgs pois_args(const snsg_ctx *c, const Geo &g, double gamma, const double *ux,
                           const double *uy, double *psi) {
   PoisArgs a;
   a.g = g;
@@ -552,8 +545,8 @@ static size_t bytes_ux(const snsg_ctx *c) {
 static size_t bytes_c(const snsg_ctx *c) {
   return sizeof(double) * ((size_t)c->geo_st.pc * (c->geo_st.jtot + 2) + 16);
 }
-static int alloc_c(const snsg_ctx *c, double **p) {
-  int rc = ctx_alloc((void **)p, bytes_c(c));
+static int alloc_c(snsg_ctx *c, double **p) {
+  int rc = ctx_alloc((void **)p, bytes_c(c), &c->mem);
   if (!rc) *p += kStateCOff;
   return rc;
 }
@@ -566,17 +559,17 @@ extern "C" int snsg_state_alloc(snsg_ctx *c, snsg_state *st) {
   st->ld_ux = c->geo_st.pu;
   st->ld_c = c->geo_st.pc;
   int rc;
-  if ((rc = ctx_alloc((void **)&st->ux, bytes_ux(c)))) return rc;
+  if ((rc = ctx_alloc((void **)&st->ux, bytes_ux(c), &c->mem))) return rc;
   if ((rc = alloc_c(c, &st->uy))) return rc;
   if ((rc = alloc_c(c, &st->p))) return rc;
   if ((rc = alloc_c(c, &st->psi))) return rc;
   if ((rc = alloc_c(c, &st->temp))) return rc;
   if ((rc = alloc_c(c, &st->temp_alt))) return rc;
-  if ((rc = ctx_alloc((void **)&st->ux_star, bytes_ux(c)))) return rc;
+  if ((rc = ctx_alloc((void **)&st->ux_star, bytes_ux(c), &c->mem))) return rc;
   if ((rc = alloc_c(c, &st->uy_star))) return rc;
-  if ((rc = ctx_alloc((void **)&st->srcuxa, sizeof(double) * (size_t)(itot - 1) * jtot))) return rc;
-  if ((rc = ctx_alloc((void **)&st->srcuya, sizeof(double) * (size_t)itot * jtot))) return rc;
-  if ((rc = ctx_alloc((void **)&st->srctempa, sizeof(double) * (size_t)itot * jtot))) return rc;
+  if ((rc = ctx_alloc((void **)&st->srcuxa, sizeof(double) * (size_t)(itot - 1) * jtot, &c->mem))) return rc;
+  if ((rc = ctx_alloc((void **)&st->srcuya, sizeof(double) * (size_t)itot * jtot, &c->mem))) return rc;
+  if ((rc = ctx_alloc((void **)&st->srctempa, sizeof(double) * (size_t)itot * jtot, &c->mem))) return rc;
   return 0;
 }
 
@@ -589,7 +582,74 @@ extern "C" int snsg_state_free(snsg_ctx *c, snsg_state *st) {
     if (p) hipFree(p);
   for (double *p : cell)
     if (p) hipFree(p - kStateCOff);
+  if (st->ux) {
+    const size_t sx = sizeof(double) * (size_t)(c->geo.itot - 1) * c->geo.jtot;
+    const size_t sy = sizeof(double) * (size_t)c->geo.itot * c->geo.jtot;
+    c->mem -= 2 * bytes_ux(c) + 6 * bytes_c(c) + sx + 2 * sy;
+  }
   memset(st, 0, sizeof *st);
+  return 0;
+}
+
+// Zeroes every array of a state (on the instance's stream), as a new one is.
+int state_reset(snsg_ctx *c, snsg_state *st) {
+  const size_t bu = bytes_ux(c), bc = bytes_c(c);
+  const size_t sx = sizeof(double) * (size_t)(c->geo.itot - 1) * c->geo.jtot;
+  const size_t sy = sizeof(double) * (size_t)c->geo.itot * c->geo.jtot;
+  struct {
+    void *p;
+    size_t bytes;
+  } a[] = {{st->ux, bu},           {st->ux_star, bu},       {st->srcuxa, sx},
+           {st->srcuya, sy},       {st->srctempa, sy},      {st->uy - kStateCOff, bc},
+           {st->p - kStateCOff, bc}, {st->psi - kStateCOff, bc}, {st->temp - kStateCOff, bc},
+           {st->temp_alt - kStateCOff, bc}, {st->uy_star - kStateCOff, bc}};
+  for (auto &x : a) HIPCHECK(hipMemsetAsync(x.p, 0, x.bytes, c->stream));
+  return 0;
+}
+
+extern "C" int snsg_memory(const snsg_ctx *c, size_t *bytes) {
+  if (!c || !bytes) return set_err(SNSG_E_ARG, __func__);
+  *bytes = c->mem;
+  return 0;
+}
+
+extern "C" int snsg_state_init_temperature(snsg_ctx *c, snsg_state *st, uint64_t seed,
+                                           double amplitude) {
+  if (!c || !st || !st->temp) return set_err(SNSG_E_ARG, __func__);
+  HIPCHECK(hipSetDevice(c->device));
+  launch_init_temp(c->stream, c->geo_st, st->temp, seed, amplitude);
+  return check_launch(__func__);
+}
+
+// Device bytes of an instance of cfg's size and treatment with its state and
+// the scratch of the fused step: the allocations of snsg_create,
+// snsg_state_alloc and the fused step's first use, each as ctx_alloc counts it.
+extern "C" int snsg_instance_bytes(const snsg_config *cfg, size_t *bytes) {
+  if (!cfg || !bytes) return set_err(SNSG_E_ARG, __func__);
+  Fft2Plan pl;
+  int rc = check_sizes(cfg, pl);
+  if (rc) return rc;
+  const size_t itot = cfg->itot, jtot = cfg->jtot, d = sizeof(double), z = sizeof(double2);
+  const size_t stride = itot + 2;
+  const size_t pu = (size_t)state_pitch((int)itot + 1), pc = (size_t)state_pitch((int)itot + 2);
+  size_t n = 0;
+  // snsg_create
+  n += d * itot + d * (itot + 1) + d + d + d * (jtot / 2 + 1) + z * jtot;
+  n += z * itot * (jtot / 2 + 1) + z * (size_t)pl.npair * jtot;
+  n += 2 * d * (itot + 1) * jtot + sizeof(unsigned long long);
+  n += d * kTables * stride + z * kTables2 * stride;
+  n += d * (jtot / 2 + 1) * 3 * (size_t)poisson_factor_entries((int)itot);
+  // snsg_state_alloc
+  n += 2 * d * pu * (jtot + 2) + 6 * d * (pc * (jtot + 2) + 16);
+  n += d * (itot - 1) * jtot + 2 * d * itot * jtot;
+  // first fused step
+  if (cfg->implicity) n += d * (2 * itot + itot - 1) * jtot;
+  if (cfg->implicitx) n += d * 27 * (size_t)lu_table_entries((int)itot);
+  Geo g{};
+  g.itot = (int)itot;
+  g.jtot = (int)jtot;
+  n += d * (size_t)corr_partials(g);
+  *bytes = n;
   return 0;
 }
 
@@ -649,7 +709,7 @@ static int diagnostics(snsg_ctx *c, const Geo &g, const double *ux, const double
   int rc;
   const size_t np = (size_t)diag_partials(g) * kDiagSlots;
   if (!c->d_diag) {
-    if ((rc = ctx_alloc((void **)&c->d_diag, sizeof(double) * (np + kDiagSlots)))) return rc;
+    if ((rc = ctx_alloc((void **)&c->d_diag, sizeof(double) * (np + kDiagSlots), &c->mem))) return rc;
     HIPCHECK(hipHostMalloc((void **)&c->h_diag, sizeof(double) * kDiagSlots, hipHostMallocDefault));
   }
   DiagArgs a{ux, uy, c->cfg.with_temperature ? temp : nullptr};
@@ -711,24 +771,10 @@ extern "C" int snsg_state_decide_dt(snsg_ctx *c, const snsg_state *st, double *d
   return 0;
 }
 
-// Argument blocks of one fused RK3 step (src/main.c:79-101) of instance c on
-// state st, with st->temp and st->temp_alt as they are at the start of the
-// step (T and T' trade places after every substage).  `reduce`: the last
-// correction stores the advective-bound minima for the next step in d_part.
-struct StepArgs {
-  LuFactorArgs lu;
-  DtArgs dt;
-  CorrArgs bound;  // advective bound of the state as it is (d_part)
-  PredArgs pred[3];
-  YArgs y[3][3];   // [substage][T, ux, uy]
-  PoisArgs pois[3];
-  CorrArgs corr[3];
-};
-
-static const int kYKinds[3] = {KIND_TEMP, KIND_UX, KIND_UY};
+const int kYKinds[3] = {KIND_TEMP, KIND_UX, KIND_UY};
 
 // Scratch of the fused step, allocated on first use
-static int step_prepare(snsg_ctx *c) {
+int step_prepare(snsg_ctx *c) {
   const Geo &g = c->geo_st;
   const int itot = g.itot, jtot = g.jtot;
   int rc;
@@ -737,20 +783,20 @@ static int step_prepare(snsg_ctx *c) {
   if (c->treat.implicity && !c->d_qf[0]) {
     const size_t nb[3] = {(size_t)itot * jtot, (size_t)(itot - 1) * jtot, (size_t)itot * jtot};
     for (int k = 0; k < 3; k++)
-      if ((rc = ctx_alloc((void **)&c->d_qf[k], sizeof(double) * nb[k]))) return rc;
+      if ((rc = ctx_alloc((void **)&c->d_qf[k], sizeof(double) * nb[k], &c->mem))) return rc;
   }
   if (c->treat.implicitx && !c->d_lu) {
     c->lu_ntm = lu_table_entries(itot);
-    if ((rc = ctx_alloc((void **)&c->d_lu, sizeof(double) * 27 * (size_t)c->lu_ntm))) return rc;
+    if ((rc = ctx_alloc((void **)&c->d_lu, sizeof(double) * 27 * (size_t)c->lu_ntm, &c->mem))) return rc;
   }
   if (!c->d_part) {
     c->nparts = corr_partials(g);
-    if ((rc = ctx_alloc((void **)&c->d_part, sizeof(double) * (size_t)c->nparts))) return rc;
+    if ((rc = ctx_alloc((void **)&c->d_part, sizeof(double) * (size_t)c->nparts, &c->mem))) return rc;
   }
   return 0;
 }
 
-static void step_args(const snsg_ctx *c, const snsg_state *st, bool reduce, StepArgs &sa) {
+void step_args(const snsg_ctx *c, const snsg_state *st, bool reduce, StepArgs &sa) {
   const Geo &g = c->geo_st;
   const bool impx = c->treat.implicitx, impy = c->treat.implicity;
   const bool with_temp = c->cfg.with_temperature != 0;
@@ -878,8 +924,9 @@ extern "C" int snsg_advance(snsg_ctx *c, snsg_state *st, int nsteps) {
 extern "C" int snsg_get_time(snsg_ctx *c, double *time, double *dt) {
   if (!c) return set_err(SNSG_E_ARG, __func__);
   HIPCHECK(hipStreamSynchronize(c->stream));
-  if (time) HIPCHECK(hipMemcpy(time, c->d_time, sizeof(double), hipMemcpyDeviceToHost));
-  if (dt) HIPCHECK(hipMemcpy(dt, c->d_dt, sizeof(double), hipMemcpyDeviceToHost));
+  if (time) HIPCHECK(hipMemcpyAsync(time, c->d_time, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (dt) HIPCHECK(hipMemcpyAsync(dt, c->d_dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
   return 0;
 }
 
@@ -924,312 +971,5 @@ extern "C" int snsg_timing_read(snsg_ctx *c, int *launches, double *total_ms) {
   *launches = (int)(c->ev_used / 2);
   *total_ms = tot;
   c->ev_used = 0;
-  return 0;
-}
-
-// ------------------------------------------------------------ batches
-
-// Instances of one shape class: same itot, jtot, treatment and temperature
-// equation, hence the same launch configuration for every kernel.  Each
-// kernel of a step is launched once for the class, with grid.z = number of
-// members and the members' argument blocks in device tables.  Only the
-// predictor and the y solve depend on which of the two temperature buffers
-// holds T (the parity), so they have a table per parity.
-struct BatchClass {
-  std::vector<int> members;
-  int impx = 0, impy = 0, with_temp = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t done = nullptr;
-  StepArgs shape;  // argument blocks of the first member (launch configuration)
-  char *d_tab = nullptr;
-  LuFactorArgs *lu = nullptr;
-  DtArgs *dt = nullptr;
-  CorrArgs *bound = nullptr;
-  PredArgs *pred[2][3] = {};
-  YArgs *y[2][3][3] = {};
-  PoisArgs *pois[3] = {};
-  CorrArgs *corr[3] = {};
-  hipGraph_t graph[2] = {};
-  hipGraphExec_t exec[2] = {};
-};
-
-struct snsg_batch {
-  int device = 0;
-  int flags = 0;
-  std::vector<snsg_ctx *> ctx;
-  std::vector<snsg_state *> st;
-  std::vector<double *> temp0;  // st->temp of parity 0
-  std::vector<BatchClass> cls;
-  std::vector<hipEvent_t> ready;  // one per member stream
-};
-
-static void batch_free(snsg_batch *b) {
-  if (!b) return;
-  (void)hipSetDevice(b->device);
-  for (BatchClass &k : b->cls) {
-    if (k.stream) (void)hipStreamSynchronize(k.stream);
-    for (int p = 0; p < 2; p++) {
-      if (k.exec[p]) (void)hipGraphExecDestroy(k.exec[p]);
-      if (k.graph[p]) (void)hipGraphDestroy(k.graph[p]);
-    }
-    if (k.d_tab) (void)hipFree(k.d_tab);
-    if (k.done) (void)hipEventDestroy(k.done);
-    if (k.stream) (void)hipStreamDestroy(k.stream);
-  }
-  for (hipEvent_t e : b->ready) (void)hipEventDestroy(e);
-  delete b;
-}
-
-// Lays out the class's argument tables in one device allocation.
-static int batch_tables(snsg_batch *b, BatchClass &k) {
-  const size_t nz = k.members.size();
-  std::vector<char> host;
-  auto reserve = [&](size_t bytes) {
-    const size_t off = (host.size() + 255) / 256 * 256;
-    host.resize(off + bytes);
-    return off;
-  };
-  std::vector<StepArgs> sa[2];
-  for (int p = 0; p < 2; p++) {
-    sa[p].resize(nz);
-    for (size_t m = 0; m < nz; m++) {
-      const int id = k.members[m];
-      snsg_state s = *b->st[id];
-      if (p == 1 && k.with_temp) std::swap(s.temp, s.temp_alt);
-      step_args(b->ctx[id], &s, true, sa[p][m]);
-    }
-  }
-  k.shape = sa[0][0];
-  // offsets first, then pointers into the device allocation
-  const size_t o_lu = reserve(nz * sizeof(LuFactorArgs));
-  const size_t o_dt = reserve(nz * sizeof(DtArgs));
-  const size_t o_bound = reserve(nz * sizeof(CorrArgs));
-  size_t o_pred[2][3], o_y[2][3][3], o_pois[3], o_corr[3];
-  for (int p = 0; p < 2; p++)
-    for (int rk = 0; rk < 3; rk++) {
-      o_pred[p][rk] = reserve(nz * sizeof(PredArgs));
-      for (int f = 0; f < 3; f++) o_y[p][rk][f] = reserve(nz * sizeof(YArgs));
-    }
-  for (int rk = 0; rk < 3; rk++) {
-    o_pois[rk] = reserve(nz * sizeof(PoisArgs));
-    o_corr[rk] = reserve(nz * sizeof(CorrArgs));
-  }
-  auto put = [&](size_t off, size_t m, const void *src, size_t sz) {
-    memcpy(host.data() + off + m * sz, src, sz);
-  };
-  for (size_t m = 0; m < nz; m++) {
-    put(o_lu, m, &sa[0][m].lu, sizeof(LuFactorArgs));
-    put(o_dt, m, &sa[0][m].dt, sizeof(DtArgs));
-    put(o_bound, m, &sa[0][m].bound, sizeof(CorrArgs));
-    for (int p = 0; p < 2; p++)
-      for (int rk = 0; rk < 3; rk++) {
-        put(o_pred[p][rk], m, &sa[p][m].pred[rk], sizeof(PredArgs));
-        for (int f = 0; f < 3; f++) put(o_y[p][rk][f], m, &sa[p][m].y[rk][f], sizeof(YArgs));
-      }
-    for (int rk = 0; rk < 3; rk++) {
-      put(o_pois[rk], m, &sa[0][m].pois[rk], sizeof(PoisArgs));
-      put(o_corr[rk], m, &sa[0][m].corr[rk], sizeof(CorrArgs));
-    }
-  }
-  int rc;
-  if ((rc = ctx_alloc((void **)&k.d_tab, host.size()))) return rc;
-  HIPCHECK(hipMemcpy(k.d_tab, host.data(), host.size(), hipMemcpyHostToDevice));
-  char *d = k.d_tab;
-  k.lu = (LuFactorArgs *)(d + o_lu);
-  k.dt = (DtArgs *)(d + o_dt);
-  k.bound = (CorrArgs *)(d + o_bound);
-  for (int p = 0; p < 2; p++)
-    for (int rk = 0; rk < 3; rk++) {
-      k.pred[p][rk] = (PredArgs *)(d + o_pred[p][rk]);
-      for (int f = 0; f < 3; f++) k.y[p][rk][f] = (YArgs *)(d + o_y[p][rk][f]);
-    }
-  for (int rk = 0; rk < 3; rk++) {
-    k.pois[rk] = (PoisArgs *)(d + o_pois[rk]);
-    k.corr[rk] = (CorrArgs *)(d + o_corr[rk]);
-  }
-  return 0;
-}
-
-// One time-loop iteration (dt decision + RK3 step) of every member of class
-// k, whose T is in the parity-p buffer, enqueued on the class stream.
-static void batch_step_launch(const BatchClass &k, int p) {
-  const int nz = (int)k.members.size();
-  const hipStream_t s = k.stream;
-  const StepArgs &sh = k.shape;
-  launch_dt_batch(s, k.dt, nz);
-  if (k.impx) launch_lu_factor_batch(s, sh.lu, k.lu, nz);
-  for (int rk = 0; rk < 3; rk++) {
-    launch_pred_batch(s, sh.pred[rk], k.impx, k.pred[p][rk], nz);
-    if (k.impy)
-      for (int f = 0; f < 3; f++) {
-        if (f == 0 && !k.with_temp) continue;
-        launch_ysolve_batch(s, kYKinds[f], sh.y[rk][f], k.y[p][rk][f], nz);
-      }
-    for (int stage = 0; stage < 3; stage++)
-      launch_poisson_batch(s, sh.pois[rk], k.pois[rk], nz, stage);
-    launch_corr_batch(s, sh.corr[rk], k.corr[rk], nz);
-  }
-}
-
-static int batch_capture(BatchClass &k, int p) {
-  HIPCHECK(hipStreamBeginCapture(k.stream, hipStreamCaptureModeThreadLocal));
-  batch_step_launch(k, p);
-  const hipError_t le = hipGetLastError();
-  hipGraph_t g = nullptr;
-  const hipError_t ce = hipStreamEndCapture(k.stream, &g);
-  if (le != hipSuccess) {
-    if (g) (void)hipGraphDestroy(g);
-    return set_err((int)le, "batch: launch during graph capture");
-  }
-  if (ce != hipSuccess) return set_err((int)ce, "hipStreamEndCapture");
-  k.graph[p] = g;
-  HIPCHECK(hipGraphInstantiate(&k.exec[p], g, nullptr, nullptr, 0));
-  return 0;
-}
-
-extern "C" int snsg_batch_create(int n, snsg_ctx *const *ctx, snsg_state *const *st, int flags,
-                                 snsg_batch **out) {
-  if (n < 1 || !ctx || !st || !out) return set_err(SNSG_E_ARG, "snsg_batch_create: arguments");
-  for (int m = 0; m < n; m++) {
-    if (!ctx[m] || !st[m] || !st[m]->ux)
-      return set_err(SNSG_E_ARG, "snsg_batch_create: null instance or unallocated state");
-    if (ctx[m]->device != ctx[0]->device)
-      return set_err(SNSG_E_ARG, "snsg_batch_create: instances on different devices");
-    for (int q = 0; q < m; q++)
-      if (st[q] == st[m] || st[q]->ux == st[m]->ux || ctx[q] == ctx[m])
-        return set_err(SNSG_E_ARG, "snsg_batch_create: an instance or state appears twice");
-  }
-  HIPCHECK(hipSetDevice(ctx[0]->device));
-  snsg_batch *b = new snsg_batch();
-  b->device = ctx[0]->device;
-  b->flags = flags;
-  b->ctx.assign(ctx, ctx + n);
-  b->st.assign(st, st + n);
-  int rc = 0;
-  auto fail = [&](int code) {
-    batch_free(b);
-    return code;
-  };
-  for (int m = 0; m < n; m++) {
-    snsg_ctx *c = ctx[m];
-    if (c->geo_st.jtot < 2) return fail(set_err(SNSG_E_SIZE, "snsg_batch_create: jtot < 2"));
-    if ((rc = step_prepare(c))) return fail(rc);
-    b->temp0.push_back(st[m]->temp);
-    // shape class
-    const int impx = c->treat.implicitx, impy = c->treat.implicity;
-    const int wt = c->cfg.with_temperature != 0;
-    int found = -1;
-    for (size_t k = 0; k < b->cls.size(); k++) {
-      const snsg_ctx *r = ctx[b->cls[k].members[0]];
-      if (r->geo.itot == c->geo.itot && r->geo.jtot == c->geo.jtot && b->cls[k].impx == impx &&
-          b->cls[k].impy == impy && b->cls[k].with_temp == wt)
-        found = (int)k;
-    }
-    if (found < 0) {
-      b->cls.emplace_back();
-      BatchClass &k = b->cls.back();
-      k.impx = impx;
-      k.impy = impy;
-      k.with_temp = wt;
-      found = (int)b->cls.size() - 1;
-    }
-    b->cls[found].members.push_back(m);
-  }
-  for (BatchClass &k : b->cls) {
-    if (hipStreamCreateWithFlags(&k.stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&k.done, hipEventDisableTiming) != hipSuccess)
-      return fail(set_err(SNSG_E_ARG, "snsg_batch_create: stream or event"));
-    if ((rc = batch_tables(b, k))) return fail(rc);
-  }
-  b->ready.resize(n, nullptr);
-  for (int m = 0; m < n; m++)
-    if (hipEventCreateWithFlags(&b->ready[m], hipEventDisableTiming) != hipSuccess)
-      return fail(set_err(SNSG_E_ARG, "snsg_batch_create: event"));
-  *out = b;
-  return 0;
-}
-
-// The parity of the members of class k (0: T in the buffer it was in at
-// creation), or -1 if they differ (some member was advanced on its own an
-// odd number of steps).
-static int batch_parity(const snsg_batch *b, const BatchClass &k) {
-  int par = -2;
-  for (int id : k.members) {
-    const int p = b->st[id]->temp == b->temp0[id] ? 0 : 1;
-    if (par == -2) par = p;
-    else if (p != par) return -1;
-  }
-  return k.with_temp ? par : 0;
-}
-
-extern "C" int snsg_batch_advance(snsg_batch *b, int nsteps) {
-  if (!b || nsteps < 0) return set_err(SNSG_E_ARG, __func__);
-  if (nsteps == 0) return 0;
-  HIPCHECK(hipSetDevice(b->device));
-  std::vector<int> par(b->cls.size());
-  for (size_t q = 0; q < b->cls.size(); q++) {
-    par[q] = batch_parity(b, b->cls[q]);
-    if (par[q] < 0)
-      return set_err(SNSG_E_ARG,
-                     "snsg_batch_advance: members of a class were advanced apart by an odd "
-                     "number of steps");
-  }
-  // the class streams start after the work already queued on the members' streams
-  for (size_t m = 0; m < b->ctx.size(); m++)
-    HIPCHECK(hipEventRecord(b->ready[m], b->ctx[m]->stream));
-  for (BatchClass &k : b->cls) {
-    for (int id : k.members) HIPCHECK(hipStreamWaitEvent(k.stream, b->ready[id], 0));
-    // first dt decision from the state as it is
-    launch_adv_bound_batch(k.stream, k.shape.bound, k.bound, (int)k.members.size());
-    int rc = check_launch("snsg_batch_advance: advective bound");
-    if (rc) return rc;
-  }
-  const bool graphs = !(b->flags & SNSG_BATCH_EAGER);
-  for (int n = 0; n < nsteps; n++) {
-    for (size_t q = 0; q < b->cls.size(); q++) {
-      BatchClass &k = b->cls[q];
-      const int p = par[q];
-      if (graphs && k.exec[p]) {
-        HIPCHECK(hipGraphLaunch(k.exec[p], k.stream));
-      } else {
-        // eager launches (also the first step of a graph-replayed class, which
-        // sets up the kernels before they are recorded)
-        batch_step_launch(k, p);
-        int rc = check_launch("snsg_batch_advance");
-        if (rc) return rc;
-        if (graphs && !k.exec[p]) {
-          if ((rc = batch_capture(k, p))) return rc;
-        }
-      }
-      if (k.with_temp) par[q] ^= 1;
-    }
-  }
-  for (BatchClass &k : b->cls) {
-    HIPCHECK(hipEventRecord(k.done, k.stream));
-    for (int id : k.members) HIPCHECK(hipStreamWaitEvent(b->ctx[id]->stream, k.done, 0));
-  }
-  // T and T' trade places once per step
-  if (nsteps & 1)
-    for (BatchClass &k : b->cls)
-      if (k.with_temp)
-        for (int id : k.members) std::swap(b->st[id]->temp, b->st[id]->temp_alt);
-  return 0;
-}
-
-extern "C" int snsg_batch_synchronize(snsg_batch *b) {
-  if (!b) return set_err(SNSG_E_ARG, __func__);
-  HIPCHECK(hipSetDevice(b->device));
-  for (BatchClass &k : b->cls) HIPCHECK(hipStreamSynchronize(k.stream));
-  return 0;
-}
-
-extern "C" int snsg_batch_classes(const snsg_batch *b, int *nclasses) {
-  if (!b || !nclasses) return set_err(SNSG_E_ARG, __func__);
-  *nclasses = (int)b->cls.size();
-  return 0;
-}
-
-extern "C" int snsg_batch_destroy(snsg_batch *b) {
-  batch_free(b);
   return 0;
 }

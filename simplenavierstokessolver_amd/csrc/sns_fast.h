@@ -34,6 +34,9 @@ __device__ __forceinline__ double div_cr(double a, double b, double r) {
 // (entry I-1 holds the value for column I unless noted).  They live in one
 // allocation of 14 double tables and one of 4 double2 tables, each table
 // `stride` entries long, so an instance's tables cost two pointers.
+// number of double and of double2 tables
+constexpr int kTables = 14, kTables2 = 4;
+
 struct Tables {
   const double *t;             // 14 tables, in the order of the accessors below
   const double2 *t2;           // 4 paired tables

@@ -122,6 +122,8 @@ void launch_pressure(hipStream_t s, const Geo &g, const Treat &tr, double gamma,
 void launch_dt_reduce(hipStream_t s, const Geo &g, const double *ux, const double *uy,
                       unsigned long long *out);
 void launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v, int n);
+// seeded initial temperature noise in the interior, then boundary values
+void launch_init_temp(hipStream_t s, const Geo &g, double *temp, uint64_t seed, double amplitude);
 void launch_copy(hipStream_t s, const void *src, void *dst, size_t n16);
 
 }  // namespace sns

@@ -405,4 +405,33 @@ void launch_dt_reduce(hipStream_t s, const Geo &g, const double *ux, const doubl
   hipLaunchKernelGGL(k_dt_reduce, dim3((unsigned)blocks), dim3(kBlock), 0, s, g, ux, uy, out);
 }
 
+// ------------------------------------------------ initial temperature
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// Interior T(i, j) = A*(2U-1), U = top 53 bits of
+// splitmix64(seed<<40 ^ ((j-1)*itot + (i-1))) times 2^-53: the seeded
+// noise of the host layer's initial condition (which replaces the rand()
+// field of src/temperature/init.c:39-47), evaluated per cell.
+__global__ void k_init_temp(Geo g, double *temp, uint64_t seed, double amplitude) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1;
+  if (i > g.itot) return;
+  const uint64_t key = (seed << 40) ^ ((uint64_t)(j - 1) * (uint64_t)g.itot + (uint64_t)(i - 1));
+  const double u = (double)(splitmix64(key) >> 11) * 0x1.0p-53;
+  temp[c_idx(g, i, j)] = 0. + amplitude * (2. * u - 1.);
+}
+
+void launch_init_temp(hipStream_t s, const Geo &g, double *temp, uint64_t seed,
+                      double amplitude) {
+  hipLaunchKernelGGL(k_init_temp, dim3((g.itot + 255) / 256, g.jtot), dim3(256), 0, s, g, temp,
+                     seed, amplitude);
+  launch_bc_temp(s, g, temp);
+}
+
 }  // namespace sns
