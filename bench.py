@@ -69,6 +69,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--per-kernel", action="store_true", help="print per-kernel timings to stderr")
+    ap.add_argument("--ensemble-steps", type=int, default=300,
+                    help="steps per job of the mixed ensemble (0: skip it)")
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="length of the served request stream (0: skip it)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="instead of the step benchmark: time a plain device copy of this "
                          "many GiB (the known-size kernel that calibrates traffic counters)")
@@ -80,9 +84,8 @@ def make_instance(sns, itot, jtot):
     inst = sns.Instance(itot, jtot, ly=w["ly"], stretch=float(itot), Ra=w["Ra"], Pr=w["Pr"],
                         implicitx=w["implicitx"], implicity=w["implicity"])
     inst.state_alloc()
-    t0 = sns.initial_temperature(itot, jtot, inst.g["xc"], w["seed"], w["amplitude"], False)
-    inst.state_upload("temp", t0)
-    inst.update_boundaries_state = None
+    # seeded noise and the wall / halo boundary values, on the device
+    inst.state_init_temperature(w["seed"], w["amplitude"])
     return inst
 
 
@@ -145,6 +148,130 @@ def copy_calibration(sns, gib, reps=5):
                       "GB/s": 2 * nbytes / (avg * 1e-3) / 1e9}))
 
 
+def log(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
+def treatments(sns, ens, jobs):
+    """Treatment of every job by the reference's rule from the recorded
+    cost table (measured now for sizes the table lacks)."""
+    try:
+        costs = ens.load_costs()
+    except (OSError, ValueError, KeyError):
+        costs = {}
+    missing = sorted({j["nx"] for j in jobs if str(j["nx"]) not in costs})
+    if missing:
+        log("measuring treatment costs for nx %s" % missing)
+        costs.update(ens.measure_costs(sns, missing))
+    for j in jobs:
+        j["implicitx"], j["implicity"] = ens.job_treatment(costs, j, sns.grid)
+    return costs
+
+
+def job_instance(sns, j):
+    inst = sns.Instance(j["nx"], j["ny"], ly=2.0, stretch=float(j["nx"]), Ra=j["Ra"], Pr=j["Pr"],
+                        implicitx=j["implicitx"], implicity=j["implicity"])
+    inst.state_alloc()
+    inst.state_init_temperature(j["seed"], j["amplitude"])
+    return inst
+
+
+def ensemble_run(sns, steps, warmup=2):
+    """The mixed ensemble: its 64 jobs advanced one after another through
+    snsg_advance (the plain single-instance path), then the same jobs from
+    the same initial state as one batch (one launch per kernel and shape
+    class, graph replay).  Both after `warmup` untimed steps."""
+    from simplenavierstokessolver_amd import ensemble as ens
+    jobs = ens.ensemble_jobs(steps=steps)
+    treatments(sns, ens, jobs)
+    cells = sum(j["nx"] * j["ny"] for j in jobs)
+    log("ensemble: %d jobs, %d cells, plain path" % (len(jobs), cells))
+    insts = [job_instance(sns, j) for j in jobs]
+    for i in insts:
+        i.advance(warmup)
+    for i in insts:
+        i.sync()
+    t = time.perf_counter()
+    for i in insts:
+        i.advance(steps)
+        i.sync()
+    t_plain = time.perf_counter() - t
+    for i in insts:
+        i.close()
+    log("ensemble: batched")
+    insts = [job_instance(sns, j) for j in jobs]
+    batch = sns.Batch(insts)
+    batch.advance(warmup)
+    batch.sync()
+    t = time.perf_counter()
+    batch.advance(steps)
+    batch.sync()
+    t_batch = time.perf_counter() - t
+    nclass = batch.classes()
+    batch.close()
+    for i in insts:
+        i.close()
+    mix = {}
+    for j in jobs:
+        k = ens.TREATMENTS[(j["implicitx"], j["implicity"])]
+        mix[k] = mix.get(k, 0) + 1
+    v_b, v_p = cells * steps / t_batch, cells * steps / t_plain
+    return {"config": ens.ENSEMBLE, "value": v_b, "unit": "cell-steps/s",
+            "ms_per_ensemble_step": 1e3 * t_batch / steps,
+            "plain_value": v_p, "plain_ms_per_ensemble_step": 1e3 * t_plain / steps,
+            "speedup_vs_plain": v_b / v_p, "shape_classes": nclass, "treatments": mix,
+            "cells": cells, "steps": steps, "warmup": warmup,
+            "plain": "each job in turn through snsg_advance (fused single-instance path)",
+            "batched": "snsg_batch: one launch per kernel and shape class, a stream and a "
+                       "replayed HIP graph per class"}
+
+
+def served_run(sns, window):
+    """The served request stream for `window` seconds of arrivals: jobs are
+    submitted at their arrival times to the serving engine (200 GB budget);
+    throughput and per-step latency over the job-steps completed in the
+    window.  Jobs still queued or running at its end are dropped."""
+    from simplenavierstokessolver_amd import ensemble as ens
+    jobs = ens.served_jobs(window)
+    treatments(sns, ens, jobs)
+    log("served: %d arrivals over %.0f s" % (len(jobs), window))
+    srv = sns.Server(int(ens.SERVED["budget_bytes"]))
+    srv.reset_stats()
+    t0 = time.perf_counter()
+    k = 0
+    next_log = 5.0
+    while True:
+        now = time.perf_counter() - t0
+        if now >= window:
+            break
+        while k < len(jobs) and jobs[k]["arrival"] <= now:
+            j = jobs[k]
+            srv.submit(j["nx"], j["ny"], j["steps"], Ra=j["Ra"], Pr=j["Pr"],
+                       implicitx=j["implicitx"], implicity=j["implicity"], seed=j["seed"],
+                       amplitude=j["amplitude"])
+            k += 1
+        if now >= next_log:
+            st = srv.stats()
+            log("served: t=%.0f s submitted %d admitted %d completed %d job-steps %d" %
+                (now, st["submitted"], st["admitted"], st["completed"], st["steps"]))
+            next_log += 5.0
+        nxt = jobs[k]["arrival"] if k < len(jobs) else window
+        time.sleep(max(0.0, min(0.005, nxt - now)))
+    st = srv.stats()
+    srv.close()
+    mix = {}
+    for j in jobs[:k]:
+        key = ens.TREATMENTS[(j["implicitx"], j["implicity"])]
+        mix[key] = mix.get(key, 0) + 1
+    return {"config": ens.SERVED, "window_s": window, "value": st["cell_steps"] / st["busy_s"]
+            if st["busy_s"] > 0 else 0.0, "unit": "cell-steps/s",
+            "p50_step_ms": st["p50_ms"], "p99_step_ms": st["p99_ms"], "max_step_ms": st["max_ms"],
+            "submitted": st["submitted"], "admitted": st["admitted"],
+            "completed": st["completed"], "job_steps": st["steps"], "lanes": st["lanes"],
+            "mem_peak_GB": st["mem_peak"] / 1e9, "treatments": mix,
+            "latency": "device duration of the lane step that advanced the job-step"}
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_4096x8192.json")
 
 
@@ -192,7 +319,8 @@ def main():
     inst.timing_disable()
     value = cells * a.steps / el
     res = {
-        "metric": "FP64 cell-steps/s @4096x8192 (%HBM roofline)",
+        "metric": "FP64 cell-steps/s @4096x8192 (%HBM roofline); ensemble cell-steps/s + p99 "
+                  "step lat",
         "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": 1e3 * el / a.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
@@ -218,7 +346,12 @@ def main():
             print("%-12s launches %4d  avg %8.3f ms  share %5.1f%%  %7.0f GB/s" %
                   (k, n, ms / n, 100 * ms / tot, gbs), file=sys.stderr)
     inst.close()
+    if a.ensemble_steps > 0:
+        res["ensemble"] = ensemble_run(sns, a.ensemble_steps)
+    if a.serve_seconds > 0:
+        res["served"] = served_run(sns, a.serve_seconds)
     if not a.no_cpu_baseline:
+        log("cpu baseline")
         res["cpu_baseline"] = cpu_baseline(itot, jtot, a.cpu_steps)
     print(json.dumps(res))
 
