@@ -179,6 +179,9 @@ int snsg_state_init_temperature(snsg_ctx *ctx, snsg_state *st, uint64_t seed, do
  * state once it has stepped (snsg_create + snsg_state_alloc + the fused
  * step's scratch), for admission against a memory budget. */
 int snsg_instance_bytes(const snsg_config *cfg, size_t *bytes);
+/* 1 when the grid is uniform bit for bit away from the walls (the kernels
+ * then take the spacings of the interior columns as constants), else 0 */
+int snsg_uniform_grid(const snsg_ctx *ctx, int *uniform);
 /* Device bytes an instance holds now (its state and scratch included) */
 int snsg_memory(const snsg_ctx *ctx, size_t *bytes);
 /* ---- batches (no counterpart in the reference, which runs one instance per

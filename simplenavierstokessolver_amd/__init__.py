@@ -108,6 +108,7 @@ def _load_hip():
                                                   C.c_double]
         L.snsg_instance_bytes.argtypes = [C.POINTER(SnsgConfig), C.POINTER(C.c_size_t)]
         L.snsg_memory.argtypes = [_vp, C.POINTER(C.c_size_t)]
+        L.snsg_uniform_grid.argtypes = [_vp, C.POINTER(C.c_int)]
         L.snsg_server_create.argtypes = [C.c_int, C.c_size_t, C.POINTER(_vp)]
         L.snsg_server_submit.argtypes = [_vp, C.POINTER(SnsgJob), C.POINTER(C.c_int)]
         L.snsg_server_wait.argtypes = [_vp, C.c_int, C.c_double]
@@ -429,6 +430,11 @@ class Instance:
         _check(self.L.snsg_state_init_temperature(self.ctx, C.byref(self.state), int(seed),
                                                   float(amplitude)),
                "snsg_state_init_temperature")
+
+    def uniform_grid(self):
+        u = C.c_int(0)
+        _check(self.L.snsg_uniform_grid(self.ctx, C.byref(u)), "snsg_uniform_grid")
+        return bool(u.value)
 
     def memory(self):
         n = C.c_size_t(0)

@@ -209,6 +209,24 @@ int ctx_configure(snsg_ctx *c, const snsg_config *cfg, const double *dxf, const 
   tb.stride = (long)stride;
   tb.dy = dy;
   tb.rdy = 1. / dy;
+  // uniform interior: columns I = 2..itot-1 (entry I-1) share every value
+  // the paired tables hold for them; cT_xm(I) for I >= 2 and cT_xp(I) for
+  // I <= itot-1 (the same weight), cU_xm and cU_xp likewise
+  {
+    auto same = [&](const double *v, int from, int to, double &val) {
+      val = v[from];
+      for (int k = from; k <= to; k++)
+        if (memcmp(&v[k], &val, sizeof val) != 0) return false;
+      return true;
+    };
+    double cxp = 0., uxp = 0.;
+    tb.uniform = itot >= 4 && same(t[1], 1, itot - 1, tb.u_dxc) &&
+                 same(t[3], 1, itot - 1, tb.u_rdxc) && same(t[0], 0, itot - 1, tb.u_dxf) &&
+                 same(t[2], 0, itot - 1, tb.u_rdxf) && same(t[4], 1, itot - 1, tb.u_cT) &&
+                 same(t[5], 0, itot - 2, cxp) && memcmp(&cxp, &tb.u_cT, sizeof cxp) == 0 &&
+                 same(t[6], 1, itot - 1, tb.u_cU) && same(t[7], 1, itot - 1, uxp) &&
+                 memcmp(&uxp, &tb.u_cU, sizeof uxp) == 0;
+  }
   // fixed diffusive time-step bounds (src/param/decide_dt.c:48-66)
   const double diffusivity = fmax(sqrtPr / sqrtRa, 1. / sqrtPr / sqrtRa);
   double dx = 1.;
@@ -604,6 +622,12 @@ int state_reset(snsg_ctx *c, snsg_state *st) {
            {st->p - kStateCOff, bc}, {st->psi - kStateCOff, bc}, {st->temp - kStateCOff, bc},
            {st->temp_alt - kStateCOff, bc}, {st->uy_star - kStateCOff, bc}};
   for (auto &x : a) HIPCHECK(hipMemsetAsync(x.p, 0, x.bytes, c->stream));
+  return 0;
+}
+
+extern "C" int snsg_uniform_grid(const snsg_ctx *c, int *uniform) {
+  if (!c || !uniform) return set_err(SNSG_E_ARG, __func__);
+  *uniform = c->tab.uniform;
   return 0;
 }
 

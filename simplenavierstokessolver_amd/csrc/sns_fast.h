@@ -8,8 +8,8 @@
 // theorem (IBM J. Res. Dev. 34(1), 1990; Muller et al., Handbook of
 // Floating-Point Arithmetic, Thm. 4.10) q2 = RN(a/b): the quotient is the
 // correctly rounded IEEE one, bit for bit, in five FMA-class instructions
-// instead of the generic division sequence.  Zero quotients keep the sign
-// of q0, which is the sign of a/b.
+// instead of the generic division sequence.  The result takes the sign of
+// q0, which is the sign of a/b (this only matters for zero quotients).
 //
 // All other per-column constants (interpolation weights, Crank-Nicolson
 // coefficients) are tabulated on the host with the reference's operation
@@ -27,7 +27,26 @@ __device__ __forceinline__ double div_cr(double a, double b, double r) {
   const double q1 = __builtin_fma(r, e0, q0);
   const double e1 = __builtin_fma(-b, q1, a);
   const double q2 = __builtin_fma(r, e1, q1);
-  return q0 == 0.0 ? q0 : q2;
+  // q2 carries the sign of a/b except for a zero quotient, where the
+  // refinement may turn -0 into +0; q0 = a*r has the sign of a/b always
+  return __builtin_copysign(q2, q0);
+}
+
+// ---------------------------------------------------------- lane shifts
+
+// Value of v in lane l-1 (lane 0: 0).
+__device__ __forceinline__ double from_left(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), 0x138, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// Value of v in lane l+1 (lane 63: 0).
+__device__ __forceinline__ double from_right(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), 0x130, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 // Per-column tables, all indexed like the reference arrays they derive from
@@ -42,6 +61,12 @@ struct Tables {
   const double2 *t2;           // 4 paired tables
   long stride;
   double dy, rdy;
+  // Set when the grid is uniform bit for bit away from the walls: every
+  // column I = 2..itot-1 has the same DXF, DXC and interpolation weights
+  // (u_*).  A wave whose columns all lie there uses these values instead of
+  // loading the paired tables; the values are the table entries themselves.
+  int uniform;
+  double u_dxc, u_rdxc, u_dxf, u_rdxf, u_cT, u_cU;
   __host__ __device__ const double *tab(int k) const { return t + k * stride; }
   __host__ __device__ const double *dxf() const { return tab(0); }    // DXF(I)
   __host__ __device__ const double *dxc() const { return tab(1); }    // DXC(I)
@@ -119,12 +144,16 @@ struct TempPt {
 struct CellX {
   double dxc, rdxc, dxc_p, rdxc_p, dxf, rdxf, c_xm, c_xp;
 };
+// LANE: the lanes hold consecutive columns, so the gradient at the left face
+// of column i is the one the lane of column i-1 forms at its right face,
+// bit for bit (same operands, same operations); it is taken from there.
+template <bool LANE = false>
 __device__ __forceinline__ void temp_src_v(const Geo &g, const Tables &tb, const CellX &x,
                                            const TempPt &v, double wex, double wey, double wix,
                                            double wiy, double &srca, double &srcg) {
   const double dy = tb.dy, rdy = tb.rdy;
-  const double dtdx_xm = div_cr(-v.t_xm + v.t_c, x.dxc, x.rdxc);
   const double dtdx_xp = div_cr(-v.t_c + v.t_xp, x.dxc_p, x.rdxc_p);
+  const double dtdx_xm = LANE ? from_left(dtdx_xp) : div_cr(-v.t_xm + v.t_c, x.dxc, x.rdxc);
   const double dtdy_ym = div_cr(-v.t_ym + v.t_c, dy, rdy);
   const double dtdy_yp = div_cr(-v.t_c + v.t_yp, dy, rdy);
   const double adv1 = -x.c_xm * v.ux_c * dtdx_xm - x.c_xp * v.ux_p * dtdx_xp;
@@ -147,12 +176,14 @@ struct UxPt {
 struct FaceX {
   double dxf_m, rdxf_m, dxf, rdxf, c_xm, c_xp, dxc, rdxc;
 };
+template <bool LANE = false>
 __device__ __forceinline__ void ux_src_v(const Geo &g, const Tables &tb, const FaceX &x,
                                          const UxPt &v, double wex, double wey, double wix,
                                          double wiy, double &srca, double &srcg) {
   const double dy = tb.dy, rdy = tb.rdy;
-  const double duxdx_xm = div_cr(-v.u_xm + v.u_c, x.dxf_m, x.rdxf_m);
   const double duxdx_xp = div_cr(-v.u_c + v.u_xp, x.dxf, x.rdxf);
+  const double duxdx_xm =
+      LANE ? from_left(duxdx_xp) : div_cr(-v.u_xm + v.u_c, x.dxf_m, x.rdxf_m);
   const double duxdy_ym = div_cr(-v.u_ym + v.u_c, dy, rdy);
   const double duxdy_yp = div_cr(-v.u_c + v.u_yp, dy, rdy);
   const double ux_xm = 0.5 * v.u_xm + 0.5 * v.u_c;
@@ -174,12 +205,13 @@ struct UyPt {
   double ux_mc, ux_cc, ux_mp, ux_cp;   // UX(i,j-1), UX(i,j), UX(i+1,j-1), UX(i+1,j)
   double p_c, p_ym;                    // P(i,j), P(i,j-1)
 };
+template <bool LANE = false>
 __device__ __forceinline__ void uy_src_v(const Geo &g, const Tables &tb, const CellX &x,
                                          const UyPt &v, double wex, double wey, double wix,
                                          double wiy, double &srca, double &srcg) {
   const double dy = tb.dy, rdy = tb.rdy;
-  const double duydx_xm = div_cr(-v.v_xm + v.v_c, x.dxc, x.rdxc);
   const double duydx_xp = div_cr(-v.v_c + v.v_xp, x.dxc_p, x.rdxc_p);
+  const double duydx_xm = LANE ? from_left(duydx_xp) : div_cr(-v.v_xm + v.v_c, x.dxc, x.rdxc);
   const double duydy_ym = div_cr(-v.v_ym + v.v_c, dy, rdy);
   const double duydy_yp = div_cr(-v.v_c + v.v_yp, dy, rdy);
   const double ux_xm = 0.5 * v.ux_mc + 0.5 * v.ux_cc;
@@ -274,23 +306,6 @@ __device__ __forceinline__ void put_c(const Geo &g, double *f, int i, int j, dou
     str(row, iw, wv);
     if (WALL != 2 && copy) str(halo, iw, wv);
   }
-}
-
-// ---------------------------------------------------------- lane shifts
-
-// Value of v in lane l-1 (lane 0: 0).
-__device__ __forceinline__ double from_left(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), 0x138, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-// Value of v in lane l+1 (lane 63: 0).
-__device__ __forceinline__ double from_right(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), 0x130, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }

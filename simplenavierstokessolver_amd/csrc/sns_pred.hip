@@ -159,164 +159,277 @@ __device__ __forceinline__ int pad_idx(int r) {
 
 constexpr int kWaveCols = 62;  // output columns per wave (lanes 1..62)
 
-// Workgroup w: field w % nfld (+1 without the temperature equation) of rows
-// a = 2q+1 and b = a+1, q = w / nfld.  For odd jtot the last pair is shifted
-// down to (jtot-1, jtot) and writes row b only.  Consecutive w share an XCD.
-template <int NT, int M, class S>
-__global__ void __launch_bounds__(NT) k_pred_rows(S src, int nfld) {
-  const PredArgs a = src.get();
-  const double *lu = a.lu;
-  const int ntm = a.ntm;
-  extern __shared__ double lds_raw[];
-  PredRowsLds<NT, M> &L = *reinterpret_cast<PredRowsLds<NT, M> *>(lds_raw);
-  constexpr int NW = NT / 64;
+// Lane position of stencil block k: unknown index ii (lanes 1..62 of each
+// wave own a column, lanes 0 and 63 hold the neighbours), the column i and
+// the clamped element indices of the cell arrays (ci), ux (ui), the column
+// tables (ti) and the RK history (hi).
+struct LanePos {
+  int ii, i, ci, ui, ti, hi;
+  bool out;
+  bool interior;  // all 64 lanes of the wave on columns 2..itot-1 (wave-uniform)
+};
+
+__device__ __forceinline__ LanePos lane_pos(int k, int nw, int i0, int n, int itot) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  LanePos p;
+  p.ii = (k * nw + wave) * kWaveCols + lane - 1;
+  p.i = p.ii + i0;
+  p.out = lane >= 1 && lane <= kWaveCols && p.ii < n;
+  p.ci = clampi(p.i, 0, itot + 1);
+  p.ui = clampi(p.i - 1, 0, itot);
+  p.ti = clampi(p.i - 1, 0, itot + 1);
+  p.hi = clampi(p.ii, 0, n - 1);
+  const int c0 = (k * nw + wave) * kWaveCols - 1 + i0;  // column of lane 0
+  p.interior = c0 >= 2 && c0 + 63 <= itot - 1;
+  return p;
+}
+
+// The values one lane loads for the two rows of field F (0: T, 1: ux,
+// 2: uy): the substage-start fields around rows ja, jb, the column tables
+// and the RK history.
+template <int F>
+struct Raw;
+template <>
+struct Raw<0> {
+  double t_am, t_a, t_b, t_bp, u_a, u_b, v_a, v_b, v_bp, ba, bb;
+  double2 c2, f2, w2;
+};
+template <>
+struct Raw<1> {
+  double u_am, u_a, u_b, u_bp, v_a, v_b, v_bp, p_a, p_b, t_a, t_b, ba, bb;
+  double2 c2, f2, w2;
+};
+template <>
+struct Raw<2> {
+  double v_am, v_a, v_b, v_bp, u_am, u_a, u_b, p_am, p_a, p_b, ba, bb;
+  double2 c2, f2, w2;
+};
+
+template <int F>
+__device__ __forceinline__ Raw<F> raw_load(const PredArgs &a, const LanePos &p, int ja, int jb,
+                                           int n) {
   const Geo &g = a.g;
   const Tables &tb = a.tb;
-  const int itot = g.itot;
-  const int w = xcd_remap(blockIdx.x, gridDim.x);
-  const int q = w / nfld;
-  const int f = w % nfld + (nfld == 3 ? 0 : 1);
-  const int ja = min(2 * q + 1, g.jtot - 1), jb = ja + 1;
-  const bool write_a = ja == 2 * q + 1;
-  const int n = f == 1 ? itot - 1 : itot;
-  const int i0 = f == 1 ? 2 : 1;
-  const double dt = *a.dtp;
-  const double wex = a.tr.w_ex_x, wey = a.tr.w_ex_y, wix = a.tr.w_im_x, wiy = a.tr.w_im_y;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const size_t wc = (size_t)g.pc, wu = (size_t)g.pu;
+  const double *T = a.temp, *U = a.ux, *V = a.uy, *P = a.p;
+  Raw<F> r;
+  const double *H = F == 0 ? a.hist_t : (F == 1 ? a.hist_ux : a.hist_uy);
+  r.ba = a.read_hist ? ldr(H + (size_t)(ja - 1) * n, p.hi) : 0.;
+  r.bb = a.read_hist ? ldr(H + (size_t)(jb - 1) * n, p.hi) : 0.;
+  if (p.interior && tb.uniform) {
+    r.c2 = make_double2(tb.u_dxc, tb.u_rdxc);
+    r.f2 = make_double2(tb.u_dxf, tb.u_rdxf);
+    const double w = F == 1 ? tb.u_cU : tb.u_cT;
+    r.w2 = make_double2(w, w);
+  } else {
+    r.c2 = ldr2(tb.dxc2(), p.ti);
+    r.f2 = ldr2(tb.dxf2(), p.ti);
+    r.w2 = ldr2(F == 1 ? tb.cU2() : tb.cT2(), p.ti);
+  }
+  if constexpr (F == 0) {
+    r.t_am = ldr(T + (ja - 1) * wc, p.ci);
+    r.t_a = ldr(T + ja * wc, p.ci);
+    r.t_b = ldr(T + jb * wc, p.ci);
+    r.t_bp = ldr(T + (jb + 1) * wc, p.ci);
+    r.u_a = ldr(U + ja * wu, p.ui);
+    r.u_b = ldr(U + jb * wu, p.ui);
+    r.v_a = ldr(V + ja * wc, p.ci);
+    r.v_b = ldr(V + jb * wc, p.ci);
+    r.v_bp = ldr(V + (jb + 1) * wc, p.ci);
+  } else if constexpr (F == 1) {
+    r.u_am = ldr(U + (ja - 1) * wu, p.ui);
+    r.u_a = ldr(U + ja * wu, p.ui);
+    r.u_b = ldr(U + jb * wu, p.ui);
+    r.u_bp = ldr(U + (jb + 1) * wu, p.ui);
+    r.v_a = ldr(V + ja * wc, p.ci);
+    r.v_b = ldr(V + jb * wc, p.ci);
+    r.v_bp = ldr(V + (jb + 1) * wc, p.ci);
+    r.p_a = ldr(P + ja * wc, p.ci);
+    r.p_b = ldr(P + jb * wc, p.ci);
+    r.t_a = a.forcing ? ldr(T + ja * wc, p.ci) : 0.;
+    r.t_b = a.forcing ? ldr(T + jb * wc, p.ci) : 0.;
+  } else {
+    r.v_am = ldr(V + (ja - 1) * wc, p.ci);
+    r.v_a = ldr(V + ja * wc, p.ci);
+    r.v_b = ldr(V + jb * wc, p.ci);
+    r.v_bp = ldr(V + (jb + 1) * wc, p.ci);
+    r.u_am = ldr(U + (ja - 1) * wu, p.ui);
+    r.u_a = ldr(U + ja * wu, p.ui);
+    r.u_b = ldr(U + jb * wu, p.ui);
+    r.p_am = ldr(P + (ja - 1) * wc, p.ci);
+    r.p_a = ldr(P + ja * wc, p.ci);
+    r.p_b = ldr(P + jb * wc, p.ci);
+  }
+  return r;
+}
+
+// Source terms and RK increments of both rows from the loaded values (x
+// neighbours from the adjacent lanes); the new RK history is stored.
+template <int F>
+__device__ __forceinline__ void raw_eval(const PredArgs &a, const Raw<F> &r, const LanePos &p,
+                                         int ja, int jb, int n, bool write_a, double dt,
+                                         double &da, double &db) {
+  const Geo &g = a.g;
+  const Tables &tb = a.tb;
+  const double wex = a.tr.w_ex_x, wey = a.tr.w_ex_y, wix = a.tr.w_im_x, wiy = a.tr.w_im_y;
+  double *H = F == 0 ? a.hist_t : (F == 1 ? a.hist_ux : a.hist_uy);
+  double *HA = H + (size_t)(ja - 1) * n, *HB = H + (size_t)(jb - 1) * n;
+  double A, G;
+  if constexpr (F == 0) {
+    CellX x;
+    x.dxc = r.c2.x;
+    x.rdxc = r.c2.y;
+    x.dxf = r.f2.x;
+    x.rdxf = r.f2.y;
+    x.c_xm = r.w2.x;
+    x.c_xp = r.w2.y;
+    x.dxc_p = from_right(x.dxc);
+    x.rdxc_p = from_right(x.rdxc);
+    const TempPt pa = {r.t_a, from_left(r.t_a), from_right(r.t_a), r.t_am, r.t_b,
+                       r.u_a, from_right(r.u_a), r.v_a, r.v_b};
+    const TempPt pb = {r.t_b, from_left(r.t_b), from_right(r.t_b), r.t_a, r.t_bp,
+                       r.u_b, from_right(r.u_b), r.v_b, r.v_bp};
+    temp_src_v<true>(g, tb, x, pa, wex, wey, wix, wiy, A, G);
+    if (a.write_hist && p.out && write_a) str(HA, p.ii, A);
+    da = rk_incr(a.alpha, a.beta, a.gamma, dt, A, r.ba, G);
+    temp_src_v<true>(g, tb, x, pb, wex, wey, wix, wiy, A, G);
+    if (a.write_hist && p.out) str(HB, p.ii, A);
+    db = rk_incr(a.alpha, a.beta, a.gamma, dt, A, r.bb, G);
+  } else if constexpr (F == 1) {
+    FaceX x;
+    x.dxf = r.f2.x;
+    x.rdxf = r.f2.y;
+    x.c_xm = r.w2.x;
+    x.c_xp = r.w2.y;
+    x.dxc = r.c2.x;
+    x.rdxc = r.c2.y;
+    x.dxf_m = from_left(x.dxf);
+    x.rdxf_m = from_left(x.rdxf);
+    double tf_a = 0., tf_b = 0.;
+    if (a.forcing) {
+      tf_a = 0.5 * from_left(r.t_a) + 0.5 * r.t_a;
+      tf_b = 0.5 * from_left(r.t_b) + 0.5 * r.t_b;
+    }
+    const double v_bm = from_left(r.v_b);
+    const UxPt pa = {r.u_a, from_left(r.u_a), from_right(r.u_a), r.u_am, r.u_b,
+                     from_left(r.v_a), r.v_a, v_bm, r.v_b, r.p_a, from_left(r.p_a), tf_a};
+    const UxPt pb = {r.u_b, from_left(r.u_b), from_right(r.u_b), r.u_a, r.u_bp,
+                     v_bm, r.v_b, from_left(r.v_bp), r.v_bp, r.p_b, from_left(r.p_b), tf_b};
+    ux_src_v<true>(g, tb, x, pa, wex, wey, wix, wiy, A, G);
+    if (a.write_hist && p.out && write_a) str(HA, p.ii, A);
+    da = rk_incr(a.alpha, a.beta, a.gamma, dt, A, r.ba, G);
+    ux_src_v<true>(g, tb, x, pb, wex, wey, wix, wiy, A, G);
+    if (a.write_hist && p.out) str(HB, p.ii, A);
+    db = rk_incr(a.alpha, a.beta, a.gamma, dt, A, r.bb, G);
+  } else {
+    CellX x;
+    x.dxc = r.c2.x;
+    x.rdxc = r.c2.y;
+    x.dxf = r.f2.x;
+    x.rdxf = r.f2.y;
+    x.c_xm = r.w2.x;
+    x.c_xp = r.w2.y;
+    x.dxc_p = from_right(x.dxc);
+    x.rdxc_p = from_right(x.rdxc);
+    const double u_ap = from_right(r.u_a);
+    const UyPt pa = {r.v_a, from_left(r.v_a), from_right(r.v_a), r.v_am, r.v_b,
+                     r.u_am, r.u_a, from_right(r.u_am), u_ap, r.p_a, r.p_am};
+    const UyPt pb = {r.v_b, from_left(r.v_b), from_right(r.v_b), r.v_a, r.v_bp,
+                     r.u_a, r.u_b, u_ap, from_right(r.u_b), r.p_b, r.p_a};
+    uy_src_v<true>(g, tb, x, pa, wex, wey, wix, wiy, A, G);
+    if (a.write_hist && p.out && write_a) str(HA, p.ii, A);
+    da = rk_incr(a.alpha, a.beta, a.gamma, dt, A, r.ba, G);
+    uy_src_v<true>(g, tb, x, pb, wex, wey, wix, wiy, A, G);
+    if (a.write_hist && p.out) str(HB, p.ii, A);
+    db = rk_incr(a.alpha, a.beta, a.gamma, dt, A, r.bb, G);
+  }
+}
+
+// Blocks of kWaveCols columns per wave of the stencil phase: at most
+// ceil(ceil(NT*M / kWaveCols) / (NT/64)), 9 for M = 8.
+template <int NT, int M>
+constexpr int stencil_blocks() {
+  return ((NT * M + kWaveCols - 1) / kWaveCols + NT / 64 - 1) / (NT / 64);
+}
+
+// The workgroup's own field at rows ja, jb in Raw (the values the increments
+// are added to).
+template <int F>
+__device__ __forceinline__ double2 raw_own(const Raw<F> &r) {
+  if constexpr (F == 0) return make_double2(r.t_a, r.t_b);
+  else if constexpr (F == 1) return make_double2(r.u_a, r.u_b);
+  else return make_double2(r.v_a, r.v_b);
+}
+
+// Stencil phase of one workgroup: blocks K.. of the nk blocks per wave
+// (compile-time block index).  The workgroup's own field at rows ja, jb is
+// kept per block in own[] for the final values, so those rows are not read
+// a second time after the line solve.
+template <int K, int F, int NT, int M>
+__device__ __forceinline__ void pred_stencils(const PredArgs &a, PredRowsLds<NT, M> &L, int ja,
+                                              int jb, bool write_a, int n, int i0, double dt,
+                                              int nk, double2 (&own)[stencil_blocks<NT, M>()]) {
+  if constexpr (K < stencil_blocks<NT, M>()) {
+    if (K >= nk) return;
+    const LanePos p = lane_pos(K, NT / 64, i0, n, a.g.itot);
+    const Raw<F> r = raw_load<F>(a, p, ja, jb, n);
+    double da, db;
+    raw_eval<F>(a, r, p, ja, jb, n, write_a, dt, da, db);
+    own[K] = raw_own<F>(r);
+    if (p.out) {
+      L.d[0][pad_idx<M>(p.ii)] = da;
+      L.d[1][pad_idx<M>(p.ii)] = db;
+    }
+    pred_stencils<K + 1, F, NT, M>(a, L, ja, jb, write_a, n, i0, dt, nk, own);
+  }
+}
+
+// Final values of blocks K.. of the stencil phase (compile-time block
+// index, so own[] stays in registers): own + increment with boundary values,
+// or the increment itself when a y solve follows.
+template <int K, int F, int NT, int M>
+__device__ __forceinline__ void pred_emit(const PredArgs &a, const PredRowsLds<NT, M> &L,
+                                          const double2 (&own)[stencil_blocks<NT, M>()], int ja,
+                                          int jb, bool write_a, int n, int i0, int nk) {
+  if constexpr (K < stencil_blocks<NT, M>()) {
+    if (K >= nk) return;
+    const LanePos p = lane_pos(K, NT / 64, i0, n, a.g.itot);
+    double *q = F == 0 ? a.q_t : (F == 1 ? a.q_u : a.q_v);
+    if (p.out) {
+#pragma unroll
+      for (int s = 0; s < 2; s++) {
+        if (s == 0 && !write_a) continue;
+        const int j = s == 0 ? ja : jb;
+        const double d = L.d[s][pad_idx<M>(p.ii)];
+        const double o = s == 0 ? own[K].x : own[K].y;
+        if (q) str(q + (size_t)(j - 1) * n, p.ii, d);
+        else if (F == 0) put_c<2>(a.g, a.out_t, p.i, j, o + d);
+        else if (F == 1) put_ux(a.g, a.out_u, p.i, j, o + d);
+        else put_c<0>(a.g, a.out_v, p.i, j, o + d);
+      }
+    }
+    pred_emit<K + 1, F, NT, M>(a, L, own, ja, jb, write_a, n, i0, nk);
+  }
+}
+
+// One workgroup's field F for rows ja and jb (write_a: row ja is written):
+// stencils into LDS, the CN-x solve of both rows, final values.
+template <int F, int NT, int M>
+__device__ __forceinline__ void pred_rows_body(const PredArgs &a, PredRowsLds<NT, M> &L, int ja,
+                                               int jb, bool write_a) {
+  constexpr int NW = NT / 64;
+  constexpr int KB = stencil_blocks<NT, M>();
+  const Geo &g = a.g;
+  const int itot = g.itot;
+  const int n = F == 1 ? itot - 1 : itot;
+  const int i0 = F == 1 ? 2 : 1;
+  const double dt = *a.dtp;
   const int nk = ((n + kWaveCols - 1) / kWaveCols + NW - 1) / NW;
 
   // ---- stencils and RK increments of both rows into LDS (lane order)
-  for (int k = 0; k < nk; k++) {
-    const int ii = (k * NW + wave) * kWaveCols + lane - 1;  // unknown index of this lane
-    const int i = ii + i0;                                    // column
-    const bool out = lane >= 1 && lane <= kWaveCols && ii < n;
-    const int ci = clampi(i, 0, itot + 1);      // cell arrays, columns 0..itot+1
-    const int ui = clampi(i - 1, 0, itot);      // ux, faces 1..itot+1
-    const int ti = clampi(i - 1, 0, itot + 1);  // column tables, entry i-1
-    const int hi = clampi(ii, 0, n - 1);        // RK history, unknown index
-    double da = 0., db = 0.;
-    if (f == 0) {
-      const double *T = a.temp, *U = a.ux, *V = a.uy;
-      const double t_am = ldr(T + (ja - 1) * wc, ci), t_a = ldr(T + ja * wc, ci);
-      const double t_b = ldr(T + jb * wc, ci), t_bp = ldr(T + (jb + 1) * wc, ci);
-      const double u_a = ldr(U + ja * wu, ui), u_b = ldr(U + jb * wu, ui);
-      const double v_a = ldr(V + ja * wc, ci), v_b = ldr(V + jb * wc, ci);
-      const double v_bp = ldr(V + (jb + 1) * wc, ci);
-      CellX x;
-      const double2 c2 = ldr2(tb.dxc2(), ti), f2 = ldr2(tb.dxf2(), ti), w2 = ldr2(tb.cT2(), ti);
-      x.dxc = c2.x;
-      x.rdxc = c2.y;
-      x.dxf = f2.x;
-      x.rdxf = f2.y;
-      x.c_xm = w2.x;
-      x.c_xp = w2.y;
-      x.dxc_p = from_right(x.dxc);
-      x.rdxc_p = from_right(x.rdxc);
-      const double *HA = a.hist_t + (size_t)(ja - 1) * n, *HB = a.hist_t + (size_t)(jb - 1) * n;
-      const double ba = a.read_hist ? ldr(HA, hi) : 0., bb = a.read_hist ? ldr(HB, hi) : 0.;
-      const TempPt pa = {t_a, from_left(t_a), from_right(t_a), t_am, t_b,
-                         u_a, from_right(u_a), v_a, v_b};
-      const TempPt pb = {t_b, from_left(t_b), from_right(t_b), t_a, t_bp,
-                         u_b, from_right(u_b), v_b, v_bp};
-      double A, G;
-      temp_src_v(g, tb, x, pa, wex, wey, wix, wiy, A, G);
-      if (a.write_hist && out && write_a) str((double *)HA, ii, A);
-      da = rk_incr(a.alpha, a.beta, a.gamma, dt, A, ba, G);
-      temp_src_v(g, tb, x, pb, wex, wey, wix, wiy, A, G);
-      if (a.write_hist && out) str((double *)HB, ii, A);
-      db = rk_incr(a.alpha, a.beta, a.gamma, dt, A, bb, G);
-    } else if (f == 1) {
-      const double *T = a.temp, *U = a.ux, *V = a.uy, *P = a.p;
-      const double u_am = ldr(U + (ja - 1) * wu, ui), u_a = ldr(U + ja * wu, ui);
-      const double u_b = ldr(U + jb * wu, ui), u_bp = ldr(U + (jb + 1) * wu, ui);
-      const double v_a = ldr(V + ja * wc, ci), v_b = ldr(V + jb * wc, ci);
-      const double v_bp = ldr(V + (jb + 1) * wc, ci);
-      const double p_a = ldr(P + ja * wc, ci), p_b = ldr(P + jb * wc, ci);
-      FaceX x;
-      const double2 c2 = ldr2(tb.dxc2(), ti), f2 = ldr2(tb.dxf2(), ti), w2 = ldr2(tb.cU2(), ti);
-      x.dxf = f2.x;
-      x.rdxf = f2.y;
-      x.c_xm = w2.x;
-      x.c_xp = w2.y;
-      x.dxc = c2.x;
-      x.rdxc = c2.y;
-      x.dxf_m = from_left(x.dxf);
-      x.rdxf_m = from_left(x.rdxf);
-      double tf_a = 0., tf_b = 0.;
-      if (a.forcing) {
-        const double t_a = ldr(T + ja * wc, ci), t_b = ldr(T + jb * wc, ci);
-        tf_a = 0.5 * from_left(t_a) + 0.5 * t_a;
-        tf_b = 0.5 * from_left(t_b) + 0.5 * t_b;
-      }
-      const double *HA = a.hist_ux + (size_t)(ja - 1) * n, *HB = a.hist_ux + (size_t)(jb - 1) * n;
-      const double ba = a.read_hist ? ldr(HA, hi) : 0., bb = a.read_hist ? ldr(HB, hi) : 0.;
-      const double v_bm = from_left(v_b);
-      const UxPt pa = {u_a, from_left(u_a), from_right(u_a), u_am, u_b,
-                       from_left(v_a), v_a, v_bm, v_b, p_a, from_left(p_a), tf_a};
-      const UxPt pb = {u_b, from_left(u_b), from_right(u_b), u_a, u_bp,
-                       v_bm, v_b, from_left(v_bp), v_bp, p_b, from_left(p_b), tf_b};
-      double A, G;
-      ux_src_v(g, tb, x, pa, wex, wey, wix, wiy, A, G);
-      if (a.write_hist && out && write_a) str((double *)HA, ii, A);
-      da = rk_incr(a.alpha, a.beta, a.gamma, dt, A, ba, G);
-      ux_src_v(g, tb, x, pb, wex, wey, wix, wiy, A, G);
-      if (a.write_hist && out) str((double *)HB, ii, A);
-      db = rk_incr(a.alpha, a.beta, a.gamma, dt, A, bb, G);
-    } else {
-      const double *U = a.ux, *V = a.uy, *P = a.p;
-      const double v_am = ldr(V + (ja - 1) * wc, ci), v_a = ldr(V + ja * wc, ci);
-      const double v_b = ldr(V + jb * wc, ci), v_bp = ldr(V + (jb + 1) * wc, ci);
-      const double u_am = ldr(U + (ja - 1) * wu, ui), u_a = ldr(U + ja * wu, ui);
-      const double u_b = ldr(U + jb * wu, ui);
-      const double p_am = ldr(P + (ja - 1) * wc, ci), p_a = ldr(P + ja * wc, ci);
-      const double p_b = ldr(P + jb * wc, ci);
-      CellX x;
-      const double2 c2 = ldr2(tb.dxc2(), ti), f2 = ldr2(tb.dxf2(), ti), w2 = ldr2(tb.cT2(), ti);
-      x.dxc = c2.x;
-      x.rdxc = c2.y;
-      x.dxf = f2.x;
-      x.rdxf = f2.y;
-      x.c_xm = w2.x;
-      x.c_xp = w2.y;
-      x.dxc_p = from_right(x.dxc);
-      x.rdxc_p = from_right(x.rdxc);
-      const double *HA = a.hist_uy + (size_t)(ja - 1) * n, *HB = a.hist_uy + (size_t)(jb - 1) * n;
-      const double ba = a.read_hist ? ldr(HA, hi) : 0., bb = a.read_hist ? ldr(HB, hi) : 0.;
-      const double u_ap = from_right(u_a);
-      const UyPt pa = {v_a, from_left(v_a), from_right(v_a), v_am, v_b,
-                       u_am, u_a, from_right(u_am), u_ap, p_a, p_am};
-      const UyPt pb = {v_b, from_left(v_b), from_right(v_b), v_a, v_bp,
-                       u_a, u_b, u_ap, from_right(u_b), p_b, p_a};
-      double A, G;
-      uy_src_v(g, tb, x, pa, wex, wey, wix, wiy, A, G);
-      if (a.write_hist && out && write_a) str((double *)HA, ii, A);
-      da = rk_incr(a.alpha, a.beta, a.gamma, dt, A, ba, G);
-      uy_src_v(g, tb, x, pb, wex, wey, wix, wiy, A, G);
-      if (a.write_hist && out) str((double *)HB, ii, A);
-      db = rk_incr(a.alpha, a.beta, a.gamma, dt, A, bb, G);
-    }
-    if (out) {
-      L.d[0][pad_idx<M>(ii)] = da;
-      L.d[1][pad_idx<M>(ii)] = db;
-    }
-  }
+  double2 own[KB];
+  pred_stencils<0, F, NT, M>(a, L, ja, jb, write_a, n, i0, dt, nk, own);
   __syncthreads();
-
-  // ---- values the increments are added to, fetched now so that their
-  // latency overlaps the line solve (emit order: unknown t + k*NT)
-  const double *oldf = f == 0 ? a.temp : (f == 1 ? a.ux : a.uy);
-  const size_t wf = f == 1 ? wu : wc;
-  const int io = f == 1 ? -1 : 0;  // column i -> element offset in the row
-  double old[2][M];
-#pragma unroll
-  for (int s = 0; s < 2; s++)
-#pragma unroll
-    for (int k = 0; k < M; k++) {
-      const int ii = threadIdx.x + k * NT;
-      const int j = s == 0 ? ja : jb;
-      old[s][k] = (ii < n && !a.q_t) ? ldr(oldf + j * wf, ii + i0 + io) : 0.;
-    }
 
   // ---- Crank-Nicolson in x of both rows (chunk order)
   double dd[2][M];
@@ -327,8 +440,8 @@ __global__ void __launch_bounds__(NT) k_pred_rows(S src, int nfld) {
       const int r = threadIdx.x * M + k;
       dd[s][k] = r < n ? L.d[s][pad_idx<M>(r)] : 0.;
     }
-  const double *b = lu + (size_t)(f * 3) * ntm;
-  const LuTab tab = {b, b + ntm, b + 2 * ntm};
+  const double *b = a.lu + (size_t)(F * 3) * a.ntm;
+  const LuTab tab = {b, b + a.ntm, b + 2 * a.ntm};
   lu_solve_shared<NT, M, 2>(dd, tab, L.scr);
 #pragma unroll
   for (int s = 0; s < 2; s++)
@@ -339,29 +452,27 @@ __global__ void __launch_bounds__(NT) k_pred_rows(S src, int nfld) {
     }
   __syncthreads();
 
-  // ---- final values with boundary values, or the increments for a y solve
-#pragma unroll
-  for (int k = 0; k < M; k++) {
-    const int ii = threadIdx.x + k * NT;
-    if (ii >= n) break;
-    const int i = ii + i0;
-#pragma unroll
-    for (int s = 0; s < 2; s++) {
-      if (s == 0 && !write_a) continue;
-      const int j = s == 0 ? ja : jb;
-      const double d = L.d[s][pad_idx<M>(ii)];
-      if (f == 0) {
-        if (a.q_t) str(a.q_t + (size_t)(j - 1) * n, ii, d);
-        else put_c<2>(g, a.out_t, i, j, old[s][k] + d);
-      } else if (f == 1) {
-        if (a.q_u) str(a.q_u + (size_t)(j - 1) * n, ii, d);
-        else put_ux(g, a.out_u, i, j, old[s][k] + d);
-      } else {
-        if (a.q_v) str(a.q_v + (size_t)(j - 1) * n, ii, d);
-        else put_c<0>(g, a.out_v, i, j, old[s][k] + d);
-      }
-    }
-  }
+  // ---- final values with boundary values, or the increments for a y
+  // solve, in the lane order of the stencil phase
+  pred_emit<0, F, NT, M>(a, L, own, ja, jb, write_a, n, i0, nk);
+}
+
+// Workgroup w: field w % nfld (+1 without the temperature equation) of rows
+// a = 2q+1 and b = a+1, q = w / nfld.  For odd jtot the last pair is shifted
+// down to (jtot-1, jtot) and writes row b only.  Consecutive w share an XCD.
+template <int NT, int M, class S>
+__global__ void __launch_bounds__(NT) k_pred_rows(S src, int nfld) {
+  const PredArgs a = src.get();
+  extern __shared__ double lds_raw[];
+  PredRowsLds<NT, M> &L = *reinterpret_cast<PredRowsLds<NT, M> *>(lds_raw);
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  const int q = w / nfld;
+  const int f = w % nfld + (nfld == 3 ? 0 : 1);
+  const int ja = min(2 * q + 1, a.g.jtot - 1), jb = ja + 1;
+  const bool write_a = ja == 2 * q + 1;
+  if (f == 0) pred_rows_body<0, NT, M>(a, L, ja, jb, write_a);
+  else if (f == 1) pred_rows_body<1, NT, M>(a, L, ja, jb, write_a);
+  else pred_rows_body<2, NT, M>(a, L, ja, jb, write_a);
 }
 
 // ---------------------------------------------------------------- launch

@@ -258,6 +258,7 @@ STEP_CASES = [
     (40, 45, 3.0, 1e8, 1.0, 1, 0, 5, 1e-2, 30),     # odd jtot
     (600, 16, 3.0, 1e9, 1.0, 1, 0, 9, 1e-2, 5),     # wide rows, strongly stretched
     (4200, 8, 100.0, 1e11, 0.7, 1, 0, 9, 1e-2, 5),
+    (512, 64, 512.0, 1e10, 1.0, 1, 0, 6, 1e-2, 10),  # uniform grid
 ]
 
 
@@ -326,6 +327,8 @@ FUSED_CASES = [
     (1100, 12, 100.0, 1e10, 1.0, 1, 0),
     (2100, 8, 3.0, 1e10, 0.7, 1, 0),
     (4200, 8, 100.0, 1e11, 0.7, 1, 0),
+    (512, 32, 512.0, 1e9, 1.0, 1, 0),   # uniform grid: interior waves skip the column tables
+    (4096, 8, 4096.0, 1e11, 0.7, 1, 0),
 ]
 
 
@@ -474,4 +477,14 @@ def test_statistics_collect_bitwise(sns, itot, jtot, state):
     for n in inst.STAT_NAMES:
         got = inst.download(n)
         assert np.array_equal(got.view(np.uint64), ref[n].view(np.uint64)), n
+    inst.close()
+
+
+@pytest.mark.parametrize("itot,stretch,uniform", [(512, 512.0, True), (512, 3.0, False),
+                                                 (1000, 1000.0, False), (64, 64.0, True)])
+def test_uniform_grid_detection(sns, itot, stretch, uniform):
+    """The column tables count as uniform only when every interior entry is
+    bitwise equal (a power-of-two itot on lx = 1 is; 1000 cells are not)."""
+    inst = sns.Instance(itot, 16, stretch=stretch)
+    assert inst.uniform_grid() == uniform
     inst.close()
