@@ -233,6 +233,13 @@ int ctx_configure(snsg_ctx *c, const snsg_config *cfg, const double *dxf, const 
   for (int i = 2; i <= itot; i++) dx = fmin(dx, DXC(i));
   c->dt_dif_x = 0.25 / diffusivity * pow(dx, 2.) * 0.9;
   c->dt_dif_y = 0.25 / diffusivity * pow(dy, 2.) * 0.9;
+  {
+    std::vector<double> plu(2 * (size_t)poisson_factor_entries(itot));
+    poisson_lu_coefs(itot, t[12], t[13], plu.data());
+    HIPCHECK(hipMemcpyAsync(c->d_plu, plu.data(), sizeof(double) * plu.size(),
+                            hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  }
   launch_poisson_factor(c->stream, c->geo, c->tab, c->d_eigen, c->d_pfac);
   hipLaunchKernelGGL(k_set_f64, dim3(1), dim3(1), 0, c->stream, c->d_time, 0.);
   hipLaunchKernelGGL(k_set_f64, dim3(1), dim3(1), 0, c->stream, c->d_dt, 0.);
@@ -261,7 +268,7 @@ extern "C" int snsg_create(int device, const snsg_config *cfg, const double *dxf
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(set_err(SNSG_E_ARG, "hipStreamCreate"));
   const size_t stride = (size_t)itot + 2;
-  const size_t nfac = (size_t)(jtot / 2 + 1) * 3 * poisson_factor_entries(itot);
+  const size_t nfac = poisson_factor_doubles(itot, jtot);
   struct {
     void **p;
     size_t bytes;
@@ -280,6 +287,7 @@ extern "C" int snsg_create(int device, const snsg_config *cfg, const double *dxf
       {(void **)&c->d_tab, sizeof(double) * kTables * stride},
       {(void **)&c->d_tab2, sizeof(double2) * kTables2 * stride},
       {(void **)&c->d_pfac, sizeof(double) * nfac},
+      {(void **)&c->d_plu, sizeof(double) * 2 * poisson_factor_entries(itot)},
   };
   for (auto &b : bufs)
     if ((rc = ctx_alloc(b.p, b.bytes, &c->mem))) return fail(rc);
@@ -297,7 +305,7 @@ extern "C" int snsg_destroy(snsg_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *bufs[] = {c->d_dxf, c->d_dxc, c->d_dt, c->d_eigen, c->d_tw, c->d_psihat, c->d_Y, c->d_q,
                   c->d_qy, c->d_red, c->d_tab, c->d_time, c->d_qf[0], c->d_qf[1],
-                  c->d_qf[2], c->d_lu, c->d_part, c->d_pfac, c->d_tab2, c->d_diag};
+                  c->d_qf[2], c->d_lu, c->d_part, c->d_pfac, c->d_tab2, c->d_diag, c->d_plu};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (c->h_red) hipHostFree(c->h_red);
@@ -416,6 +424,7 @@ PoisArgs pois_args(const snsg_ctx *c, const Geo &g, double gamma, const double *
   a.uy = uy;
   a.tw = c->d_tw;
   a.fac = c->d_pfac;
+  a.plu = c->d_plu;
   a.Y = c->d_Y;
   a.psihat = c->d_psihat;
   a.psi = psi;
@@ -662,7 +671,7 @@ extern "C" int snsg_instance_bytes(const snsg_config *cfg, size_t *bytes) {
   n += z * itot * (jtot / 2 + 1) + z * (size_t)pl.npair * jtot;
   n += 2 * d * (itot + 1) * jtot + sizeof(unsigned long long);
   n += d * kTables * stride + z * kTables2 * stride;
-  n += d * (jtot / 2 + 1) * 3 * (size_t)poisson_factor_entries((int)itot);
+  n += d * poisson_factor_doubles((int)itot, (int)jtot) + d * 2 * (size_t)poisson_factor_entries((int)itot);
   // snsg_state_alloc
   n += 2 * d * pu * (jtot + 2) + 6 * d * (pc * (jtot + 2) + 16);
   n += d * (itot - 1) * jtot + 2 * d * itot * jtot;

@@ -23,6 +23,7 @@ struct snsg_ctx {
   double *d_dt = nullptr;
   double *d_eigen = nullptr;
   double *d_pfac = nullptr;             // Thomas factors of the Poisson x systems
+  double *d_plu = nullptr;              // their shared row coefficients l, u (interleaved)
   double2 *d_tw = nullptr;
   double2 *d_psihat = nullptr;
   double *d_q = nullptr, *d_qy = nullptr;

@@ -24,6 +24,30 @@ namespace sns {
 
 struct LuTab {
   const double *r, *al, *be;  // each NT*M entries, interleaved
+  // entry k of this thread's chunk (interleaved index k*NT + t)
+  __device__ __forceinline__ double get_r(int k, int nt) const { return r[k * nt + threadIdx.x]; }
+  __device__ __forceinline__ double get_al(int k, int nt) const { return al[k * nt + threadIdx.x]; }
+  __device__ __forceinline__ double get_be(int k, int nt) const { return be[k * nt + threadIdx.x]; }
+};
+
+// The same factors when al_i = -(r_i*l_i) and be_i = -(r_i*u_i) hold for
+// 0 < i < n-1 with row coefficients l, u that do not depend on the system
+// (only r does): l and u are interleaved tables like r.  Row 0 has al = 0 and
+// be = -up0 (the first pivot's u'_0 = u_0/c_0), the last row be = 0.
+struct LuTabLU {
+  const double *r, *l, *u;
+  int n, m;    // unknowns; entries per thread
+  double up0;  // u'_0 of this system
+  __device__ __forceinline__ double get_r(int k, int nt) const { return r[k * nt + threadIdx.x]; }
+  __device__ __forceinline__ double get_al(int k, int nt) const {
+    const int row = threadIdx.x * m + k;
+    return row == 0 ? 0. : -(r[k * nt + threadIdx.x] * l[k * nt + threadIdx.x]);
+  }
+  __device__ __forceinline__ double get_be(int k, int nt) const {
+    const int row = threadIdx.x * m + k;
+    return row == n - 1 ? 0.
+                        : (row == 0 ? -up0 : -(r[k * nt + threadIdx.x] * u[k * nt + threadIdx.x]));
+  }
 };
 
 // composition of affine maps x -> A x + B: (A2,B2) after (A1,B1)
@@ -80,10 +104,9 @@ __device__ __forceinline__ void affine_scan_excl(double (&A)[NF], double (&B)[NF
 
 // Solve NS systems that share one matrix (tab) for this thread's chunk
 // d[s][0..M-1] (unknowns t*M .. t*M+M-1).
-template <int NT, int M, int NS>
-__device__ __forceinline__ void lu_solve_shared(double (&d)[NS][M], const LuTab &tab,
+template <int NT, int M, int NS, class TAB = LuTab>
+__device__ __forceinline__ void lu_solve_shared(double (&d)[NS][M], const TAB &tab,
                                                 double *scr) {
-  const int t = threadIdx.x;
   double A[NS], B[NS], C[NS];
   double P[M];
   {
@@ -93,7 +116,7 @@ __device__ __forceinline__ void lu_solve_shared(double (&d)[NS][M], const LuTab 
     double pr = 1.;
 #pragma unroll
     for (int k = 0; k < M; k++) {
-      const double r = tab.r[k * NT + t], al = tab.al[k * NT + t];
+      const double r = tab.get_r(k, NT), al = tab.get_al(k, NT);
       pr *= al;
       P[k] = pr;
 #pragma unroll
@@ -117,7 +140,7 @@ __device__ __forceinline__ void lu_solve_shared(double (&d)[NS][M], const LuTab 
     double pr = 1.;
 #pragma unroll
     for (int k = M - 1; k >= 0; k--) {
-      const double be = tab.be[k * NT + t];
+      const double be = tab.get_be(k, NT);
       pr *= be;
       Q[k] = pr;
 #pragma unroll

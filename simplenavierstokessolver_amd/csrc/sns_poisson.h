@@ -17,9 +17,12 @@ struct Fft2Plan {
 bool fft2_plan(int n, int itot, Fft2Plan &pl);
 
 // Thomas factors of the x systems of all wavenumbers (once per instance):
-// fac holds (jtot/2+1) groups of 3 tables of poisson_factor_entries(itot)
-// entries (sns_lu.h layout)
+// fac holds (jtot/2+1) tables of poisson_factor_entries(itot) reciprocal
+// pivots (sns_lu.h layout), then u'_0 of each wavenumber; plu the shared
+// row coefficients l and u in the same layout (poisson_lu_coefs, 2 tables)
 int poisson_factor_entries(int itot);
+size_t poisson_factor_doubles(int itot, int jtot);
+void poisson_lu_coefs(int itot, const double *pl, const double *pu, double *plu);
 void launch_poisson_factor(hipStream_t s, const Geo &g, const Tables &tb, const double *eigen,
                            double *fac);
 
@@ -34,7 +37,7 @@ struct PoisArgs {
   const double *dtp;
   const double *ux, *uy;
   const double2 *tw;
-  const double *fac;
+  const double *fac, *plu;
   double2 *Y, *psihat;
   double *psi;
 };

@@ -222,23 +222,27 @@ __device__ __forceinline__ void pinv_a(const Geo &g, const Fft2Plan &pl,
 // the Neumann walls folded into c_0 and c_{n-1}.  It does not change during
 // a run, so its Thomas factorisation (src/linalg.c:32-50: u'_i and the
 // reciprocal pivots, in the reference's sequential order) is computed once
-// per instance by k_poisson_factor, one thread per wavenumber, and stored as
-// the interleaved tables of sns_lu.h (r = 1/pivot, al = -r l, be = -u').
+// per instance by k_poisson_factor, one thread per wavenumber.  Only the
+// reciprocal pivots r_i depend on J; the substitutions need al_i = -(r_i l_i)
+// and be_i = -u'_i = -(r_i u_i) for 0 < i < n-1, so the factor table holds r
+// (interleaved as in sns_lu.h, ntm entries per J) followed by u'_0 = u_0/c_0
+// of every J, and l, u are shared interleaved tables (LuTabLU).
 // The J = 0 system is singular; its last equation is replaced by
-// psi_hat(itot) = 0 (r = al = 0 on that row).  Each solve then needs only
-// the two substitutions, evaluated by lu_solve_shared for the real and
-// imaginary parts together; with the factors exact, the substitutions keep
-// the accuracy of the sequential sweep on the nearly singular low-J systems.
+// psi_hat(itot) = 0 (r = 0 on that row).  Each solve then needs only the two
+// substitutions, evaluated by lu_solve_shared for the real and imaginary
+// parts together; with the factors exact, the substitutions keep the
+// accuracy of the sequential sweep on the nearly singular low-J systems.
 __device__ __forceinline__ int lu_slot(int i, int NT, int M) { return (i % M) * NT + i / M; }
 
 __global__ void __launch_bounds__(64)
     k_poisson_factor(Geo g, Tables tb, const double *__restrict__ eigen, int NT, int M, int ntm,
                      double *__restrict__ fac) {
   const int J = blockIdx.x * blockDim.x + threadIdx.x;
-  if (J > g.jtot / 2) return;
+  const int nJ = g.jtot / 2 + 1;
+  if (J >= nJ) return;
   const int n = g.itot;
   const double lam = eigen[J];
-  double *R = fac + (size_t)J * 3 * ntm, *AL = R + ntm, *BE = R + 2 * ntm;
+  double *R = fac + (size_t)J * ntm;
   auto coef = [&](int i, double &l, double &c, double &u) {
     l = tb.pl()[i];
     u = tb.pu()[i];
@@ -251,9 +255,8 @@ __global__ void __launch_bounds__(64)
   double up = u / c;  // u'_0
   double r = 1. / c;
   R[lu_slot(0, NT, M)] = r;
-  AL[lu_slot(0, NT, M)] = 0.;
+  fac[(size_t)nJ * ntm + J] = up;
   for (int i = 1; i < n; i++) {
-    BE[lu_slot(i - 1, NT, M)] = -up;
     coef(i, l, c, u);
     const double val = c - l * up;
     if (fabs(val) < DBL_EPSILON || (J == 0 && i == n - 1)) {
@@ -264,9 +267,7 @@ __global__ void __launch_bounds__(64)
       up = r * u;
     }
     R[lu_slot(i, NT, M)] = r;
-    AL[lu_slot(i, NT, M)] = -(r * l);
   }
-  BE[lu_slot(n - 1, NT, M)] = 0.;
 }
 
 template <int NT, int M>
@@ -278,7 +279,8 @@ struct PxLds {
 // One workgroup per wavenumber row; the row is staged through LDS (coalesced
 // global access, chunk order for the substitutions).
 template <int NT, int M>
-__device__ __forceinline__ void poisson_x3(const Geo &g, const double *__restrict__ fac, int ntm,
+__device__ __forceinline__ void poisson_x3(const Geo &g, const double *__restrict__ fac,
+                                           const double *__restrict__ plu, int ntm,
                                            double2 *__restrict__ psihat) {
   extern __shared__ double lds_raw[];
   PxLds<NT, M> &L = *reinterpret_cast<PxLds<NT, M> *>(lds_raw);
@@ -295,8 +297,8 @@ __device__ __forceinline__ void poisson_x3(const Geo &g, const double *__restric
     d[0][k] = v.x;
     d[1][k] = v.y;
   }
-  const double *R = fac + (size_t)J * 3 * ntm;
-  const LuTab tab = {R, R + ntm, R + 2 * ntm};
+  const int nJ = g.jtot / 2 + 1;
+  const LuTabLU tab = {fac + (size_t)J * ntm, plu, plu + ntm, n, M, fac[(size_t)nJ * ntm + J]};
   lu_solve_shared<NT, M, 2>(d, tab, L.scr);
 #pragma unroll
   for (int k = 0; k < M; k++) {
@@ -326,7 +328,7 @@ __global__ void __launch_bounds__(NTF) k_pfwd_b(S src) {
 template <int NT, int M, class S>
 __global__ void __launch_bounds__(NT) k_poisson_x3(S src) {
   const PoisArgs a = src.get();
-  poisson_x3<NT, M>(a.g, a.fac, NT * M, a.psihat);
+  poisson_x3<NT, M>(a.g, a.fac, a.plu, NT * M, a.psihat);
 }
 
 template <class S>
@@ -352,6 +354,20 @@ static int px_threads(int n) {
 }
 
 int poisson_factor_entries(int itot) { return px_threads(itot) * kPxM; }
+
+size_t poisson_factor_doubles(int itot, int jtot) {
+  return (size_t)(jtot / 2 + 1) * (poisson_factor_entries(itot) + 1);
+}
+
+void poisson_lu_coefs(int itot, const double *pl, const double *pu, double *plu) {
+  const int NT = px_threads(itot), ntm = NT * kPxM;
+  for (int k = 0; k < 2 * ntm; k++) plu[k] = 0.;
+  for (int i = 0; i < itot; i++) {
+    const int slot = (i % kPxM) * NT + i / kPxM;
+    plu[slot] = pl[i];
+    plu[ntm + slot] = pu[i];
+  }
+}
 
 void launch_poisson_factor(hipStream_t s, const Geo &g, const Tables &tb, const double *eigen,
                            double *fac) {
