@@ -29,6 +29,20 @@ __device__ __forceinline__ double2 cmul_si(double2 a) {
   return SIGN < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
 }
 
+// Quotients of small non-negative integers by a divisor fixed for a loop:
+// for 0 <= x < 2^16 and 1 <= d < 2^16, floor(x * ceil(2^32/d) / 2^32) =
+// floor(x/d) (the error term x*(ceil(2^32/d) - 2^32/d)/2^32 stays below
+// 2^-16, less than the distance 1/d of x/d to the next integer).
+struct FastDiv {
+  unsigned m;
+  int d;
+  __host__ __device__ explicit FastDiv(int dv)
+      : m(dv == 1 ? 0u : (unsigned)(0xFFFFFFFFull / (unsigned)dv + 1)), d(dv) {}
+  __device__ __forceinline__ int div(int x) const {
+    return d == 1 ? x : (int)__umulhi((unsigned)x, m);
+  }
+};
+
 template <int R, int SIGN>
 struct Dft;
 
@@ -124,12 +138,13 @@ __device__ __noinline__ void fft_pass_il(double2 *buf, int S, int n, int Ns,
   const int nb = n / R;
   const int total = S * nb;
   const int tstep = (n / (Ns * R)) * twmul;
+  const FastDiv fs(S), fns(Ns);
   double2 x[BMAX][R];
 #pragma unroll
   for (int k = 0; k < BMAX; k++) {
     const int bidx = threadIdx.x + k * NT;
     if (bidx < total) {
-      const int j = bidx / S, s = bidx - j * S;
+      const int j = fs.div(bidx), s = bidx - j * S;
 #pragma unroll
       for (int r = 0; r < R; r++) x[k][r] = buf[(j + r * nb) * S + s];
     }
@@ -139,8 +154,8 @@ __device__ __noinline__ void fft_pass_il(double2 *buf, int S, int n, int Ns,
   for (int k = 0; k < BMAX; k++) {
     const int bidx = threadIdx.x + k * NT;
     if (bidx < total) {
-      const int j = bidx / S, s = bidx - j * S;
-      const int kk = j % Ns;
+      const int j = fs.div(bidx), s = bidx - j * S;
+      const int kk = j - fns.div(j) * Ns;
       if (kk != 0) {
 #pragma unroll
         for (int r = 1; r < R; r++) {

@@ -60,9 +60,10 @@ __device__ __forceinline__ void pfwd_a(const Geo &g, const Tables &tb, const Fft
   __shared__ double2 buf[EMAXF * NTF];
   const int tile = blockIdx.x, j20 = blockIdx.y * pl.G;
   const int S = pl.G * TILE, n1 = pl.n1, n2 = pl.n2;
+  const FastDiv fS(S);
   const double inv_gdt = 1. / (gamma * (*dtp));
   for (int e = threadIdx.x; e < S * n1; e += NTF) {
-    const int j1 = e / S, s = e - j1 * S;
+    const int j1 = fS.div(e), s = e - j1 * S;
     const int gg = s / TILE, q = s - gg * TILE;
     const int j = n2 * j1 + j20 + gg;  // 0-based row
     const int ia = 2 * (tile * TILE + q) + 1;
@@ -74,7 +75,7 @@ __device__ __forceinline__ void pfwd_a(const Geo &g, const Tables &tb, const Fft
   __syncthreads();
   fft_il<NTF, EMAXF, -1>(buf, S, n1, pl.np1, pl.code1, tw, pl.n / n1);
   for (int e = threadIdx.x; e < S * n1; e += NTF) {
-    const int k1 = e / S, s = e - k1 * S;
+    const int k1 = fS.div(e), s = e - k1 * S;
     const int gg = s / TILE, q = s - gg * TILE;
     const int j2 = j20 + gg;
     const int pair = tile * TILE + q;
@@ -98,8 +99,9 @@ __device__ __forceinline__ void pfwd_b(const Geo &g, const Fft2Plan &pl,
   const int k1a = blockIdx.y, k1b = (n1 - k1a) % n1;
   const int nh = (k1a == k1b) ? 1 : 2;
   const int S = nh * TILE;
+  const FastDiv fS(S);
   for (int e = threadIdx.x; e < S * n2; e += NTF) {
-    const int j2 = e / S, s = e - j2 * S;
+    const int j2 = fS.div(e), s = e - j2 * S;
     const int h = s / TILE, q = s - h * TILE;
     const int pair = tile * TILE + q;
     const int k1 = h ? k1b : k1a;
@@ -110,7 +112,7 @@ __device__ __forceinline__ void pfwd_b(const Geo &g, const Fft2Plan &pl,
   // separate the two real columns of each pair: X_a = (Z_k + conj Z_{n-k})/2,
   // X_b = (Z_k - conj Z_{n-k})/(2i), for the wavenumbers k <= n/2 of this group
   for (int e = threadIdx.x; e < S * n2; e += NTF) {
-    const int k2 = e / S, s = e - k2 * S;
+    const int k2 = fS.div(e), s = e - k2 * S;
     const int h = s / TILE, q = s - h * TILE;
     const int k1 = h ? k1b : k1a;
     const int k = k1 + n1 * k2;
@@ -150,10 +152,11 @@ __device__ __forceinline__ void pinv_b(const Geo &g, const Fft2Plan &pl,
   const int k1a = blockIdx.y, k1b = (n1 - k1a) % n1;
   const int nh = (k1a == k1b) ? 1 : 2;
   const int S = nh * TILE;
+  const FastDiv fS(S);
   // Hermitian extension of the two real columns' spectra packed as
   // Z_k = X_a,k + i X_b,k (src/fluid/compute_potential.c:147-149: c2r input)
   for (int e = threadIdx.x; e < S * n2; e += NTF) {
-    const int k2 = e / S, s = e - k2 * S;
+    const int k2 = fS.div(e), s = e - k2 * S;
     const int h = s / TILE, q = s - h * TILE;
     const int k1 = h ? k1b : k1a;
     const int k = k1 + n1 * k2;
@@ -170,7 +173,7 @@ __device__ __forceinline__ void pinv_b(const Geo &g, const Fft2Plan &pl,
   __syncthreads();
   fft_il<NTF, EMAXF, +1>(buf, S, n2, pl.np2, pl.code2, tw, n / n2);
   for (int e = threadIdx.x; e < S * n2; e += NTF) {
-    const int j2 = e / S, s = e - j2 * S;
+    const int j2 = fS.div(e), s = e - j2 * S;
     const int h = s / TILE, q = s - h * TILE;
     const int k1 = h ? k1b : k1a;
     const int pair = tile * TILE + q;
@@ -194,8 +197,9 @@ __device__ __forceinline__ void pinv_a(const Geo &g, const Fft2Plan &pl,
   __shared__ double2 buf[EMAXF * NTF];
   const int tile = blockIdx.x, j20 = blockIdx.y * pl.G;
   const int S = pl.G * TILE, n1 = pl.n1, n2 = pl.n2;
+  const FastDiv fS(S);
   for (int e = threadIdx.x; e < S * n1; e += NTF) {
-    const int k1 = e / S, s = e - k1 * S;
+    const int k1 = fS.div(e), s = e - k1 * S;
     const int gg = s / TILE, q = s - gg * TILE;
     const int pair = tile * TILE + q;
     buf[e] = pair < pl.npair ? Y[(size_t)(k1 * n2 + j20 + gg) * pl.npair + pair]
@@ -205,7 +209,7 @@ __device__ __forceinline__ void pinv_a(const Geo &g, const Fft2Plan &pl,
   fft_il<NTF, EMAXF, +1>(buf, S, n1, pl.np1, pl.code1, tw, pl.n / n1);
   const double njtot = 1. * g.jtot;
   for (int e = threadIdx.x; e < S * n1; e += NTF) {
-    const int j1 = e / S, s = e - j1 * S;
+    const int j1 = fS.div(e), s = e - j1 * S;
     const int gg = s / TILE, q = s - gg * TILE;
     const int j = n2 * j1 + j20 + gg + 1;  // 1-based row
     const int ia = 2 * (tile * TILE + q) + 1;
