@@ -28,6 +28,7 @@ GROUPS = [
     ("k_poisson_x3", "poisson_x"), ("k_poisson_factor", "setup"),
     ("k_pinv_b", "poisson_inv"), ("k_pinv_a", "poisson_inv"),
     ("k_corr", "corr"), ("k_dt_reduce", "dt_reduce"), ("k_fill_u64", "dt_reduce"),
+    ("k_adv_bound", "dt_reduce"),
     ("k_dt", "dt"), ("k_lu_factor", "lu_factor"), ("k_copy", "copy"),
     ("k_transpose", "ysolve"), ("k_cny_rows", "ysolve"), ("k_add_out", "ysolve"),
 ]
