@@ -21,7 +21,7 @@ OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "lib")
 
 HIP_SOURCES = ["sns_stencil.hip", "sns_solve.hip", "sns_poisson.hip", "sns_fused.hip", "sns_pred.hip",
-               "sns_diag.hip", "sns_capi.hip", "sns_batch.hip"]
+               "sns_diag.hip", "sns_capi.hip", "sns_batch.hip", "sns_ysolve.hip"]
 HOST_SOURCES = ["common.c", "parallel.c", "param.c", "fluid.c", "temperature.c", "fileio.c",
                 "save.c", "logging.c", "statistics.c"]
 

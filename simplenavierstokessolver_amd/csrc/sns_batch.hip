@@ -41,6 +41,7 @@ struct Lane {
   char *d_tab = nullptr;
   size_t tab_cap = 0;
   LuFactorArgs *lu = nullptr;
+  YFactorArgs *yf = nullptr;
   DtArgs *dt = nullptr;
   CorrArgs *bound = nullptr;
   PredArgs *pred[2][3] = {};
@@ -115,6 +116,7 @@ int lane_build(Lane &k) {
   for (size_t m = 0; m < nz; m++) k.temp0[m] = k.mem[m].st->temp;
   k.shape = sa[0][0];
   const size_t o_lu = reserve(nz * sizeof(LuFactorArgs));
+  const size_t o_yf = reserve(nz * sizeof(YFactorArgs));
   const size_t o_dt = reserve(nz * sizeof(DtArgs));
   const size_t o_bound = reserve(nz * sizeof(CorrArgs));
   size_t o_pred[2][3], o_y[2][3][3], o_pois[3], o_corr[3];
@@ -132,6 +134,7 @@ int lane_build(Lane &k) {
   };
   for (size_t m = 0; m < nz; m++) {
     put(o_lu, m, &sa[0][m].lu, sizeof(LuFactorArgs));
+    put(o_yf, m, &sa[0][m].yf, sizeof(YFactorArgs));
     put(o_dt, m, &sa[0][m].dt, sizeof(DtArgs));
     put(o_bound, m, &sa[0][m].bound, sizeof(CorrArgs));
     for (int p = 0; p < 2; p++)
@@ -157,6 +160,7 @@ int lane_build(Lane &k) {
   HIPCHECK(hipStreamSynchronize(k.stream));
   char *d = k.d_tab;
   k.lu = (LuFactorArgs *)(d + o_lu);
+  k.yf = (YFactorArgs *)(d + o_yf);
   k.dt = (DtArgs *)(d + o_dt);
   k.bound = (CorrArgs *)(d + o_bound);
   for (int p = 0; p < 2; p++)
@@ -179,12 +183,14 @@ void lane_launch(const Lane &k, int p) {
   const StepArgs &sh = k.shape;
   launch_dt_batch(s, k.dt, nz);
   if (k.impx) launch_lu_factor_batch(s, sh.lu, k.lu, nz);
+  if (k.impy && sh.yf.L) launch_yfactor_batch(s, k.yf, nz);
   for (int rk = 0; rk < 3; rk++) {
     launch_pred_batch(s, sh.pred[rk], k.impx, k.pred[p][rk], nz);
     if (k.impy)
       for (int f = 0; f < 3; f++) {
         if (f == 0 && !k.with_temp) continue;
-        launch_ysolve_batch(s, kYKinds[f], sh.y[rk][f], k.y[p][rk][f], nz);
+        if (sh.y[rk][f].L) launch_ypart_batch(s, kYKinds[f], sh.y[rk][f], k.y[p][rk][f], nz);
+        else launch_ysolve_batch(s, kYKinds[f], sh.y[rk][f], k.y[p][rk][f], nz);
       }
     for (int stage = 0; stage < 3; stage++)
       launch_poisson_batch(s, sh.pois[rk], k.pois[rk], nz, stage);

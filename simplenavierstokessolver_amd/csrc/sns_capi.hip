@@ -305,7 +305,8 @@ extern "C" int snsg_destroy(snsg_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *bufs[] = {c->d_dxf, c->d_dxc, c->d_dt, c->d_eigen, c->d_tw, c->d_psihat, c->d_Y, c->d_q,
                   c->d_qy, c->d_red, c->d_tab, c->d_time, c->d_qf[0], c->d_qf[1],
-                  c->d_qf[2], c->d_lu, c->d_part, c->d_pfac, c->d_tab2, c->d_diag, c->d_plu};
+                  c->d_qf[2], c->d_lu, c->d_part, c->d_pfac, c->d_tab2, c->d_diag, c->d_plu,
+                  c->d_yfac, c->d_qz};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (c->h_red) hipHostFree(c->h_red);
@@ -676,7 +677,11 @@ extern "C" int snsg_instance_bytes(const snsg_config *cfg, size_t *bytes) {
   n += 2 * d * pu * (jtot + 2) + 6 * d * (pc * (jtot + 2) + 16);
   n += d * (itot - 1) * jtot + 2 * d * itot * jtot;
   // first fused step
-  if (cfg->implicity) n += d * (2 * itot + itot - 1) * jtot;
+  if (cfg->implicity) {
+    n += d * (2 * itot + itot - 1) * jtot;
+    const int L = ysolve_chunk((int)jtot);
+    if (L) n += d * 9 * ysolve_table_doubles(L) + d * (jtot / L) * itot;
+  }
   if (cfg->implicitx) n += d * 27 * (size_t)lu_table_entries((int)itot);
   Geo g{};
   g.itot = (int)itot;
@@ -817,6 +822,13 @@ int step_prepare(snsg_ctx *c) {
     const size_t nb[3] = {(size_t)itot * jtot, (size_t)(itot - 1) * jtot, (size_t)itot * jtot};
     for (int k = 0; k < 3; k++)
       if ((rc = ctx_alloc((void **)&c->d_qf[k], sizeof(double) * nb[k], &c->mem))) return rc;
+    c->ychunk = ysolve_chunk(jtot);
+    if (c->ychunk) {
+      const size_t nt = 9 * ysolve_table_doubles(c->ychunk);
+      const size_t nz = (size_t)(jtot / c->ychunk) * itot;
+      if ((rc = ctx_alloc((void **)&c->d_yfac, sizeof(double) * nt, &c->mem))) return rc;
+      if ((rc = ctx_alloc((void **)&c->d_qz, sizeof(double) * nz, &c->mem))) return rc;
+    }
   }
   if (c->treat.implicitx && !c->d_lu) {
     c->lu_ntm = lu_table_entries(itot);
@@ -855,6 +867,15 @@ void step_args(const snsg_ctx *c, const snsg_state *st, bool reduce, StepArgs &s
   sa.bound.ux = st->ux;
   sa.bound.uy = st->uy;
   sa.bound.red = c->d_part;
+  const int yL = impy ? c->ychunk : 0;
+  const size_t ystride = yL ? ysolve_table_doubles(yL) : 0;
+  sa.yf.g = g;
+  sa.yf.dtp = c->d_dt;
+  for (int k = 0; k < 3; k++) sa.yf.gamma[k] = c->cfg.gamma[k];
+  sa.yf.out = c->d_yfac;
+  sa.yf.L = yL;
+  sa.yf.S = yL ? g.jtot / yL : 0;
+  sa.yf.stride = (long)ystride;
   double *temp = st->temp, *temp_alt = st->temp_alt;
   for (int rk = 0; rk < 3; rk++) {
     PredArgs &pa = sa.pred[rk];
@@ -887,6 +908,10 @@ void step_args(const snsg_ctx *c, const snsg_state *st, bool reduce, StepArgs &s
       y.qf = c->d_qf[f];
       y.qy = c->d_qy;
       y.out = outs[f];
+      y.L = yL;
+      y.S = yL ? g.jtot / yL : 0;
+      y.yfac = yL ? c->d_yfac + (size_t)(rk * 3 + f) * ystride : nullptr;
+      y.qz = c->d_qz;
     }
     sa.pois[rk] = pois_args(c, g, c->cfg.gamma[rk], st->ux_star, st->uy_star, st->psi);
     CorrArgs &ca = sa.corr[rk];
@@ -913,12 +938,17 @@ static int integrate_fused(snsg_ctx *c, snsg_state *st, bool reduce) {
   const bool impx = c->treat.implicitx, impy = c->treat.implicity;
   const bool with_temp = c->cfg.with_temperature != 0;
   if (impx && (rc = run(c, "lu_factor", [&] { launch_lu_factor(c->stream, sa.lu); }))) return rc;
+  if (impy && sa.yf.L && (rc = run(c, "ysolve", [&] { launch_yfactor(c->stream, sa.yf); })))
+    return rc;
   for (int rk = 0; rk < 3; rk++) {
     if ((rc = run(c, "pred", [&] { launch_pred(c->stream, sa.pred[rk], impx); }))) return rc;
     if (impy) {
       for (int f = 0; f < 3; f++) {
         if (f == 0 && !with_temp) continue;
-        rc = run(c, "ysolve", [&] { launch_ysolve(c->stream, kYKinds[f], sa.y[rk][f]); });
+        rc = run(c, "ysolve", [&] {
+          if (sa.y[rk][f].L) launch_ypart(c->stream, kYKinds[f], sa.y[rk][f]);
+          else launch_ysolve(c->stream, kYKinds[f], sa.y[rk][f]);
+        });
         if (rc) return rc;
       }
     }

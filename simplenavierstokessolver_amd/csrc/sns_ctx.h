@@ -41,6 +41,10 @@ struct snsg_ctx {
   Tables tab{};
   // fused-step scratch: x-aligned increments of T, ux, uy when a y solve follows
   double *d_qf[3] = {nullptr, nullptr, nullptr};
+  // partitioned y solves: chunk length (0: transposed solves), factor
+  // tables of the 3 substages x 3 fields, interface values
+  int ychunk = 0;
+  double *d_yfac = nullptr, *d_qz = nullptr;
   // LU tables of the CN-x operators (3 substages x 3 fields x 5 tables)
   double *d_lu = nullptr;
   int lu_ntm = 0;
@@ -83,6 +87,7 @@ struct StepArgs {
   LuFactorArgs lu;
   DtArgs dt;
   CorrArgs bound;  // advective bound of the state as it is (d_part)
+  YFactorArgs yf;  // factors of the partitioned y solves (ychunk > 0)
   PredArgs pred[3];
   YArgs y[3][3];   // [substage][T, ux, uy]
   PoisArgs pois[3];

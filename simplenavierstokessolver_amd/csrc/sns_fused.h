@@ -55,6 +55,26 @@ struct LuFactorArgs {
   int ntm;
 };
 
+// Factors of the partitioned Crank-Nicolson y solves of all three substages
+// and fields (sns_ysolve.hip): table (s*3+f) at out + (s*3+f)*stride
+struct YFactorArgs {
+  Geo g;
+  const double *dtp;
+  double gamma[3];
+  double *out;
+  int L, S;
+  long stride;
+};
+// chunk length of the partitioned y solve for jtot rows (0: not applicable)
+int ysolve_chunk(int jtot);
+size_t ysolve_table_doubles(int L);
+void launch_yfactor(hipStream_t s, const YFactorArgs &a);
+void launch_yfactor_batch(hipStream_t s, const YFactorArgs *dev, int nz);
+// y solve of one field (kind) on the row-major increments a.qf, then
+// out = in + increment with boundary values (a.L > 0)
+void launch_ypart(hipStream_t s, int kind, const YArgs &a);
+void launch_ypart_batch(hipStream_t s, int kind, const YArgs &shape, const YArgs *dev, int nz);
+
 bool pred_impx_supported(int itot);
 int lu_table_entries(int itot);
 void launch_lu_factor(hipStream_t s, const LuFactorArgs &a);

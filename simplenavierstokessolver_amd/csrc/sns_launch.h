@@ -69,6 +69,12 @@ struct YArgs {
   const double *in;  // substage-start field
   double *qf, *qy;
   double *out;
+  // partitioned solve on the row-major increments (sns_ysolve.hip) when L > 0:
+  // S chunks of L rows, the factor table of this substage and field, and the
+  // interface values (S rows of n)
+  int L, S;
+  const double *yfac;
+  double *qz;
 };
 
 // ---- boundary values (src/fluid/boundary_conditions.c, src/temperature/boundary_conditions.c)

@@ -1,0 +1,295 @@
+// sns_ysolve.hip -- Crank-Nicolson in y on the row-major increments (gfx950).
+//
+// The implicit-y operator of one field (src/temperature/update.c:184-203,
+// src/fluid/update_velocity.c:302-316, 428-468) is, for every column, the
+// same periodic tridiagonal matrix with constant coefficients: l = u =
+// lambda = -pf/dy^2 off the diagonal, c = 1 + 2 pf/dy^2 on it.  The jtot rows
+// are cut into S chunks of L rows: an interface row sL and L-1 interior rows
+// sL+1 .. sL+L-1.
+//   * With the interface values p_s known, the interior rows of chunk s
+//     solve T y = q (T: the (L-1)-row Dirichlet block, the same for every
+//     chunk and column) corrected by the spikes of the two couplings:
+//       x = y - lambda p_s v - lambda p_{s+1} w,   T v = e_0, w = reverse(v)
+//     (T is symmetric and persymmetric, so T w = e_{L-2}).
+//   * Substituting the interior rows into the interface equations leaves a
+//     cyclic tridiagonal system in p with constant coefficients
+//       O p_{s-1} + D p_s + O p_{s+1} = q_sL - lambda (y_{s-1,L-2} + y_{s,0}),
+//       D = c - 2 lambda^2 v_0,  O = -lambda^2 v_{L-2},
+//     solved per column as the reference solves periodic systems
+//     (src/linalg.c:52-108): the (S-1)-row block with the right-hand side
+//     and with [-O, 0, .., 0, -O], then the last unknown, then the rest.
+// Three kernels, each a pass over the row-major increments: k_ylocal (the
+// interior solves, y in place), k_yreduced (the interface system, one wave
+// per column, into qz), k_yapply (the corrections and out = in + x with
+// boundary values).  The factors of T and of the reduced system depend on
+// dt; k_yfactor computes them once per step for the three substages and
+// fields.  Rounding differs from the reference's sequential sweep at the
+// ulp level.
+#include "sns_launch.h"
+#include "sns_fast.h"
+#include "sns_fused.h"
+#include "sns_lu.h"
+
+namespace sns {
+
+constexpr int kYMaxL = 32;  // chunk length bound
+constexpr int kYRedNT = 64;  // threads of the reduced solve (one wave)
+constexpr int kYRedM = 8;    // reduced unknowns per thread: S-1 <= 512
+constexpr int kYHead = 8;
+
+// Chunk length: the longest that still leaves at least 32 chunks (the
+// interior solves are sequential within a chunk, so short columns need more,
+// shorter chunks to fill the GPU), else the longest with at least 4.
+int ysolve_chunk(int jtot) {
+  for (int minS : {32, 4})
+    for (int L = kYMaxL; L >= 4; L--)
+      if (jtot % L == 0 && jtot / L >= minS && jtot / L - 1 <= kYRedNT * kYRedM) return L;
+  return 0;
+}
+
+size_t ysolve_table_doubles(int L) {
+  return kYHead + 3 * (size_t)(L - 1) + 4 * (size_t)(kYRedNT * kYRedM);
+}
+
+// Table of one (substage, field): header [lambda, c, D, O, den, 0, 0, 0],
+// the interior factors r, beta and the spike v (L-1 entries each), then the
+// reduced (S-1)-row block's interleaved LU tables r, al, be (sns_lu.h) and
+// its matrix-only solution q1 in the same layout.
+struct YTab {
+  const double *p;
+  int L1;
+  __device__ __forceinline__ double lambda() const { return p[0]; }
+  __device__ __forceinline__ double O() const { return p[3]; }
+  __device__ __forceinline__ double den() const { return p[4]; }
+  __device__ __forceinline__ const double *r() const { return p + kYHead; }
+  __device__ __forceinline__ const double *beta() const { return p + kYHead + L1; }
+  __device__ __forceinline__ const double *v() const { return p + kYHead + 2 * L1; }
+  __device__ __forceinline__ const double *red(int k) const {
+    return p + kYHead + 3 * L1 + (size_t)k * kYRedNT * kYRedM;
+  }
+};
+
+template <int KIND>
+__device__ __forceinline__ double cny_pf(const Geo &g, double gamma, double dt) {
+  // T:     (gamma*dt)/(2.*sqrt(Pr)*sqrt(Ra))   src/temperature/update.c:142
+  // ux,uy: (gamma*dt*sqrt(Pr))/(2.*sqrt(Ra))   src/fluid/update_velocity.c:254,399
+  return KIND == KIND_TEMP ? (gamma * dt) / g.two_sqrtPr_sqrtRa
+                           : (gamma * dt * g.sqrtPr) / g.two_sqrtRa;
+}
+
+// ------------------------------------------------------------- factors
+
+// Workgroup (substage blockIdx.x, field blockIdx.y); thread 0 runs the
+// sequential recurrences, the others clear the padding of the reduced tables.
+template <class S>
+__global__ void __launch_bounds__(64) k_yfactor(S src) {
+  const YFactorArgs a = src.get();
+  const int s = blockIdx.x, f = blockIdx.y;
+  const int L1 = a.L - 1, SR = a.S - 1;  // interior rows; rows of the reduced block
+  double *t = a.out + (size_t)(s * 3 + f) * a.stride;
+  double *rr = t + kYHead + 3 * L1, *al = rr + kYRedNT * kYRedM, *be = al + kYRedNT * kYRedM,
+         *q1 = be + kYRedNT * kYRedM;
+  for (int k = threadIdx.x; k < kYRedNT * kYRedM; k += 64) rr[k] = al[k] = be[k] = q1[k] = 0.;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double gam = s == 0 ? a.gamma[0] : s == 1 ? a.gamma[1] : a.gamma[2];
+  const double dt = *a.dtp;
+  const double pf = f == 0 ? cny_pf<KIND_TEMP>(a.g, gam, dt) : cny_pf<KIND_UX>(a.g, gam, dt);
+  // coefficients as src/temperature/update.c:188-197
+  const double coef_ym = 1. / a.g.dy / a.g.dy, coef_yp = 1. / a.g.dy / a.g.dy;
+  const double coef_y0 = -coef_ym - coef_yp;
+  const double lam = -pf * coef_ym;
+  const double c = 1. - pf * coef_y0;
+  // interior Dirichlet block: r_m = 1/pivot, beta_m = lambda r_m (u')
+  double *r = t + kYHead, *beta = r + L1, *v = beta + L1;
+  double prev_beta = 0.;
+  for (int m = 0; m < L1; m++) {
+    const double piv = m == 0 ? c : c - lam * prev_beta;
+    r[m] = 1. / piv;
+    beta[m] = lam * r[m];
+    prev_beta = beta[m];
+  }
+  // spike v = T^{-1} e_0
+  double y = 0.;
+  for (int m = 0; m < L1; m++) {
+    y = ((m == 0 ? 1. : 0.) - lam * y) * r[m];
+    v[m] = y;
+  }
+  for (int m = L1 - 2; m >= 0; m--) v[m] = v[m] - beta[m] * v[m + 1];
+  const double lam2 = lam * lam;
+  const double D = c - lam2 * v[0] - lam2 * v[0];
+  const double O = -(lam2 * v[L1 - 1]);
+  // reduced (S-1)-row block: interleaved factors (row k at lu_slot)
+  auto slot = [](int k) { return (k % kYRedM) * kYRedNT + k / kYRedM; };
+  double rp = 0., bp = 0.;
+  for (int k = 0; k < SR; k++) {
+    const double piv = k == 0 ? D : D - O * bp;
+    rp = 1. / piv;
+    bp = O * rp;
+    rr[slot(k)] = rp;
+    al[slot(k)] = k == 0 ? 0. : -(rp * O);
+    be[slot(k)] = k == SR - 1 ? 0. : -bp;
+  }
+  // q1: the block's solution for the right-hand side [-O, 0, .., 0, -O]
+  double yq = 0.;
+  for (int k = 0; k < SR; k++) {
+    const double rhs = (k == 0 ? -O : 0.) + (k == SR - 1 ? -O : 0.);
+    yq = rr[slot(k)] * rhs + al[slot(k)] * yq;
+    q1[slot(k)] = yq;
+  }
+  for (int k = SR - 2; k >= 0; k--) q1[slot(k)] = q1[slot(k)] + be[slot(k)] * q1[slot(k + 1)];
+  t[0] = lam;
+  t[1] = c;
+  t[2] = D;
+  t[3] = O;
+  t[4] = D + O * q1[slot(0)] + O * q1[slot(SR - 1)];
+  t[5] = t[6] = t[7] = 0.;
+}
+
+// ------------------------------------------------------ interior solves
+
+// Wave w of the workgroup: chunk blockIdx.y*4 + w; lane: column.  The L-1
+// interior rows of the chunk are solved in registers and written back.
+template <class S>
+__global__ void __launch_bounds__(256) k_ylocal(S src) {
+  const YArgs a = src.get();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int s = blockIdx.y * 4 + wave, col = blockIdx.x * 64 + lane;
+  if (s >= a.S || col >= a.n) return;
+  const int L1 = a.L - 1;
+  const YTab tb{a.yfac, L1};
+  const double lam = tb.lambda();
+  double *q = a.qf + ((size_t)s * a.L + 1) * a.n + col;
+  double x[kYMaxL - 1];
+#pragma unroll
+  for (int m = 0; m < kYMaxL - 1; m++)
+    if (m < L1) x[m] = q[(size_t)m * a.n];
+  double y = 0.;
+#pragma unroll
+  for (int m = 0; m < kYMaxL - 1; m++)
+    if (m < L1) {
+      y = (x[m] - lam * y) * tb.r()[m];
+      x[m] = y;
+    }
+#pragma unroll
+  for (int m = kYMaxL - 3; m >= 0; m--)
+    if (m < L1 - 1) x[m] = x[m] - tb.beta()[m] * x[m + 1];
+#pragma unroll
+  for (int m = 0; m < kYMaxL - 1; m++)
+    if (m < L1) q[(size_t)m * a.n] = x[m];
+}
+
+// -------------------------------------------------- interface system
+
+// One wave per column blockIdx.x: the S interface values into qz (row k at
+// qz[k*n + col]).
+template <class S>
+__global__ void __launch_bounds__(kYRedNT) k_yreduced(S src) {
+  const YArgs a = src.get();
+  __shared__ double scr[2 * 2];
+  __shared__ double ends[3];
+  const int col = blockIdx.x;
+  const int n = a.n, L = a.L, SR = a.S - 1;
+  const YTab tb{a.yfac, L - 1};
+  const double lam = tb.lambda(), O = tb.O();
+  const double *q = a.qf + col;
+  // right-hand sides of the interface rows (chunk k: row kL; its neighbours
+  // are the last interior row of chunk k-1 and the first of chunk k)
+  auto rhs = [&](int k) {
+    const int rl = k == 0 ? a.S * L - 1 : k * L - 1;
+    return q[(size_t)k * L * n] - lam * (q[(size_t)rl * n] + q[((size_t)k * L + 1) * n]);
+  };
+  double d[1][kYRedM];
+#pragma unroll
+  for (int kk = 0; kk < kYRedM; kk++) {
+    const int k = threadIdx.x * kYRedM + kk;
+    d[0][kk] = k < SR ? rhs(k) : 0.;
+  }
+  const LuTab lt = {tb.red(0), tb.red(1), tb.red(2)};
+  lu_solve_shared<kYRedNT, kYRedM, 1>(d, lt, scr);
+  // the last unknown from the first and last rows of the block
+#pragma unroll
+  for (int kk = 0; kk < kYRedM; kk++) {
+    const int k = threadIdx.x * kYRedM + kk;
+    if (k == 0) ends[0] = d[0][kk];
+    if (k == SR - 1) ends[1] = d[0][kk];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) ends[2] = (rhs(SR) - O * ends[0] - O * ends[1]) / tb.den();
+  __syncthreads();
+  const double zl = ends[2];
+  double *z = a.qz + col;
+  const double *q1 = tb.red(3);
+#pragma unroll
+  for (int kk = 0; kk < kYRedM; kk++) {
+    const int k = threadIdx.x * kYRedM + kk;
+    if (k < SR) z[(size_t)k * n] = d[0][kk] + zl * q1[kk * kYRedNT + threadIdx.x];
+  }
+  if (threadIdx.x == 0) z[(size_t)SR * n] = zl;
+}
+
+// ------------------------------------------------------------- apply
+
+// Row blockIdx.y (j = row+1): x = p_s on an interface row, else the interior
+// solution with the spikes of both interfaces; out = in + x with boundary
+// values.
+template <int KIND, class S>
+__global__ void __launch_bounds__(256) k_yapply(S src) {
+  const YArgs a = src.get();
+  const Geo &g = a.g;
+  const int n = a.n;
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = blockIdx.y, j = row + 1;
+  if (ii >= n) return;
+  const int s = row / a.L, m = row - s * a.L;
+  const int s1 = s + 1 == a.S ? 0 : s + 1;
+  const YTab tb{a.yfac, a.L - 1};
+  const double zs = a.qz[(size_t)s * n + ii];
+  double x;
+  if (m == 0) {
+    x = zs;
+  } else {
+    const double lam = tb.lambda();
+    const double zn = a.qz[(size_t)s1 * n + ii];
+    const double y = a.qf[(size_t)row * n + ii];
+    x = y - lam * zs * tb.v()[m - 1] - lam * zn * tb.v()[a.L - 1 - m];
+  }
+  const int i = ii + (KIND == KIND_UX ? 2 : 1);
+  if (KIND == KIND_UX) put_ux(g, a.out, i, j, a.in[ux_idx(g, i, j)] + x);
+  else if (KIND == KIND_UY) put_c<0>(g, a.out, i, j, a.in[c_idx(g, i, j)] + x);
+  else put_c<2>(g, a.out, i, j, a.in[c_idx(g, i, j)] + x);
+}
+
+// ------------------------------------------------------------- launch
+
+template <class S>
+static void ypart(hipStream_t s, int kind, const YArgs &shape, S src, int nz) {
+  const int n = shape.n;
+  hipLaunchKernelGGL(k_ylocal<S>, dim3((n + 63) / 64, (shape.S + 3) / 4, nz), dim3(256), 0, s,
+                     src);
+  hipLaunchKernelGGL(k_yreduced<S>, dim3(n, 1, nz), dim3(kYRedNT), 0, s, src);
+  const dim3 gr((n + 255) / 256, shape.g.jtot, nz);
+  if (kind == KIND_UX) hipLaunchKernelGGL((k_yapply<KIND_UX, S>), gr, dim3(256), 0, s, src);
+  else if (kind == KIND_UY) hipLaunchKernelGGL((k_yapply<KIND_UY, S>), gr, dim3(256), 0, s, src);
+  else hipLaunchKernelGGL((k_yapply<KIND_TEMP, S>), gr, dim3(256), 0, s, src);
+}
+
+void launch_ypart(hipStream_t s, int kind, const YArgs &a) {
+  ypart(s, kind, a, ArgOne<YArgs>{a}, 1);
+}
+
+void launch_ypart_batch(hipStream_t s, int kind, const YArgs &shape, const YArgs *dev, int nz) {
+  ypart(s, kind, shape, arg_batch<YArgs>(dev), nz);
+}
+
+void launch_yfactor(hipStream_t s, const YFactorArgs &a) {
+  hipLaunchKernelGGL(k_yfactor<ArgOne<YFactorArgs>>, dim3(3, 3), dim3(64), 0, s,
+                     ArgOne<YFactorArgs>{a});
+}
+
+void launch_yfactor_batch(hipStream_t s, const YFactorArgs *dev, int nz) {
+  hipLaunchKernelGGL(k_yfactor<ArgBatch<YFactorArgs>>, dim3(3, 3, nz), dim3(64), 0, s,
+                     arg_batch<YFactorArgs>(dev));
+}
+
+}  // namespace sns
