@@ -73,6 +73,9 @@ def parse():
                     help="steps per job of the mixed ensemble (0: skip it)")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the served request stream (0: skip it)")
+    ap.add_argument("--serve-slots", type=int, default=0,
+                    help="most jobs on the GPU at once in the served stream (0: memory budget "
+                         "alone)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="instead of the step benchmark: time a plain device copy of this "
                          "many GiB (the known-size kernel that calibrates traffic counters)")
@@ -226,7 +229,7 @@ def ensemble_run(sns, steps, warmup=2):
                        "replayed HIP graph per class"}
 
 
-def served_run(sns, window):
+def served_run(sns, window, slots=0):
     """The served request stream for `window` seconds of arrivals: jobs are
     submitted at their arrival times to the serving engine (200 GB budget);
     throughput and per-step latency over the job-steps completed in the
@@ -235,7 +238,7 @@ def served_run(sns, window):
     jobs = ens.served_jobs(window)
     treatments(sns, ens, jobs)
     log("served: %d arrivals over %.0f s" % (len(jobs), window))
-    srv = sns.Server(int(ens.SERVED["budget_bytes"]))
+    srv = sns.Server(int(ens.SERVED["budget_bytes"]), max_active=slots)
     srv.reset_stats()
     t0 = time.perf_counter()
     k = 0
@@ -268,6 +271,7 @@ def served_run(sns, window):
             "p50_step_ms": st["p50_ms"], "p99_step_ms": st["p99_ms"], "max_step_ms": st["max_ms"],
             "submitted": st["submitted"], "admitted": st["admitted"],
             "completed": st["completed"], "job_steps": st["steps"], "lanes": st["lanes"],
+            "slots": slots, "active_peak": st["active_peak"],
             "mem_peak_GB": st["mem_peak"] / 1e9, "treatments": mix,
             "latency": "device duration of the lane step that advanced the job-step"}
 
@@ -349,7 +353,7 @@ def main():
     if a.ensemble_steps > 0:
         res["ensemble"] = ensemble_run(sns, a.ensemble_steps)
     if a.serve_seconds > 0:
-        res["served"] = served_run(sns, a.serve_seconds)
+        res["served"] = served_run(sns, a.serve_seconds, a.serve_slots)
     if not a.no_cpu_baseline:
         log("cpu baseline")
         res["cpu_baseline"] = cpu_baseline(itot, jtot, a.cpu_steps)

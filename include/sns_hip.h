@@ -208,8 +208,8 @@ int snsg_batch_classes(const snsg_batch *b, int *nclasses);
 int snsg_batch_destroy(snsg_batch *b);
 /* ---- serving engine (no counterpart in the reference).  Simulation jobs
  * of any size arrive from any thread (snsg_server_submit); an engine thread
- * admits them in arrival order while their device memory
- * (snsg_instance_bytes) fits the budget, keeps every admitted job's fields
+ * admits them in arrival order while a slot is free and their device
+ * memory (snsg_instance_bytes) fits the budget, keeps every admitted job's fields
  * resident and advances all admitted jobs continuously: the jobs of one shape
  * class form a lane (as in a batch) that starts its next step as soon as the
  * previous one is done, jobs joining and leaving it between steps.  Each
@@ -229,6 +229,7 @@ typedef struct {
 } snsg_job;
 typedef struct {
   long submitted, admitted, completed, active, queued;
+  long active_peak;          /* most jobs on the GPU at once */
   long steps;                /* job-steps completed since the last reset */
   double cell_steps;         /* their itot*jtot summed */
   double busy_s;             /* from the first admission (or the reset) to the last step */
@@ -240,7 +241,9 @@ typedef struct {
 #define SNSG_JOB_ACTIVE 1
 #define SNSG_JOB_DONE 2
 #define SNSG_JOB_RELEASED 3
-int snsg_server_create(int device, size_t mem_budget, snsg_server **out);
+/* mem_budget: device bytes of the admitted jobs' instances; max_active:
+ * at most this many jobs on the GPU at once (slots; 0: memory alone) */
+int snsg_server_create(int device, size_t mem_budget, int max_active, snsg_server **out);
 int snsg_server_submit(snsg_server *s, const snsg_job *job, int *id);
 int snsg_server_wait(snsg_server *s, int id, double timeout_s);
 int snsg_server_drain(snsg_server *s, double timeout_s);

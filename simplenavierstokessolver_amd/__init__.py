@@ -48,7 +48,7 @@ class SnsgJob(C.Structure):
 
 class SnsgServerStats(C.Structure):
     _fields_ = [(n, C.c_long) for n in ("submitted", "admitted", "completed", "active",
-                                        "queued", "steps")] + [
+                                        "queued", "active_peak", "steps")] + [
         ("cell_steps", C.c_double), ("busy_s", C.c_double), ("p50_ms", C.c_double),
         ("p99_ms", C.c_double), ("max_ms", C.c_double), ("mem_used", C.c_size_t),
         ("mem_peak", C.c_size_t), ("lanes", C.c_int)]
@@ -109,7 +109,7 @@ def _load_hip():
         L.snsg_instance_bytes.argtypes = [C.POINTER(SnsgConfig), C.POINTER(C.c_size_t)]
         L.snsg_memory.argtypes = [_vp, C.POINTER(C.c_size_t)]
         L.snsg_uniform_grid.argtypes = [_vp, C.POINTER(C.c_int)]
-        L.snsg_server_create.argtypes = [C.c_int, C.c_size_t, C.POINTER(_vp)]
+        L.snsg_server_create.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(_vp)]
         L.snsg_server_submit.argtypes = [_vp, C.POINTER(SnsgJob), C.POINTER(C.c_int)]
         L.snsg_server_wait.argtypes = [_vp, C.c_int, C.c_double]
         L.snsg_server_drain.argtypes = [_vp, C.c_double]
@@ -580,11 +580,11 @@ class Server:
     admitted against a device-memory budget and advanced continuously, one
     lane per shape class."""
 
-    def __init__(self, mem_budget, device=0):
+    def __init__(self, mem_budget, max_active=0, device=0):
         self.L = _load_hip()
         self.s = _vp()
-        _check(self.L.snsg_server_create(int(device), int(mem_budget), C.byref(self.s)),
-               "snsg_server_create")
+        _check(self.L.snsg_server_create(int(device), int(mem_budget), int(max_active),
+                                         C.byref(self.s)), "snsg_server_create")
         self.sizes = {}
 
     def submit(self, itot, jtot, nsteps, Ra=1e8, Pr=1.0, implicitx=0, implicity=0, seed=1,

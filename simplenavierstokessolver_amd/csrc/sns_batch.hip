@@ -423,6 +423,7 @@ using Clock = std::chrono::steady_clock;
 struct snsg_server {
   int device = 0;
   size_t budget = 0;
+  int max_active = 0;
   std::thread engine;
   // shared with the API threads (under mu)
   std::mutex mu;
@@ -434,7 +435,7 @@ struct snsg_server {
   bool stop = false;
   int failed = 0;
   std::string error;
-  long submitted = 0, admitted = 0, completed = 0, steps = 0;
+  long submitted = 0, admitted = 0, completed = 0, steps = 0, active_peak = 0;
   double cell_steps = 0.;
   std::vector<float> lat;  // per job-step latency (ms)
   Clock::time_point t_start, t_last;
@@ -501,6 +502,8 @@ static int engine_admit(snsg_server *s, bool &progressed) {
     {
       std::lock_guard<std::mutex> g(s->mu);
       if (s->queue.empty() || s->stop) return 0;
+      // slots: at most max_active jobs on the GPU at once (0: no limit)
+      if (s->max_active > 0 && s->admitted - s->completed >= s->max_active) return 0;
       j = s->queue.front();
     }
     const LaneKey key = key_of(j->cfg);
@@ -544,6 +547,7 @@ static int engine_admit(snsg_server *s, bool &progressed) {
     std::lock_guard<std::mutex> g(s->mu);
     j->state = JOB_ACTIVE;
     s->admitted++;
+    s->active_peak = std::max(s->active_peak, s->admitted - s->completed);
     if (!s->started) {
       s->started = true;
       s->t_start = s->t_last = Clock::now();
@@ -666,14 +670,16 @@ static void engine_main(snsg_server *s) {
   }
 }
 
-extern "C" int snsg_server_create(int device, size_t mem_budget, snsg_server **out) {
-  if (!out || mem_budget == 0) return set_err(SNSG_E_ARG, __func__);
+extern "C" int snsg_server_create(int device, size_t mem_budget, int max_active,
+                                  snsg_server **out) {
+  if (!out || mem_budget == 0 || max_active < 0) return set_err(SNSG_E_ARG, __func__);
   int n = 0;
   HIPCHECK(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return set_err(SNSG_E_ARG, "snsg_server_create: device");
   snsg_server *s = new snsg_server();
   s->device = device;
   s->budget = mem_budget;
+  s->max_active = max_active;
   s->engine = std::thread(engine_main, s);
   *out = s;
   return 0;
@@ -800,6 +806,7 @@ extern "C" int snsg_server_stats_get(snsg_server *s, snsg_server_stats *out) {
     out->completed = s->completed;
     out->queued = (long)s->queue.size();
     out->active = s->admitted - s->completed;
+    out->active_peak = s->active_peak;
     out->steps = s->steps;
     out->cell_steps = s->cell_steps;
     out->busy_s = s->started ? std::chrono::duration<double>(s->t_last - s->t_start).count() : 0.;

@@ -128,3 +128,18 @@ def test_server_stream_drains(sns):
     assert st["cell_steps"] == sum(s[0] * s[1] * s[2] for s in specs)
     assert st["mem_peak"] <= budget
     srv.close()
+
+
+def test_server_slots_bound_the_active_jobs(sns):
+    """With two slots, at most two jobs are on the GPU at once; the rest
+    wait in arrival order and all complete."""
+    sizes = [sns.instance_bytes(sns.make_config(s[0], s[1], implicitx=s[5], implicity=s[6]))
+             for s in SPECS]
+    srv = sns.Server(2 * sum(sizes), max_active=2)
+    for spec in SPECS:
+        submit(srv, spec, keep=False)
+    srv.drain(120.0)
+    st = srv.stats()
+    assert st["completed"] == len(SPECS) and st["active_peak"] == 2
+    assert st["steps"] == sum(s[2] for s in SPECS)
+    srv.close()
