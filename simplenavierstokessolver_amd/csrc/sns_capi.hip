@@ -227,6 +227,18 @@ int ctx_configure(snsg_ctx *c, const snsg_config *cfg, const double *dxf, const 
                  same(t[6], 1, itot - 1, tb.u_cU) && same(t[7], 1, itot - 1, uxp) &&
                  memcmp(&uxp, &tb.u_cU, sizeof uxp) == 0;
   }
+  // every divisor (DXF(1..itot), DXC(1..itot+1), dy) a power of two with an
+  // exact reciprocal: divisions become multiplications, bit for bit
+  {
+    auto p2 = [](double v) {
+      int e;
+      return v > 0. && frexp(v, &e) == 0.5 && (1. / v) * v == 1.;
+    };
+    bool ok = p2(dy);
+    for (int i = 1; i <= itot && ok; i++) ok = p2(DXF(i));
+    for (int i = 1; i <= itot + 1 && ok; i++) ok = p2(DXC(i));
+    tb.pow2 = ok;
+  }
   // fixed diffusive time-step bounds (src/param/decide_dt.c:48-66)
   const double diffusivity = fmax(sqrtPr / sqrtRa, 1. / sqrtPr / sqrtRa);
   double dx = 1.;

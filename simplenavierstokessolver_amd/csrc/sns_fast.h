@@ -32,6 +32,15 @@ __device__ __forceinline__ double div_cr(double a, double b, double r) {
   return __builtin_copysign(q2, q0);
 }
 
+// a/b for a divisor b with reciprocal r = 1/b.  P2: b is a power of two,
+// so r is exact and a*r is the correctly rounded quotient bit for bit
+// (subnormal and overflowing results included: both round the same real
+// number a*2^-k); otherwise the exact division above.
+template <bool P2>
+__device__ __forceinline__ double dv(double a, double b, double r) {
+  return P2 ? a * r : div_cr(a, b, r);
+}
+
 // ---------------------------------------------------------- lane shifts
 
 // Value of v in lane l-1 (lane 0: 0).
@@ -48,7 +57,6 @@ __device__ __forceinline__ double from_right(double v) {
   const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, false);
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
-
 // Per-column tables, all indexed like the reference arrays they derive from
 // (entry I-1 holds the value for column I unless noted).  They live in one
 // allocation of 14 double tables and one of 4 double2 tables, each table
@@ -67,6 +75,10 @@ struct Tables {
   // loading the paired tables; the values are the table entries themselves.
   int uniform;
   double u_dxc, u_rdxc, u_dxf, u_rdxf, u_cT, u_cU;
+  // Set when every divisor of the stencils (DXF, DXC of all columns and dy)
+  // is a power of two: the kernels then divide by multiplying with the
+  // tabulated reciprocal (see dv).
+  int pow2;
   __host__ __device__ const double *tab(int k) const { return t + k * stride; }
   __host__ __device__ const double *dxf() const { return tab(0); }    // DXF(I)
   __host__ __device__ const double *dxc() const { return tab(1); }    // DXC(I)
@@ -110,7 +122,6 @@ __device__ __forceinline__ double ldr(const double *row, int k) {
 __device__ __forceinline__ void str(double *row, int k, double v) {
   *reinterpret_cast<double *>(reinterpret_cast<char *>(row) + ((unsigned)k << 3)) = v;
 }
-
 // Rows j-1, j, j+1 of a field: pointers to the first stored element of each
 // row (I = 0 for uy, p, psi, T; I = 1 for ux).
 struct Rows3 {
@@ -147,21 +158,33 @@ struct CellX {
 // LANE: the lanes hold consecutive columns, so the gradient at the left face
 // of column i is the one the lane of column i-1 forms at its right face,
 // bit for bit (same operands, same operations); it is taken from there.
-template <bool LANE = false>
+// The x-gradients at the faces of cell i, dtdx_xm = (T(i)-T(i-1))/DXC(i) and
+// dtdx_xp = (T(i+1)-T(i))/DXC(i+1), are given (the t_xm, t_xp entries of v
+// are not read): a kernel holding consecutive columns forms each face
+// gradient once and hands it to both cells.
+template <bool P2 = false>
+__device__ __forceinline__ void temp_src_g(const Geo &g, const Tables &tb, const CellX &x,
+                                           const TempPt &v, double dtdx_xm, double dtdx_xp,
+                                           double wex, double wey, double wix, double wiy,
+                                           double &srca, double &srcg) {
+  const double dy = tb.dy, rdy = tb.rdy;
+  const double dtdy_ym = dv<P2>(-v.t_ym + v.t_c, dy, rdy);
+  const double dtdy_yp = dv<P2>(-v.t_c + v.t_yp, dy, rdy);
+  const double adv1 = -x.c_xm * v.ux_c * dtdx_xm - x.c_xp * v.ux_p * dtdx_xp;
+  const double adv2 = -0.5 * v.uy_c * dtdy_ym - 0.5 * v.uy_p * dtdy_yp;
+  const double dif1 = dv<P2>(g.kappa_T * (dtdx_xp - dtdx_xm), x.dxf, x.rdxf);
+  const double dif2 = dv<P2>(g.kappa_T * (dtdy_yp - dtdy_ym), dy, rdy);
+  srca = +(adv1 + adv2) + wex * dif1 + wey * dif2;
+  srcg = +wix * dif1 + wiy * dif2;
+}
+
+template <bool LANE = false, bool P2 = false>
 __device__ __forceinline__ void temp_src_v(const Geo &g, const Tables &tb, const CellX &x,
                                            const TempPt &v, double wex, double wey, double wix,
                                            double wiy, double &srca, double &srcg) {
-  const double dy = tb.dy, rdy = tb.rdy;
-  const double dtdx_xp = div_cr(-v.t_c + v.t_xp, x.dxc_p, x.rdxc_p);
-  const double dtdx_xm = LANE ? from_left(dtdx_xp) : div_cr(-v.t_xm + v.t_c, x.dxc, x.rdxc);
-  const double dtdy_ym = div_cr(-v.t_ym + v.t_c, dy, rdy);
-  const double dtdy_yp = div_cr(-v.t_c + v.t_yp, dy, rdy);
-  const double adv1 = -x.c_xm * v.ux_c * dtdx_xm - x.c_xp * v.ux_p * dtdx_xp;
-  const double adv2 = -0.5 * v.uy_c * dtdy_ym - 0.5 * v.uy_p * dtdy_yp;
-  const double dif1 = div_cr(g.kappa_T * (dtdx_xp - dtdx_xm), x.dxf, x.rdxf);
-  const double dif2 = div_cr(g.kappa_T * (dtdy_yp - dtdy_ym), dy, rdy);
-  srca = +(adv1 + adv2) + wex * dif1 + wey * dif2;
-  srcg = +wix * dif1 + wiy * dif2;
+  const double dtdx_xp = dv<P2>(-v.t_c + v.t_xp, x.dxc_p, x.rdxc_p);
+  const double dtdx_xm = LANE ? from_left(dtdx_xp) : dv<P2>(-v.t_xm + v.t_c, x.dxc, x.rdxc);
+  temp_src_g<P2>(g, tb, x, v, dtdx_xm, dtdx_xp, wex, wey, wix, wiy, srca, srcg);
 }
 
 // x-momentum at face (i, j), i = 2..itot: src/fluid/update_velocity.c:34-101
@@ -176,27 +199,37 @@ struct UxPt {
 struct FaceX {
   double dxf_m, rdxf_m, dxf, rdxf, c_xm, c_xp, dxc, rdxc;
 };
-template <bool LANE = false>
-__device__ __forceinline__ void ux_src_v(const Geo &g, const Tables &tb, const FaceX &x,
-                                         const UxPt &v, double wex, double wey, double wix,
-                                         double wiy, double &srca, double &srcg) {
+// The x-gradients (UX(i)-UX(i-1))/DXF(i-1) and (UX(i+1)-UX(i))/DXF(i) are
+// given, as for temp_src_g (u_xm and u_xp still enter the interpolations).
+template <bool P2 = false>
+__device__ __forceinline__ void ux_src_g(const Geo &g, const Tables &tb, const FaceX &x,
+                                         const UxPt &v, double duxdx_xm, double duxdx_xp,
+                                         double wex, double wey, double wix, double wiy,
+                                         double &srca, double &srcg) {
   const double dy = tb.dy, rdy = tb.rdy;
-  const double duxdx_xp = div_cr(-v.u_c + v.u_xp, x.dxf, x.rdxf);
-  const double duxdx_xm =
-      LANE ? from_left(duxdx_xp) : div_cr(-v.u_xm + v.u_c, x.dxf_m, x.rdxf_m);
-  const double duxdy_ym = div_cr(-v.u_ym + v.u_c, dy, rdy);
-  const double duxdy_yp = div_cr(-v.u_c + v.u_yp, dy, rdy);
+  const double duxdy_ym = dv<P2>(-v.u_ym + v.u_c, dy, rdy);
+  const double duxdy_yp = dv<P2>(-v.u_c + v.u_yp, dy, rdy);
   const double ux_xm = 0.5 * v.u_xm + 0.5 * v.u_c;
   const double ux_xp = 0.5 * v.u_c + 0.5 * v.u_xp;
   const double adv1 = -x.c_xm * ux_xm * duxdx_xm - x.c_xp * ux_xp * duxdx_xp;
   const double uy_ym = x.c_xm * v.uy_cm + x.c_xp * v.uy_cc;
   const double uy_yp = x.c_xm * v.uy_pm + x.c_xp * v.uy_pc;
   const double adv2 = -0.5 * uy_ym * duxdy_ym - 0.5 * uy_yp * duxdy_yp;
-  const double dif1 = div_cr(g.kappa_u * (duxdx_xp - duxdx_xm), x.dxc, x.rdxc);
-  const double dif2 = div_cr(g.kappa_u * (duxdy_yp - duxdy_ym), dy, rdy);
-  const double pre = div_cr(-(v.p_c - v.p_m), x.dxc, x.rdxc);
+  const double dif1 = dv<P2>(g.kappa_u * (duxdx_xp - duxdx_xm), x.dxc, x.rdxc);
+  const double dif2 = dv<P2>(g.kappa_u * (duxdy_yp - duxdy_ym), dy, rdy);
+  const double pre = dv<P2>(-(v.p_c - v.p_m), x.dxc, x.rdxc);
   srca = +(adv1 + adv2) + wex * dif1 + wey * dif2 + v.tf;
   srcg = +wix * dif1 + wiy * dif2 + pre;
+}
+
+template <bool LANE = false, bool P2 = false>
+__device__ __forceinline__ void ux_src_v(const Geo &g, const Tables &tb, const FaceX &x,
+                                         const UxPt &v, double wex, double wey, double wix,
+                                         double wiy, double &srca, double &srcg) {
+  const double duxdx_xp = dv<P2>(-v.u_c + v.u_xp, x.dxf, x.rdxf);
+  const double duxdx_xm =
+      LANE ? from_left(duxdx_xp) : dv<P2>(-v.u_xm + v.u_c, x.dxf_m, x.rdxf_m);
+  ux_src_g<P2>(g, tb, x, v, duxdx_xm, duxdx_xp, wex, wey, wix, wiy, srca, srcg);
 }
 
 // y-momentum at (i, j), i = 1..itot: src/fluid/update_velocity.c:128-189
@@ -205,26 +238,36 @@ struct UyPt {
   double ux_mc, ux_cc, ux_mp, ux_cp;   // UX(i,j-1), UX(i,j), UX(i+1,j-1), UX(i+1,j)
   double p_c, p_ym;                    // P(i,j), P(i,j-1)
 };
-template <bool LANE = false>
-__device__ __forceinline__ void uy_src_v(const Geo &g, const Tables &tb, const CellX &x,
-                                         const UyPt &v, double wex, double wey, double wix,
-                                         double wiy, double &srca, double &srcg) {
+// The x-gradients (UY(i)-UY(i-1))/DXC(i) and (UY(i+1)-UY(i))/DXC(i+1) are
+// given, as for temp_src_g.
+template <bool P2 = false>
+__device__ __forceinline__ void uy_src_g(const Geo &g, const Tables &tb, const CellX &x,
+                                         const UyPt &v, double duydx_xm, double duydx_xp,
+                                         double wex, double wey, double wix, double wiy,
+                                         double &srca, double &srcg) {
   const double dy = tb.dy, rdy = tb.rdy;
-  const double duydx_xp = div_cr(-v.v_c + v.v_xp, x.dxc_p, x.rdxc_p);
-  const double duydx_xm = LANE ? from_left(duydx_xp) : div_cr(-v.v_xm + v.v_c, x.dxc, x.rdxc);
-  const double duydy_ym = div_cr(-v.v_ym + v.v_c, dy, rdy);
-  const double duydy_yp = div_cr(-v.v_c + v.v_yp, dy, rdy);
+  const double duydy_ym = dv<P2>(-v.v_ym + v.v_c, dy, rdy);
+  const double duydy_yp = dv<P2>(-v.v_c + v.v_yp, dy, rdy);
   const double ux_xm = 0.5 * v.ux_mc + 0.5 * v.ux_cc;
   const double ux_xp = 0.5 * v.ux_mp + 0.5 * v.ux_cp;
   const double adv1 = -x.c_xm * ux_xm * duydx_xm - x.c_xp * ux_xp * duydx_xp;
   const double uy_ym = 0.5 * v.v_ym + 0.5 * v.v_c;
   const double uy_yp = 0.5 * v.v_c + 0.5 * v.v_yp;
   const double adv2 = -0.5 * uy_ym * duydy_ym - 0.5 * uy_yp * duydy_yp;
-  const double dif1 = div_cr(g.kappa_u * (duydx_xp - duydx_xm), x.dxf, x.rdxf);
-  const double dif2 = div_cr(g.kappa_u * (duydy_yp - duydy_ym), dy, rdy);
-  const double pre = div_cr(-(v.p_c - v.p_ym), dy, rdy);
+  const double dif1 = dv<P2>(g.kappa_u * (duydx_xp - duydx_xm), x.dxf, x.rdxf);
+  const double dif2 = dv<P2>(g.kappa_u * (duydy_yp - duydy_ym), dy, rdy);
+  const double pre = dv<P2>(-(v.p_c - v.p_ym), dy, rdy);
   srca = +(adv1 + adv2) + wex * dif1 + wey * dif2;
   srcg = +wix * dif1 + wiy * dif2 + pre;
+}
+
+template <bool LANE = false, bool P2 = false>
+__device__ __forceinline__ void uy_src_v(const Geo &g, const Tables &tb, const CellX &x,
+                                         const UyPt &v, double wex, double wey, double wix,
+                                         double wiy, double &srca, double &srcg) {
+  const double duydx_xp = dv<P2>(-v.v_c + v.v_xp, x.dxc_p, x.rdxc_p);
+  const double duydx_xm = LANE ? from_left(duydx_xp) : dv<P2>(-v.v_xm + v.v_c, x.dxc, x.rdxc);
+  uy_src_g<P2>(g, tb, x, v, duydx_xm, duydx_xp, wex, wey, wix, wiy, srca, srcg);
 }
 
 // ------------------------------------------------ loaders (pointer form)
@@ -238,6 +281,7 @@ __device__ __forceinline__ FaceX face_x(const Tables &tb, int i) {
           ldr(tb.cU_xm(), i - 1), ldr(tb.cU_xp(), i - 1), ldr(tb.dxc(), i - 1), ldr(tb.rdxc(), i - 1)};
 }
 
+template <bool P2 = false>
 __device__ __forceinline__ void temp_src_t(const Geo &g, const Tables &tb, const Rows3 &ux,
                                            const Rows3 &uy, const Rows3 &temp, int i, double wex,
                                            double wey, double wix, double wiy, double &srca,
@@ -245,9 +289,10 @@ __device__ __forceinline__ void temp_src_t(const Geo &g, const Tables &tb, const
   const TempPt v = {ldr(temp.c, i), ldr(temp.c, i - 1), ldr(temp.c, i + 1), ldr(temp.m, i),
                     ldr(temp.p, i), ldr(ux.c, i - 1),   ldr(ux.c, i),       ldr(uy.c, i),
                     ldr(uy.p, i)};
-  temp_src_v(g, tb, cell_x(tb, i), v, wex, wey, wix, wiy, srca, srcg);
+  temp_src_v<false, P2>(g, tb, cell_x(tb, i), v, wex, wey, wix, wiy, srca, srcg);
 }
 
+template <bool P2 = false>
 __device__ __forceinline__ void ux_src_t(const Geo &g, const Tables &tb, const Rows3 &ux,
                                          const Rows3 &uy, const Rows3 &p, double tf, int i,
                                          double wex, double wey, double wix, double wiy,
@@ -255,9 +300,10 @@ __device__ __forceinline__ void ux_src_t(const Geo &g, const Tables &tb, const R
   const UxPt v = {ldr(ux.c, i - 1), ldr(ux.c, i - 2), ldr(ux.c, i), ldr(ux.m, i - 1),
                   ldr(ux.p, i - 1), ldr(uy.c, i - 1), ldr(uy.c, i), ldr(uy.p, i - 1),
                   ldr(uy.p, i),     ldr(p.c, i),      ldr(p.c, i - 1), tf};
-  ux_src_v(g, tb, face_x(tb, i), v, wex, wey, wix, wiy, srca, srcg);
+  ux_src_v<false, P2>(g, tb, face_x(tb, i), v, wex, wey, wix, wiy, srca, srcg);
 }
 
+template <bool P2 = false>
 __device__ __forceinline__ void uy_src_t(const Geo &g, const Tables &tb, const Rows3 &ux,
                                          const Rows3 &uy, const Rows3 &p, int i, double wex,
                                          double wey, double wix, double wiy, double &srca,
@@ -265,7 +311,7 @@ __device__ __forceinline__ void uy_src_t(const Geo &g, const Tables &tb, const R
   const UyPt v = {ldr(uy.c, i),     ldr(uy.c, i - 1), ldr(uy.c, i + 1), ldr(uy.m, i),
                   ldr(uy.p, i),     ldr(ux.m, i - 1), ldr(ux.c, i - 1), ldr(ux.m, i),
                   ldr(ux.c, i),     ldr(p.c, i),      ldr(p.m, i)};
-  uy_src_v(g, tb, cell_x(tb, i), v, wex, wey, wix, wiy, srca, srcg);
+  uy_src_v<false, P2>(g, tb, cell_x(tb, i), v, wex, wey, wix, wiy, srca, srcg);
 }
 
 // ---------------------------------------------------------------- stores

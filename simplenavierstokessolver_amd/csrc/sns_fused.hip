@@ -13,6 +13,8 @@
 // boundary values follow the reference expression for expression (see
 // sns_fast.h for the division scheme); only the line solves round
 // differently.
+#include <stdlib.h>
+#include <string.h>
 #include "sns_launch.h"
 #include "sns_fast.h"
 #include "sns_fused.h"
@@ -44,24 +46,25 @@ __device__ __forceinline__ PredRows pred_rows(const PredArgs &a, int j) {
 
 // Increment of field f (0: T, 1: ux, 2: uy) at (i, j); the RK history of the
 // explicit terms is read (beta != 0) and rewritten in place.
+template <bool P2>
 __device__ __forceinline__ double pred_field(const PredArgs &a, const PredRows &R, int f, int i,
                                              double dt) {
   const Geo &g = a.g;
   double A, G, B = 0.;
   if (f == 0) {
-    temp_src_t(g, a.tb, R.ux, R.uy, R.t, i, a.tr.w_ex_x, a.tr.w_ex_y, a.tr.w_im_x, a.tr.w_im_y,
+    temp_src_t<P2>(g, a.tb, R.ux, R.uy, R.t, i, a.tr.w_ex_x, a.tr.w_ex_y, a.tr.w_im_x, a.tr.w_im_y,
                A, G);
     if (a.read_hist) B = ldr(R.ht, i - 1);
     if (a.write_hist) str(R.ht, i - 1, A);
   } else if (f == 1) {
     double tf = 0.;
     if (a.forcing) tf = 0.5 * ldr(R.t.c, i - 1) + 0.5 * ldr(R.t.c, i);
-    ux_src_t(g, a.tb, R.ux, R.uy, R.p, tf, i, a.tr.w_ex_x, a.tr.w_ex_y, a.tr.w_im_x,
+    ux_src_t<P2>(g, a.tb, R.ux, R.uy, R.p, tf, i, a.tr.w_ex_x, a.tr.w_ex_y, a.tr.w_im_x,
              a.tr.w_im_y, A, G);
     if (a.read_hist) B = ldr(R.hu, i - 2);
     if (a.write_hist) str(R.hu, i - 2, A);
   } else {
-    uy_src_t(g, a.tb, R.ux, R.uy, R.p, i, a.tr.w_ex_x, a.tr.w_ex_y, a.tr.w_im_x, a.tr.w_im_y, A,
+    uy_src_t<P2>(g, a.tb, R.ux, R.uy, R.p, i, a.tr.w_ex_x, a.tr.w_ex_y, a.tr.w_im_x, a.tr.w_im_y, A,
              G);
     if (a.read_hist) B = ldr(R.hv, i - 1);
     if (a.write_hist) str(R.hv, i - 1, A);
@@ -87,7 +90,7 @@ __device__ __forceinline__ void emit_field(const PredArgs &a, const PredRows &R,
 }
 
 // explicit diffusion in x: point-wise, the three fields per thread
-template <class S>
+template <bool P2, class S>
 __global__ void __launch_bounds__(256) k_pred_exp(S src) {
   const PredArgs a = src.get();
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -95,25 +98,34 @@ __global__ void __launch_bounds__(256) k_pred_exp(S src) {
   if (i > a.g.itot) return;
   const double dt = *a.dtp;
   const PredRows R = pred_rows(a, j);
-  if (a.with_temp) emit_field(a, R, 0, i, j, pred_field(a, R, 0, i, dt));
-  if (i >= 2) emit_field(a, R, 1, i, j, pred_field(a, R, 1, i, dt));
-  emit_field(a, R, 2, i, j, pred_field(a, R, 2, i, dt));
+  if (a.with_temp) emit_field(a, R, 0, i, j, pred_field<P2>(a, R, 0, i, dt));
+  if (i >= 2) emit_field(a, R, 1, i, j, pred_field<P2>(a, R, 1, i, dt));
+  emit_field(a, R, 2, i, j, pred_field<P2>(a, R, 2, i, dt));
 }
 
+// Power-of-two divisors select the variant that divides by multiplying;
+// SNSG_EXACT_DIV=1 keeps the exact division sequence (both return the same
+// bits).
 template <class S>
-static void pred_exp_launch(hipStream_t s, const Geo &g, S src, int nz) {
-  hipLaunchKernelGGL(k_pred_exp<S>, dim3((g.itot + 255) / 256, g.jtot, nz), dim3(256), 0, s, src);
+static void pred_exp_launch(hipStream_t s, const PredArgs &shape, S src, int nz) {
+  const Geo &g = shape.g;
+  const dim3 grid((g.itot + 255) / 256, g.jtot, nz);
+  const char *ex = getenv("SNSG_EXACT_DIV");
+  if (shape.tb.pow2 && !(ex && strcmp(ex, "1") == 0))
+    hipLaunchKernelGGL((k_pred_exp<true, S>), grid, dim3(256), 0, s, src);
+  else
+    hipLaunchKernelGGL((k_pred_exp<false, S>), grid, dim3(256), 0, s, src);
 }
 
 void launch_pred(hipStream_t s, const PredArgs &a, int implicitx) {
   if (implicitx) launch_pred_rows(s, a);
-  else pred_exp_launch(s, a.g, ArgOne<PredArgs>{a}, 1);
+  else pred_exp_launch(s, a, ArgOne<PredArgs>{a}, 1);
 }
 
 void launch_pred_batch(hipStream_t s, const PredArgs &shape, int implicitx, const PredArgs *dev,
                        int nz) {
   if (implicitx) launch_pred_rows_batch(s, shape, dev, nz);
-  else pred_exp_launch(s, shape.g, arg_batch<PredArgs>(dev), nz);
+  else pred_exp_launch(s, shape, arg_batch<PredArgs>(dev), nz);
 }
 
 // ------------------------------------------------- correction + pressure

@@ -14,6 +14,8 @@
 //    values of the columns on either side;
 //  * the y neighbours of one row are the centre values of the other, and the
 //    per-column metrics and the LU tables serve both rows.
+#include <stdlib.h>
+#include <string.h>
 #include "sns_launch.h"
 #include "sns_fast.h"
 #include "sns_fused.h"
@@ -265,7 +267,7 @@ __device__ __forceinline__ Raw<F> raw_load(const PredArgs &a, const LanePos &p, 
 
 // Source terms and RK increments of both rows from the loaded values (x
 // neighbours from the adjacent lanes); the new RK history is stored.
-template <int F>
+template <int F, bool P2>
 __device__ __forceinline__ void raw_eval(const PredArgs &a, const Raw<F> &r, const LanePos &p,
                                          int ja, int jb, int n, bool write_a, double dt,
                                          double &da, double &db) {
@@ -289,10 +291,10 @@ __device__ __forceinline__ void raw_eval(const PredArgs &a, const Raw<F> &r, con
                        r.u_a, from_right(r.u_a), r.v_a, r.v_b};
     const TempPt pb = {r.t_b, from_left(r.t_b), from_right(r.t_b), r.t_a, r.t_bp,
                        r.u_b, from_right(r.u_b), r.v_b, r.v_bp};
-    temp_src_v<true>(g, tb, x, pa, wex, wey, wix, wiy, A, G);
+    temp_src_v<true, P2>(g, tb, x, pa, wex, wey, wix, wiy, A, G);
     if (a.write_hist && p.out && write_a) str(HA, p.ii, A);
     da = rk_incr(a.alpha, a.beta, a.gamma, dt, A, r.ba, G);
-    temp_src_v<true>(g, tb, x, pb, wex, wey, wix, wiy, A, G);
+    temp_src_v<true, P2>(g, tb, x, pb, wex, wey, wix, wiy, A, G);
     if (a.write_hist && p.out) str(HB, p.ii, A);
     db = rk_incr(a.alpha, a.beta, a.gamma, dt, A, r.bb, G);
   } else if constexpr (F == 1) {
@@ -315,10 +317,10 @@ __device__ __forceinline__ void raw_eval(const PredArgs &a, const Raw<F> &r, con
                      from_left(r.v_a), r.v_a, v_bm, r.v_b, r.p_a, from_left(r.p_a), tf_a};
     const UxPt pb = {r.u_b, from_left(r.u_b), from_right(r.u_b), r.u_a, r.u_bp,
                      v_bm, r.v_b, from_left(r.v_bp), r.v_bp, r.p_b, from_left(r.p_b), tf_b};
-    ux_src_v<true>(g, tb, x, pa, wex, wey, wix, wiy, A, G);
+    ux_src_v<true, P2>(g, tb, x, pa, wex, wey, wix, wiy, A, G);
     if (a.write_hist && p.out && write_a) str(HA, p.ii, A);
     da = rk_incr(a.alpha, a.beta, a.gamma, dt, A, r.ba, G);
-    ux_src_v<true>(g, tb, x, pb, wex, wey, wix, wiy, A, G);
+    ux_src_v<true, P2>(g, tb, x, pb, wex, wey, wix, wiy, A, G);
     if (a.write_hist && p.out) str(HB, p.ii, A);
     db = rk_incr(a.alpha, a.beta, a.gamma, dt, A, r.bb, G);
   } else {
@@ -336,10 +338,10 @@ __device__ __forceinline__ void raw_eval(const PredArgs &a, const Raw<F> &r, con
                      r.u_am, r.u_a, from_right(r.u_am), u_ap, r.p_a, r.p_am};
     const UyPt pb = {r.v_b, from_left(r.v_b), from_right(r.v_b), r.v_a, r.v_bp,
                      r.u_a, r.u_b, u_ap, from_right(r.u_b), r.p_b, r.p_a};
-    uy_src_v<true>(g, tb, x, pa, wex, wey, wix, wiy, A, G);
+    uy_src_v<true, P2>(g, tb, x, pa, wex, wey, wix, wiy, A, G);
     if (a.write_hist && p.out && write_a) str(HA, p.ii, A);
     da = rk_incr(a.alpha, a.beta, a.gamma, dt, A, r.ba, G);
-    uy_src_v<true>(g, tb, x, pb, wex, wey, wix, wiy, A, G);
+    uy_src_v<true, P2>(g, tb, x, pb, wex, wey, wix, wiy, A, G);
     if (a.write_hist && p.out) str(HB, p.ii, A);
     db = rk_incr(a.alpha, a.beta, a.gamma, dt, A, r.bb, G);
   }
@@ -365,7 +367,7 @@ __device__ __forceinline__ double2 raw_own(const Raw<F> &r) {
 // (compile-time block index).  The workgroup's own field at rows ja, jb is
 // kept per block in own[] for the final values, so those rows are not read
 // a second time after the line solve.
-template <int K, int F, int NT, int M>
+template <int K, int F, int NT, int M, bool P2>
 __device__ __forceinline__ void pred_stencils(const PredArgs &a, PredRowsLds<NT, M> &L, int ja,
                                               int jb, bool write_a, int n, int i0, double dt,
                                               int nk, double2 (&own)[stencil_blocks<NT, M>()]) {
@@ -374,13 +376,13 @@ __device__ __forceinline__ void pred_stencils(const PredArgs &a, PredRowsLds<NT,
     const LanePos p = lane_pos(K, NT / 64, i0, n, a.g.itot);
     const Raw<F> r = raw_load<F>(a, p, ja, jb, n);
     double da, db;
-    raw_eval<F>(a, r, p, ja, jb, n, write_a, dt, da, db);
+    raw_eval<F, P2>(a, r, p, ja, jb, n, write_a, dt, da, db);
     own[K] = raw_own<F>(r);
     if (p.out) {
       L.d[0][pad_idx<M>(p.ii)] = da;
       L.d[1][pad_idx<M>(p.ii)] = db;
     }
-    pred_stencils<K + 1, F, NT, M>(a, L, ja, jb, write_a, n, i0, dt, nk, own);
+    pred_stencils<K + 1, F, NT, M, P2>(a, L, ja, jb, write_a, n, i0, dt, nk, own);
   }
 }
 
@@ -414,7 +416,7 @@ __device__ __forceinline__ void pred_emit(const PredArgs &a, const PredRowsLds<N
 
 // One workgroup's field F for rows ja and jb (write_a: row ja is written):
 // stencils into LDS, the CN-x solve of both rows, final values.
-template <int F, int NT, int M>
+template <int F, int NT, int M, bool P2>
 __device__ __forceinline__ void pred_rows_body(const PredArgs &a, PredRowsLds<NT, M> &L, int ja,
                                                int jb, bool write_a) {
   constexpr int NW = NT / 64;
@@ -428,7 +430,7 @@ __device__ __forceinline__ void pred_rows_body(const PredArgs &a, PredRowsLds<NT
 
   // ---- stencils and RK increments of both rows into LDS (lane order)
   double2 own[KB];
-  pred_stencils<0, F, NT, M>(a, L, ja, jb, write_a, n, i0, dt, nk, own);
+  pred_stencils<0, F, NT, M, P2>(a, L, ja, jb, write_a, n, i0, dt, nk, own);
   __syncthreads();
 
   // ---- Crank-Nicolson in x of both rows (chunk order)
@@ -460,7 +462,7 @@ __device__ __forceinline__ void pred_rows_body(const PredArgs &a, PredRowsLds<NT
 // Workgroup w: field w % nfld (+1 without the temperature equation) of rows
 // a = 2q+1 and b = a+1, q = w / nfld.  For odd jtot the last pair is shifted
 // down to (jtot-1, jtot) and writes row b only.  Consecutive w share an XCD.
-template <int NT, int M, class S>
+template <int NT, int M, bool P2, class S>
 __global__ void __launch_bounds__(NT) k_pred_rows(S src, int nfld) {
   const PredArgs a = src.get();
   extern __shared__ double lds_raw[];
@@ -470,25 +472,25 @@ __global__ void __launch_bounds__(NT) k_pred_rows(S src, int nfld) {
   const int f = w % nfld + (nfld == 3 ? 0 : 1);
   const int ja = min(2 * q + 1, a.g.jtot - 1), jb = ja + 1;
   const bool write_a = ja == 2 * q + 1;
-  if (f == 0) pred_rows_body<0, NT, M>(a, L, ja, jb, write_a);
-  else if (f == 1) pred_rows_body<1, NT, M>(a, L, ja, jb, write_a);
-  else pred_rows_body<2, NT, M>(a, L, ja, jb, write_a);
+  if (f == 0) pred_rows_body<0, NT, M, P2>(a, L, ja, jb, write_a);
+  else if (f == 1) pred_rows_body<1, NT, M, P2>(a, L, ja, jb, write_a);
+  else pred_rows_body<2, NT, M, P2>(a, L, ja, jb, write_a);
 }
 
 // ---------------------------------------------------------------- launch
 
-template <int NT, int M, class S>
+template <int NT, int M, bool P2, class S>
 static void pred_rows_launch(hipStream_t s, const PredArgs &shape, S src, int nz) {
   const size_t lds = sizeof(PredRowsLds<NT, M>);
   static bool attr = false;
   if (!attr) {
-    hipFuncSetAttribute((const void *)k_pred_rows<NT, M, S>,
+    hipFuncSetAttribute((const void *)k_pred_rows<NT, M, P2, S>,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr = true;
   }
   const int nfld = shape.with_temp ? 3 : 2;
   const int npair = (shape.g.jtot + 1) / 2;
-  hipLaunchKernelGGL((k_pred_rows<NT, M, S>), dim3(npair * nfld, 1, nz), dim3(NT), lds, s, src,
+  hipLaunchKernelGGL((k_pred_rows<NT, M, P2, S>), dim3(npair * nfld, 1, nz), dim3(NT), lds, s, src,
                      nfld);
 }
 
@@ -542,17 +544,27 @@ void launch_lu_factor_batch(hipStream_t s, const LuFactorArgs &shape, const LuFa
   lu_factor_dispatch(s, shape.g.itot, arg_batch<LuFactorArgs>(dev), nz);
 }
 
+template <bool P2, class S>
+static void pred_rows_select(hipStream_t s, const PredArgs &shape, S src, int nz, int NT) {
+  switch (NT) {
+    case 64: pred_rows_launch<64, kRowM, P2>(s, shape, src, nz); break;
+    case 128: pred_rows_launch<128, kRowM, P2>(s, shape, src, nz); break;
+    case 256: pred_rows_launch<256, kRowM, P2>(s, shape, src, nz); break;
+    case 512: pred_rows_launch<512, kRowM, P2>(s, shape, src, nz); break;
+    default: pred_rows_launch<1024, kRowM, P2>(s, shape, src, nz); break;
+  }
+}
+
+// Power-of-two divisors select the kernel that divides by multiplying;
+// SNSG_EXACT_DIV=1 keeps the exact division sequence (read at every launch;
+// both variants return the same bits).
 template <class S>
 static void pred_rows_dispatch(hipStream_t s, const PredArgs &shape, S src, int nz) {
   int NT, M;
   row_config(shape.g.itot, NT, M);
-  switch (NT) {
-    case 64: pred_rows_launch<64, kRowM>(s, shape, src, nz); break;
-    case 128: pred_rows_launch<128, kRowM>(s, shape, src, nz); break;
-    case 256: pred_rows_launch<256, kRowM>(s, shape, src, nz); break;
-    case 512: pred_rows_launch<512, kRowM>(s, shape, src, nz); break;
-    default: pred_rows_launch<1024, kRowM>(s, shape, src, nz); break;
-  }
+  const char *ex = getenv("SNSG_EXACT_DIV");
+  if (shape.tb.pow2 && !(ex && strcmp(ex, "1") == 0)) pred_rows_select<true>(s, shape, src, nz, NT);
+  else pred_rows_select<false>(s, shape, src, nz, NT);
 }
 
 void launch_pred_rows(hipStream_t s, const PredArgs &a) {

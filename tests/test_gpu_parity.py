@@ -361,6 +361,53 @@ def test_fused_step_matches_reference_functions(sns, case):
     inst.close()
 
 
+POW2_CASES = [
+    # (itot, jtot, imx, imy, with_temperature, with_thermal_forcing): uniform
+    # grids whose spacings dx = 1/itot, dy = 2/jtot and half cells at the
+    # walls are all powers of two
+    (64, 64, 1, 0, True, True),
+    (256, 128, 1, 1, True, True),
+    (512, 32, 1, 0, True, True),
+    (4096, 8, 1, 0, True, True),
+    (64, 64, 1, 0, False, False),
+    (128, 64, 1, 0, True, False),
+    (64, 64, 0, 0, True, True),
+    (256, 128, 0, 1, True, True),
+    (128, 64, 0, 0, False, False),
+]
+
+
+@pytest.mark.parametrize("case", POW2_CASES)
+def test_pow2_division_is_exact(sns, monkeypatch, case):
+    """With every divisor a power of two the predictors divide by multiplying
+    with the exact reciprocal; SNSG_EXACT_DIV=1 selects the exact division
+    sequence instead.  After three steps every field agrees bit for bit."""
+    import math
+    itot, jtot, imx, imy, temp, forcing = case
+    out = []
+    for exact in ("0", "1"):
+        monkeypatch.setenv("SNSG_EXACT_DIV", exact)
+        o = po.Oracle(itot, jtot, stretch=float(itot), Ra=1e9, Pr=0.7)
+        o.init_temperature(21, 1e-2, linear=False)
+        o.apply_boundaries()
+        inst = sns.Instance(itot, jtot, stretch=float(itot), Ra=1e9, Pr=0.7, implicitx=imx,
+                            implicity=imy, with_temperature=temp, with_thermal_forcing=forcing)
+        spacings = list(inst.g["dxf"]) + list(inst.g["dxc"]) + [2.0 / jtot]
+        assert all(math.frexp(v)[0] == 0.5 for v in spacings)
+        inst.state_alloc()
+        rng = np.random.default_rng(5)  # the same velocity noise for both runs
+        for n in ["ux", "uy", "p", "psi", "temp"]:
+            f = o.f[n]
+            if n in ("ux", "uy"):
+                f = f + 1e-2 * rng.standard_normal(f.shape)
+            inst.state_upload(n, f)
+        inst.advance(3)
+        out.append({n: inst.state_download(n) for n in ["ux", "uy", "p", "temp"]})
+        inst.close()
+    for n in out[0]:
+        assert_bitwise(out[0][n], out[1][n], n)
+
+
 @pytest.mark.parametrize("case", STEP_CASES)
 def test_advance_matches_oracle_time_loop(sns, pinned_gauge, case):
     """snsg_advance (time step decided on the device) against the oracle's
