@@ -173,6 +173,73 @@ __device__ __noinline__ void fft_pass_il(double2 *buf, int S, int n, int Ns,
   __syncthreads();
 }
 
+// Power-of-two lengths L = 8..128 with the pass schedule fixed at compile
+// time (radix 8 while at least 8 remains, then the remainder) and S = 2^lgS
+// sequences: the same Stockham passes as fft_pass_il, inlined, with every
+// index formed by shifts and masks.
+constexpr int p2_radix(int L, int p) {
+  int rem = L;
+  for (int q = 0; q < p; q++) rem /= rem >= 8 ? 8 : rem;
+  return rem >= 8 ? 8 : rem;
+}
+constexpr int p2_npass(int L) {
+  int np = 0;
+  for (int rem = L; rem > 1; rem /= rem >= 8 ? 8 : rem) np++;
+  return np;
+}
+
+template <int NT, int EMAX, int L, int R, int NS, int SIGN>
+__device__ __forceinline__ void fft_pass_p2(double2 *buf, int lgS,
+                                            const double2 *__restrict__ tw, int twmul) {
+  constexpr int BMAX = (EMAX + R - 1) / R;
+  constexpr int nb = L / R;
+  const int S = 1 << lgS;
+  const int total = S * nb;
+  const int tstep = (L / (NS * R)) * twmul;
+  double2 x[BMAX][R];
+#pragma unroll
+  for (int k = 0; k < BMAX; k++) {
+    const int bidx = threadIdx.x + k * NT;
+    if (bidx < total) {
+      const int j = bidx >> lgS, s = bidx & (S - 1);
+#pragma unroll
+      for (int r = 0; r < R; r++) x[k][r] = buf[((j + r * nb) << lgS) + s];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < BMAX; k++) {
+    const int bidx = threadIdx.x + k * NT;
+    if (bidx < total) {
+      const int j = bidx >> lgS, s = bidx & (S - 1);
+      const int kk = j & (NS - 1);
+      if (NS > 1 && kk != 0) {
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+          double2 w = tw[r * kk * tstep];
+          if (SIGN > 0) w.y = -w.y;
+          x[k][r] = cmul(x[k][r], w);
+        }
+      }
+      Dft<R, SIGN>::run(x[k]);
+      const int base = (j - kk) * R + kk;
+#pragma unroll
+      for (int r = 0; r < R; r++) buf[((base + r * NS) << lgS) + s] = x[k][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <int NT, int EMAX, int L, int SIGN, int P = 0, int NS = 1>
+__device__ __forceinline__ void fft_p2(double2 *buf, int lgS, const double2 *__restrict__ tw,
+                                       int twmul) {
+  if constexpr (P < p2_npass(L)) {
+    constexpr int R = p2_radix(L, P);
+    fft_pass_p2<NT, EMAX, L, R, NS, SIGN>(buf, lgS, tw, twmul);
+    fft_p2<NT, EMAX, L, SIGN, P + 1, NS * R>(buf, lgS, tw, twmul);
+  }
+}
+
 template <int NT, int EMAX, int SIGN>
 __device__ void fft_il(double2 *buf, int S, int n, int npass, unsigned long long code,
                        const double2 *__restrict__ tw, int twmul) {

@@ -52,6 +52,7 @@ __device__ __forceinline__ void put_psi(const Geo &g, double *f, int i, int j, d
 
 // ------------------------------------------------------------ forward A
 
+template <int L>
 __device__ __forceinline__ void pfwd_a(const Geo &g, const Tables &tb, const Fft2Plan &pl,
                                        double gamma, const double *__restrict__ dtp,
                                        const double *__restrict__ ux,
@@ -73,7 +74,8 @@ __device__ __forceinline__ void pfwd_a(const Geo &g, const Tables &tb, const Fft
     buf[e] = z;
   }
   __syncthreads();
-  fft_il<NTF, EMAXF, -1>(buf, S, n1, pl.np1, pl.code1, tw, pl.n / n1);
+  if constexpr (L > 0) fft_p2<NTF, EMAXF, L, -1>(buf, __builtin_ctz(S), tw, pl.n / L);
+  else fft_il<NTF, EMAXF, -1>(buf, S, n1, pl.np1, pl.code1, tw, pl.n / n1);
   for (int e = threadIdx.x; e < S * n1; e += NTF) {
     const int k1 = fS.div(e), s = e - k1 * S;
     const int gg = s / TILE, q = s - gg * TILE;
@@ -89,6 +91,7 @@ __device__ __forceinline__ void pfwd_a(const Geo &g, const Tables &tb, const Fft
 
 // ------------------------------------------------------------ forward B
 
+template <int L>
 __device__ __forceinline__ void pfwd_b(const Geo &g, const Fft2Plan &pl,
                                        const double2 *__restrict__ tw,
                                        const double2 *__restrict__ Y,
@@ -108,7 +111,8 @@ __device__ __forceinline__ void pfwd_b(const Geo &g, const Fft2Plan &pl,
     buf[e] = pair < pl.npair ? Y[(size_t)(k1 * n2 + j2) * pl.npair + pair] : make_double2(0., 0.);
   }
   __syncthreads();
-  fft_il<NTF, EMAXF, -1>(buf, S, n2, pl.np2, pl.code2, tw, n / n2);
+  if constexpr (L > 0) fft_p2<NTF, EMAXF, L, -1>(buf, __builtin_ctz(S), tw, n / L);
+  else fft_il<NTF, EMAXF, -1>(buf, S, n2, pl.np2, pl.code2, tw, n / n2);
   // separate the two real columns of each pair: X_a = (Z_k + conj Z_{n-k})/2,
   // X_b = (Z_k - conj Z_{n-k})/(2i), for the wavenumbers k <= n/2 of this group
   for (int e = threadIdx.x; e < S * n2; e += NTF) {
@@ -142,6 +146,7 @@ __device__ __forceinline__ void pfwd_b(const Geo &g, const Fft2Plan &pl,
 
 // ------------------------------------------------------------ inverse B'
 
+template <int L>
 __device__ __forceinline__ void pinv_b(const Geo &g, const Fft2Plan &pl,
                                        const double2 *__restrict__ tw,
                                        const double2 *__restrict__ psihat,
@@ -171,7 +176,8 @@ __device__ __forceinline__ void pinv_b(const Geo &g, const Fft2Plan &pl,
     buf[e] = upper ? make_double2(xa.x + xb.y, -xa.y + xb.x) : make_double2(xa.x - xb.y, xa.y + xb.x);
   }
   __syncthreads();
-  fft_il<NTF, EMAXF, +1>(buf, S, n2, pl.np2, pl.code2, tw, n / n2);
+  if constexpr (L > 0) fft_p2<NTF, EMAXF, L, +1>(buf, __builtin_ctz(S), tw, n / L);
+  else fft_il<NTF, EMAXF, +1>(buf, S, n2, pl.np2, pl.code2, tw, n / n2);
   for (int e = threadIdx.x; e < S * n2; e += NTF) {
     const int j2 = fS.div(e), s = e - j2 * S;
     const int h = s / TILE, q = s - h * TILE;
@@ -191,6 +197,7 @@ __device__ __forceinline__ void pinv_b(const Geo &g, const Fft2Plan &pl,
 
 // ------------------------------------------------------------ inverse A'
 
+template <int L>
 __device__ __forceinline__ void pinv_a(const Geo &g, const Fft2Plan &pl,
                                        const double2 *__restrict__ tw,
                                        const double2 *__restrict__ Y, double *__restrict__ psi) {
@@ -206,7 +213,8 @@ __device__ __forceinline__ void pinv_a(const Geo &g, const Fft2Plan &pl,
                              : make_double2(0., 0.);
   }
   __syncthreads();
-  fft_il<NTF, EMAXF, +1>(buf, S, n1, pl.np1, pl.code1, tw, pl.n / n1);
+  if constexpr (L > 0) fft_p2<NTF, EMAXF, L, +1>(buf, __builtin_ctz(S), tw, pl.n / L);
+  else fft_il<NTF, EMAXF, +1>(buf, S, n1, pl.np1, pl.code1, tw, pl.n / n1);
   const double njtot = 1. * g.jtot;
   for (int e = threadIdx.x; e < S * n1; e += NTF) {
     const int j1 = fS.div(e), s = e - j1 * S;
@@ -317,16 +325,16 @@ __device__ __forceinline__ void poisson_x3(const Geo &g, const double *__restric
 // Argument source S: ArgOne<PoisArgs> (one instance) or ArgBatch<PoisArgs>
 // (instance blockIdx.z of a batch).
 
-template <class S>
+template <int L, class S>
 __global__ void __launch_bounds__(NTF) k_pfwd_a(S src) {
   const PoisArgs a = src.get();
-  pfwd_a(a.g, a.tb, a.pl, a.gamma, a.dtp, a.ux, a.uy, a.tw, a.Y);
+  pfwd_a<L>(a.g, a.tb, a.pl, a.gamma, a.dtp, a.ux, a.uy, a.tw, a.Y);
 }
 
-template <class S>
+template <int L, class S>
 __global__ void __launch_bounds__(NTF) k_pfwd_b(S src) {
   const PoisArgs a = src.get();
-  pfwd_b(a.g, a.pl, a.tw, a.Y, a.psihat);
+  pfwd_b<L>(a.g, a.pl, a.tw, a.Y, a.psihat);
 }
 
 template <int NT, int M, class S>
@@ -335,16 +343,16 @@ __global__ void __launch_bounds__(NT) k_poisson_x3(S src) {
   poisson_x3<NT, M>(a.g, a.fac, a.plu, NT * M, a.psihat);
 }
 
-template <class S>
+template <int L, class S>
 __global__ void __launch_bounds__(NTF) k_pinv_b(S src) {
   const PoisArgs a = src.get();
-  pinv_b(a.g, a.pl, a.tw, a.psihat, a.Y);
+  pinv_b<L>(a.g, a.pl, a.tw, a.psihat, a.Y);
 }
 
-template <class S>
+template <int L, class S>
 __global__ void __launch_bounds__(NTF) k_pinv_a(S src) {
   const PoisArgs a = src.get();
-  pinv_a(a.g, a.pl, a.tw, a.Y, a.psi);
+  pinv_a<L>(a.g, a.pl, a.tw, a.Y, a.psi);
 }
 
 static constexpr int kPxM = 8;
@@ -445,12 +453,34 @@ static dim3 grid_b(const Fft2Plan &pl) { return dim3((pl.npair + TILE - 1) / TIL
 
 static dim3 with_z(dim3 d, int nz) { return dim3(d.x, d.y, nz); }
 
+// Compile-time length of a pass: the factor itself when it is a power of
+// two in 8..128 and the pass's sequence count is a power of two, else 0 (the
+// runtime radix schedule).
+static int p2_len(int m, int S) {
+  const bool pow2 = (m & (m - 1)) == 0 && (S & (S - 1)) == 0;
+  return pow2 && m >= 8 && m <= 128 ? m : 0;
+}
+
+// Launch kernel template K<L, S> with L the compile-time length of factor m
+#define SNS_P2_LAUNCH(K, L_, grid)                                                        \
+  switch (L_) {                                                                           \
+    case 8: hipLaunchKernelGGL((K<8, S>), grid, dim3(NTF), 0, s, src); break;              \
+    case 16: hipLaunchKernelGGL((K<16, S>), grid, dim3(NTF), 0, s, src); break;            \
+    case 32: hipLaunchKernelGGL((K<32, S>), grid, dim3(NTF), 0, s, src); break;            \
+    case 64: hipLaunchKernelGGL((K<64, S>), grid, dim3(NTF), 0, s, src); break;            \
+    case 128: hipLaunchKernelGGL((K<128, S>), grid, dim3(NTF), 0, s, src); break;          \
+    default: hipLaunchKernelGGL((K<0, S>), grid, dim3(NTF), 0, s, src); break;             \
+  }
+
 template <class S>
 static void poisson_stage(hipStream_t s, const Geo &g, const Fft2Plan &pl, S src, int nz,
                           int stage) {
+  // pass A runs G*TILE sequences of length n1, pass B up to 2*TILE of length n2
+  const int La = p2_len(pl.n1, pl.G * TILE);
+  const int Lb = p2_len(pl.n2, TILE);
   if (stage == 0 || stage < 0) {
-    hipLaunchKernelGGL(k_pfwd_a<S>, with_z(grid_a(pl), nz), dim3(NTF), 0, s, src);
-    hipLaunchKernelGGL(k_pfwd_b<S>, with_z(grid_b(pl), nz), dim3(NTF), 0, s, src);
+    SNS_P2_LAUNCH(k_pfwd_a, La, with_z(grid_a(pl), nz));
+    SNS_P2_LAUNCH(k_pfwd_b, Lb, with_z(grid_b(pl), nz));
   }
   if (stage == 1 || stage < 0) {
     switch (px_threads(g.itot)) {
@@ -462,8 +492,8 @@ static void poisson_stage(hipStream_t s, const Geo &g, const Fft2Plan &pl, S src
     }
   }
   if (stage == 2 || stage < 0) {
-    hipLaunchKernelGGL(k_pinv_b<S>, with_z(grid_b(pl), nz), dim3(NTF), 0, s, src);
-    hipLaunchKernelGGL(k_pinv_a<S>, with_z(grid_a(pl), nz), dim3(NTF), 0, s, src);
+    SNS_P2_LAUNCH(k_pinv_b, Lb, with_z(grid_b(pl), nz));
+    SNS_P2_LAUNCH(k_pinv_a, La, with_z(grid_a(pl), nz));
   }
 }
 
