@@ -50,6 +50,43 @@ struct LuTabLU {
   }
 };
 
+// LuTabLU with this thread's M reciprocal pivots loaded once into registers.
+// Rows pfx+1 .. n-2 all have the pivot rinf (the factorisation found every
+// pivot from row pfx on equal, bit for bit, up to row n-2), so only rows up
+// to pfx and the last row are read from the table: a thread whose chunk
+// holds none of them reads nothing.
+template <int M>
+struct LuTabLUr {
+  double rr[M];
+  const double *l, *u;
+  int n;
+  double up0;
+  __device__ __forceinline__ LuTabLUr(const double *r, const double *l_, const double *u_,
+                                      int n_, double up0_, int pfx, double rinf, int nt)
+      : l(l_), u(u_), n(n_), up0(up0_) {
+    const int row0 = threadIdx.x * M;
+    if (row0 <= pfx || row0 + M - 1 >= n - 1) {
+#pragma unroll
+      for (int k = 0; k < M; k++) {
+        const int row = row0 + k;
+        rr[k] = (row <= pfx || row == n - 1) ? r[k * nt + threadIdx.x] : rinf;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < M; k++) rr[k] = rinf;
+    }
+  }
+  __device__ __forceinline__ double get_r(int k, int) const { return rr[k]; }
+  __device__ __forceinline__ double get_al(int k, int nt) const {
+    const int row = threadIdx.x * M + k;
+    return row == 0 ? 0. : -(rr[k] * l[k * nt + threadIdx.x]);
+  }
+  __device__ __forceinline__ double get_be(int k, int nt) const {
+    const int row = threadIdx.x * M + k;
+    return row == n - 1 ? 0. : (row == 0 ? -up0 : -(rr[k] * u[k * nt + threadIdx.x]));
+  }
+};
+
 // composition of affine maps x -> A x + B: (A2,B2) after (A1,B1)
 __device__ __forceinline__ void aff_after(double &A, double &B, double A1, double B1) {
   B = A * B1 + B;

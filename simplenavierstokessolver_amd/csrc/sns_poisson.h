@@ -18,8 +18,10 @@ bool fft2_plan(int n, int itot, Fft2Plan &pl);
 
 // Thomas factors of the x systems of all wavenumbers (once per instance):
 // fac holds (jtot/2+1) tables of poisson_factor_entries(itot) reciprocal
-// pivots (sns_lu.h layout), then u'_0 of each wavenumber; plu the shared
-// row coefficients l and u in the same layout (poisson_lu_coefs, 2 tables)
+// pivots (sns_lu.h layout), then u'_0 of each wavenumber, the row pfx after
+// which the pivots repeat up to row itot-2 and that pivot (LuTabLUr); plu
+// the shared row coefficients l and u in the same layout (poisson_lu_coefs,
+// 2 tables)
 int poisson_factor_entries(int itot);
 size_t poisson_factor_doubles(int itot, int jtot);
 void poisson_lu_coefs(int itot, const double *pl, const double *pu, double *plu);

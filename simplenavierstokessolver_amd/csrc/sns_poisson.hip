@@ -238,7 +238,11 @@ __device__ __forceinline__ void pinv_a(const Geo &g, const Fft2Plan &pl,
 // reciprocal pivots r_i depend on J; the substitutions need al_i = -(r_i l_i)
 // and be_i = -u'_i = -(r_i u_i) for 0 < i < n-1, so the factor table holds r
 // (interleaved as in sns_lu.h, ntm entries per J) followed by u'_0 = u_0/c_0
-// of every J, and l, u are shared interleaved tables (LuTabLU).
+// of every J, then per J the last row pfx <= n-2 whose pivot differs from
+// the previous row's and that pivot (rows pfx..n-2 share it: on a grid with
+// identical interior rows the recurrence reaches a fixed point, bit for bit,
+// after a few rows for all but the lowest J); l, u are shared interleaved
+// tables (LuTabLUr reads only rows up to pfx and the last row).
 // The J = 0 system is singular; its last equation is replaced by
 // psi_hat(itot) = 0 (r = 0 on that row).  Each solve then needs only the two
 // substitutions, evaluated by lu_solve_shared for the real and imaginary
@@ -268,9 +272,12 @@ __global__ void __launch_bounds__(64)
   double r = 1. / c;
   R[lu_slot(0, NT, M)] = r;
   fac[(size_t)nJ * ntm + J] = up;
+  int pfx = 0;
+  double rpfx = r;
   for (int i = 1; i < n; i++) {
     coef(i, l, c, u);
     const double val = c - l * up;
+    const double rp = r;
     if (fabs(val) < DBL_EPSILON || (J == 0 && i == n - 1)) {
       up = 0.;
       r = 0.;
@@ -279,7 +286,13 @@ __global__ void __launch_bounds__(64)
       up = r * u;
     }
     R[lu_slot(i, NT, M)] = r;
+    if (i <= n - 2 && __double_as_longlong(r) != __double_as_longlong(rp)) {
+      pfx = i;
+      rpfx = r;
+    }
   }
+  fac[(size_t)nJ * ntm + nJ + J] = (double)pfx;
+  fac[(size_t)nJ * ntm + 2 * nJ + J] = rpfx;
 }
 
 template <int NT, int M>
@@ -310,7 +323,9 @@ __device__ __forceinline__ void poisson_x3(const Geo &g, const double *__restric
     d[1][k] = v.y;
   }
   const int nJ = g.jtot / 2 + 1;
-  const LuTabLU tab = {fac + (size_t)J * ntm, plu, plu + ntm, n, M, fac[(size_t)nJ * ntm + J]};
+  const double *tail = fac + (size_t)nJ * ntm;
+  const LuTabLUr<M> tab(fac + (size_t)J * ntm, plu, plu + ntm, n, tail[J], (int)tail[nJ + J],
+                        tail[2 * nJ + J], NT);
   lu_solve_shared<NT, M, 2>(d, tab, L.scr);
 #pragma unroll
   for (int k = 0; k < M; k++) {
@@ -368,7 +383,7 @@ static int px_threads(int n) {
 int poisson_factor_entries(int itot) { return px_threads(itot) * kPxM; }
 
 size_t poisson_factor_doubles(int itot, int jtot) {
-  return (size_t)(jtot / 2 + 1) * (poisson_factor_entries(itot) + 1);
+  return (size_t)(jtot / 2 + 1) * (poisson_factor_entries(itot) + 3);
 }
 
 void poisson_lu_coefs(int itot, const double *pl, const double *pu, double *plu) {
