@@ -59,7 +59,12 @@ __device__ __forceinline__ void pfwd_a(const Geo &g, const Tables &tb, const Fft
                                        const double *__restrict__ uy,
                                        const double2 *__restrict__ tw, double2 *__restrict__ Y) {
   __shared__ double2 buf[EMAXF * NTF];
-  const int tile = blockIdx.x, j20 = blockIdx.y * pl.G;
+  // consecutive tiles of one row group on the same XCD: a tile's last ux
+  // face is the first of the next tile's 128-byte line, which then stays in
+  // that XCD's L2
+  const int nb = gridDim.x * gridDim.y;
+  const int w = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, nb);
+  const int tile = w % gridDim.x, j20 = (w / gridDim.x) * pl.G;
   const int S = pl.G * TILE, n1 = pl.n1, n2 = pl.n2;
   const FastDiv fS(S);
   const double inv_gdt = 1. / (gamma * (*dtp));
