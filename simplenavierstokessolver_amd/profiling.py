@@ -31,6 +31,8 @@ GROUPS = [
     ("k_adv_bound", "dt_reduce"),
     ("k_dt", "dt"), ("k_lu_factor", "lu_factor"), ("k_copy", "copy"),
     ("k_transpose", "ysolve"), ("k_cny_rows", "ysolve"), ("k_add_out", "ysolve"),
+    ("k_ytrans", "ysolve"), ("k_yfactor", "ysolve"), ("k_ylocal", "ysolve"),
+    ("k_yreduced", "ysolve"), ("k_yapply", "ysolve"),
 ]
 
 
