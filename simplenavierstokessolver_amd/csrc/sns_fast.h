@@ -16,6 +16,8 @@
 // order, so every stencil below returns the same bits as its restatement
 // in sns_device.h.
 #pragma once
+#include <stdlib.h>
+#include <string.h>
 #include <hip/hip_runtime.h>
 #include "sns_device.h"
 
@@ -105,6 +107,14 @@ struct Tables {
   __host__ __device__ const double2 *cT2() const { return t2 + 2 * stride; }   // (cT_xm, cT_xp)
   __host__ __device__ const double2 *cU2() const { return t2 + 3 * stride; }   // (cU_xm, cU_xp)
 };
+
+// Host: whether the kernels take the power-of-two division path for these
+// tables (SNSG_EXACT_DIV=1 keeps the division sequence; read at every
+// launch, both paths return the same bits).
+inline bool pow2_division(const Tables &tb) {
+  const char *ex = getenv("SNSG_EXACT_DIV");
+  return tb.pow2 && !(ex && strcmp(ex, "1") == 0);
+}
 
 __device__ __forceinline__ double2 ldr2(const double2 *row, int k) {
   return *reinterpret_cast<const double2 *>(reinterpret_cast<const char *>(row) +

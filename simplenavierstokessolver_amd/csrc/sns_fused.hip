@@ -13,8 +13,6 @@
 // boundary values follow the reference expression for expression (see
 // sns_fast.h for the division scheme); only the line solves round
 // differently.
-#include <stdlib.h>
-#include <string.h>
 #include "sns_launch.h"
 #include "sns_fast.h"
 #include "sns_fused.h"
@@ -103,15 +101,12 @@ __global__ void __launch_bounds__(256) k_pred_exp(S src) {
   emit_field(a, R, 2, i, j, pred_field<P2>(a, R, 2, i, dt));
 }
 
-// Power-of-two divisors select the variant that divides by multiplying;
-// SNSG_EXACT_DIV=1 keeps the exact division sequence (both return the same
-// bits).
+// Power-of-two divisors select the variant that divides by multiplying.
 template <class S>
 static void pred_exp_launch(hipStream_t s, const PredArgs &shape, S src, int nz) {
   const Geo &g = shape.g;
   const dim3 grid((g.itot + 255) / 256, g.jtot, nz);
-  const char *ex = getenv("SNSG_EXACT_DIV");
-  if (shape.tb.pow2 && !(ex && strcmp(ex, "1") == 0))
+  if (pow2_division(shape.tb))
     hipLaunchKernelGGL((k_pred_exp<true, S>), grid, dim3(256), 0, s, src);
   else
     hipLaunchKernelGGL((k_pred_exp<false, S>), grid, dim3(256), 0, s, src);

@@ -28,12 +28,14 @@ static constexpr int TILE = 8;   // complex pairs per tile (16 real columns)
 static constexpr int NTF = 256;  // threads of the FFT passes
 static constexpr int EMAXF = 8;  // complex values per thread and pass
 
+// P2: every divisor a power of two (the quotients are products, see dv)
+template <bool P2>
 __device__ __forceinline__ double poisson_rhs2(const Geo &g, const Tables &tb, const double *ux,
                                                const double *uy, double inv_gdt, int i, int j) {
   // 1./(gamma*dt)*((ux_xp-ux_xm)/DXF(i) + (uy_yp-uy_ym)/dy)
-  const double ax = div_cr(ux[ux_idx(g, i + 1, j)] - ux[ux_idx(g, i, j)], tb.dxf()[i - 1],
+  const double ax = dv<P2>(ux[ux_idx(g, i + 1, j)] - ux[ux_idx(g, i, j)], tb.dxf()[i - 1],
                            tb.rdxf()[i - 1]);
-  const double ay = div_cr(uy[c_idx(g, i, j + 1)] - uy[c_idx(g, i, j)], tb.dy, tb.rdy);
+  const double ay = dv<P2>(uy[c_idx(g, i, j + 1)] - uy[c_idx(g, i, j)], tb.dy, tb.rdy);
   return inv_gdt * (+ax + ay);
 }
 
@@ -52,7 +54,7 @@ __device__ __forceinline__ void put_psi(const Geo &g, double *f, int i, int j, d
 
 // ------------------------------------------------------------ forward A
 
-template <int L>
+template <int L, bool P2>
 __device__ __forceinline__ void pfwd_a(const Geo &g, const Tables &tb, const Fft2Plan &pl,
                                        double gamma, const double *__restrict__ dtp,
                                        const double *__restrict__ ux,
@@ -74,8 +76,8 @@ __device__ __forceinline__ void pfwd_a(const Geo &g, const Tables &tb, const Fft
     const int j = n2 * j1 + j20 + gg;  // 0-based row
     const int ia = 2 * (tile * TILE + q) + 1;
     double2 z;
-    z.x = ia <= g.itot ? poisson_rhs2(g, tb, ux, uy, inv_gdt, ia, j + 1) : 0.;
-    z.y = ia + 1 <= g.itot ? poisson_rhs2(g, tb, ux, uy, inv_gdt, ia + 1, j + 1) : 0.;
+    z.x = ia <= g.itot ? poisson_rhs2<P2>(g, tb, ux, uy, inv_gdt, ia, j + 1) : 0.;
+    z.y = ia + 1 <= g.itot ? poisson_rhs2<P2>(g, tb, ux, uy, inv_gdt, ia + 1, j + 1) : 0.;
     buf[e] = z;
   }
   __syncthreads();
@@ -348,7 +350,14 @@ __device__ __forceinline__ void poisson_x3(const Geo &g, const double *__restric
 template <int L, class S>
 __global__ void __launch_bounds__(NTF) k_pfwd_a(S src) {
   const PoisArgs a = src.get();
-  pfwd_a<L>(a.g, a.tb, a.pl, a.gamma, a.dtp, a.ux, a.uy, a.tw, a.Y);
+  pfwd_a<L, false>(a.g, a.tb, a.pl, a.gamma, a.dtp, a.ux, a.uy, a.tw, a.Y);
+}
+
+// the same with every divisor a power of two
+template <int L, class S>
+__global__ void __launch_bounds__(NTF) k_pfwd_a_p2(S src) {
+  const PoisArgs a = src.get();
+  pfwd_a<L, true>(a.g, a.tb, a.pl, a.gamma, a.dtp, a.ux, a.uy, a.tw, a.Y);
 }
 
 template <int L, class S>
@@ -493,13 +502,14 @@ static int p2_len(int m, int S) {
   }
 
 template <class S>
-static void poisson_stage(hipStream_t s, const Geo &g, const Fft2Plan &pl, S src, int nz,
-                          int stage) {
+static void poisson_stage(hipStream_t s, const Geo &g, const Fft2Plan &pl, bool pow2, S src,
+                          int nz, int stage) {
   // pass A runs G*TILE sequences of length n1, pass B up to 2*TILE of length n2
   const int La = p2_len(pl.n1, pl.G * TILE);
   const int Lb = p2_len(pl.n2, TILE);
   if (stage == 0 || stage < 0) {
-    SNS_P2_LAUNCH(k_pfwd_a, La, with_z(grid_a(pl), nz));
+    if (pow2) SNS_P2_LAUNCH(k_pfwd_a_p2, La, with_z(grid_a(pl), nz))
+    else SNS_P2_LAUNCH(k_pfwd_a, La, with_z(grid_a(pl), nz));
     SNS_P2_LAUNCH(k_pfwd_b, Lb, with_z(grid_b(pl), nz));
   }
   if (stage == 1 || stage < 0) {
@@ -518,12 +528,13 @@ static void poisson_stage(hipStream_t s, const Geo &g, const Fft2Plan &pl, S src
 }
 
 void launch_poisson(hipStream_t s, const PoisArgs &a, int stage) {
-  poisson_stage(s, a.g, a.pl, ArgOne<PoisArgs>{a}, 1, stage);
+  poisson_stage(s, a.g, a.pl, pow2_division(a.tb), ArgOne<PoisArgs>{a}, 1, stage);
 }
 
 void launch_poisson_batch(hipStream_t s, const PoisArgs &shape, const PoisArgs *dev, int nz,
                           int stage) {
-  poisson_stage(s, shape.g, shape.pl, arg_batch<PoisArgs>(dev), nz, stage);
+  poisson_stage(s, shape.g, shape.pl, pow2_division(shape.tb), arg_batch<PoisArgs>(dev), nz,
+                stage);
 }
 
 }  // namespace sns

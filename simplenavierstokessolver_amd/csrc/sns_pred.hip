@@ -14,8 +14,6 @@
 //    values of the columns on either side;
 //  * the y neighbours of one row are the centre values of the other, and the
 //    per-column metrics and the LU tables serve both rows.
-#include <stdlib.h>
-#include <string.h>
 #include "sns_launch.h"
 #include "sns_fast.h"
 #include "sns_fused.h"
@@ -555,15 +553,12 @@ static void pred_rows_select(hipStream_t s, const PredArgs &shape, S src, int nz
   }
 }
 
-// Power-of-two divisors select the kernel that divides by multiplying;
-// SNSG_EXACT_DIV=1 keeps the exact division sequence (read at every launch;
-// both variants return the same bits).
+// Power-of-two divisors select the kernel that divides by multiplying.
 template <class S>
 static void pred_rows_dispatch(hipStream_t s, const PredArgs &shape, S src, int nz) {
   int NT, M;
   row_config(shape.g.itot, NT, M);
-  const char *ex = getenv("SNSG_EXACT_DIV");
-  if (shape.tb.pow2 && !(ex && strcmp(ex, "1") == 0)) pred_rows_select<true>(s, shape, src, nz, NT);
+  if (pow2_division(shape.tb)) pred_rows_select<true>(s, shape, src, nz, NT);
   else pred_rows_select<false>(s, shape, src, nz, NT);
 }
 

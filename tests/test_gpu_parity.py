@@ -379,9 +379,10 @@ POW2_CASES = [
 
 @pytest.mark.parametrize("case", POW2_CASES)
 def test_pow2_division_is_exact(sns, monkeypatch, case):
-    """With every divisor a power of two the predictors divide by multiplying
-    with the exact reciprocal; SNSG_EXACT_DIV=1 selects the exact division
-    sequence instead.  After three steps every field agrees bit for bit."""
+    """With every divisor a power of two the predictors and the Poisson
+    right-hand side divide by multiplying with the exact reciprocal;
+    SNSG_EXACT_DIV=1 selects the exact division sequence instead.  After
+    three steps every field agrees bit for bit."""
     import math
     itot, jtot, imx, imy, temp, forcing = case
     out = []
