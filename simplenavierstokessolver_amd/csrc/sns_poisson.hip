@@ -54,7 +54,43 @@ __device__ __forceinline__ void put_psi(const Geo &g, double *f, int i, int j, d
 
 // ------------------------------------------------------------ forward A
 
-template <int L, bool P2>
+// RHS of the column pair (ia, ia+1), ia odd, at 0-based row j in 16-byte
+// loads: ux faces ia, ia+1 (one pair), ia+2; uy of rows j+1 and j+2 (one
+// pair each).  Needs the padded state layout (ux rows and uy's column 1 on
+// 16-byte boundaries, even pitches); columns past itot give 0.
+template <bool P2>
+__device__ __forceinline__ double2 poisson_rhs_pair(const Geo &g, const Tables &tb,
+                                                    const double *ux, const double *uy,
+                                                    double inv_gdt, int ia, int j) {
+  const int itot = g.itot;
+  const int ic = ia <= itot ? ia : 1;  // lanes past the last column read column 1
+  const double *urow = ux + (size_t)(j + 1) * g.pu;
+  const double2 u01 = ldr2(reinterpret_cast<const double2 *>(urow + ic - 1), 0);
+  const double u2 = ldr(urow, min(ic + 1, itot));  // face ic+2 (only used when ic+1 <= itot)
+  const double2 v0 = ldr2(reinterpret_cast<const double2 *>(uy + (size_t)(j + 1) * g.pc + ic), 0);
+  const double2 v1 = ldr2(reinterpret_cast<const double2 *>(uy + (size_t)(j + 2) * g.pc + ic), 0);
+  double dxa, rdxa, dxb, rdxb;
+  if (tb.uniform) {
+    dxa = dxb = tb.u_dxf;
+    rdxa = rdxb = tb.u_rdxf;
+  } else {
+    dxa = tb.dxf()[ic - 1];
+    rdxa = tb.rdxf()[ic - 1];
+    dxb = tb.dxf()[min(ic, itot - 1)];
+    rdxb = tb.rdxf()[min(ic, itot - 1)];
+  }
+  // the expressions of poisson_rhs2, column by column
+  const double axa = dv<P2>(u01.y - u01.x, dxa, rdxa);
+  const double aya = dv<P2>(v1.x - v0.x, tb.dy, tb.rdy);
+  const double axb = dv<P2>(u2 - u01.y, dxb, rdxb);
+  const double ayb = dv<P2>(v1.y - v0.y, tb.dy, tb.rdy);
+  double2 z;
+  z.x = ia <= itot ? inv_gdt * (+axa + aya) : 0.;
+  z.y = ia + 1 <= itot ? inv_gdt * (+axb + ayb) : 0.;
+  return z;
+}
+
+template <int L, bool P2, bool AL>
 __device__ __forceinline__ void pfwd_a(const Geo &g, const Tables &tb, const Fft2Plan &pl,
                                        double gamma, const double *__restrict__ dtp,
                                        const double *__restrict__ ux,
@@ -76,8 +112,12 @@ __device__ __forceinline__ void pfwd_a(const Geo &g, const Tables &tb, const Fft
     const int j = n2 * j1 + j20 + gg;  // 0-based row
     const int ia = 2 * (tile * TILE + q) + 1;
     double2 z;
-    z.x = ia <= g.itot ? poisson_rhs2<P2>(g, tb, ux, uy, inv_gdt, ia, j + 1) : 0.;
-    z.y = ia + 1 <= g.itot ? poisson_rhs2<P2>(g, tb, ux, uy, inv_gdt, ia + 1, j + 1) : 0.;
+    if constexpr (AL) {
+      z = poisson_rhs_pair<P2>(g, tb, ux, uy, inv_gdt, ia, j);
+    } else {
+      z.x = ia <= g.itot ? poisson_rhs2<P2>(g, tb, ux, uy, inv_gdt, ia, j + 1) : 0.;
+      z.y = ia + 1 <= g.itot ? poisson_rhs2<P2>(g, tb, ux, uy, inv_gdt, ia + 1, j + 1) : 0.;
+    }
     buf[e] = z;
   }
   __syncthreads();
@@ -350,14 +390,20 @@ __device__ __forceinline__ void poisson_x3(const Geo &g, const double *__restric
 template <int L, class S>
 __global__ void __launch_bounds__(NTF) k_pfwd_a(S src) {
   const PoisArgs a = src.get();
-  pfwd_a<L, false>(a.g, a.tb, a.pl, a.gamma, a.dtp, a.ux, a.uy, a.tw, a.Y);
+  pfwd_a<L, false, false>(a.g, a.tb, a.pl, a.gamma, a.dtp, a.ux, a.uy, a.tw, a.Y);
 }
 
-// the same with every divisor a power of two
+// the same on the padded state layout (pair loads), with the division
+// sequence or, P2, with power-of-two divisors
+template <int L, class S>
+__global__ void __launch_bounds__(NTF) k_pfwd_a_al(S src) {
+  const PoisArgs a = src.get();
+  pfwd_a<L, false, true>(a.g, a.tb, a.pl, a.gamma, a.dtp, a.ux, a.uy, a.tw, a.Y);
+}
 template <int L, class S>
 __global__ void __launch_bounds__(NTF) k_pfwd_a_p2(S src) {
   const PoisArgs a = src.get();
-  pfwd_a<L, true>(a.g, a.tb, a.pl, a.gamma, a.dtp, a.ux, a.uy, a.tw, a.Y);
+  pfwd_a<L, true, true>(a.g, a.tb, a.pl, a.gamma, a.dtp, a.ux, a.uy, a.tw, a.Y);
 }
 
 template <int L, class S>
@@ -502,13 +548,14 @@ static int p2_len(int m, int S) {
   }
 
 template <class S>
-static void poisson_stage(hipStream_t s, const Geo &g, const Fft2Plan &pl, bool pow2, S src,
-                          int nz, int stage) {
+static void poisson_stage(hipStream_t s, const Geo &g, const Fft2Plan &pl, bool pow2,
+                          bool aligned, S src, int nz, int stage) {
   // pass A runs G*TILE sequences of length n1, pass B up to 2*TILE of length n2
   const int La = p2_len(pl.n1, pl.G * TILE);
   const int Lb = p2_len(pl.n2, TILE);
   if (stage == 0 || stage < 0) {
-    if (pow2) SNS_P2_LAUNCH(k_pfwd_a_p2, La, with_z(grid_a(pl), nz))
+    if (aligned && pow2) SNS_P2_LAUNCH(k_pfwd_a_p2, La, with_z(grid_a(pl), nz))
+    else if (aligned) SNS_P2_LAUNCH(k_pfwd_a_al, La, with_z(grid_a(pl), nz))
     else SNS_P2_LAUNCH(k_pfwd_a, La, with_z(grid_a(pl), nz));
     SNS_P2_LAUNCH(k_pfwd_b, Lb, with_z(grid_b(pl), nz));
   }
@@ -527,14 +574,22 @@ static void poisson_stage(hipStream_t s, const Geo &g, const Fft2Plan &pl, bool 
   }
 }
 
+// The padded state layout: even row pitches, ux rows and uy's column 1 on
+// 16-byte boundaries (every instance of a batch is allocated alike).
+static bool pair_layout(const PoisArgs &a) {
+  auto al16 = [](const void *p) { return ((uintptr_t)p & 15u) == 0; };
+  return a.g.pu % 2 == 0 && a.g.pc % 2 == 0 && al16(a.ux) && al16(a.uy + 1);
+}
+
 void launch_poisson(hipStream_t s, const PoisArgs &a, int stage) {
-  poisson_stage(s, a.g, a.pl, pow2_division(a.tb), ArgOne<PoisArgs>{a}, 1, stage);
+  poisson_stage(s, a.g, a.pl, pow2_division(a.tb), pair_layout(a), ArgOne<PoisArgs>{a}, 1,
+                stage);
 }
 
 void launch_poisson_batch(hipStream_t s, const PoisArgs &shape, const PoisArgs *dev, int nz,
                           int stage) {
-  poisson_stage(s, shape.g, shape.pl, pow2_division(shape.tb), arg_batch<PoisArgs>(dev), nz,
-                stage);
+  poisson_stage(s, shape.g, shape.pl, pow2_division(shape.tb), pair_layout(shape),
+                arg_batch<PoisArgs>(dev), nz, stage);
 }
 
 }  // namespace sns

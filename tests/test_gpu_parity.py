@@ -332,6 +332,8 @@ FUSED_CASES = [
     (4200, 8, 100.0, 1e11, 0.7, 1, 0),
     (512, 32, 512.0, 1e9, 1.0, 1, 0),   # uniform grid: interior waves skip the column tables
     (4096, 8, 4096.0, 1e11, 0.7, 1, 0),
+    (37, 64, 3.0, 1e8, 1.0, 1, 0),      # odd itot: the last column pair has one column
+    (45, 32, 100.0, 1e7, 1.0, 0, 0),
 ]
 
 
