@@ -206,21 +206,34 @@ __device__ __forceinline__ void pinv_b(const Geo &g, const Fft2Plan &pl,
   const int S = nh * TILE;
   const FastDiv fS(S);
   // Hermitian extension of the two real columns' spectra packed as
-  // Z_k = X_a,k + i X_b,k (src/fluid/compute_potential.c:147-149: c2r input)
+  // Z_k = X_a,k + i X_b,k (src/fluid/compute_potential.c:147-149: c2r input).
+  // Each stored wavenumber m <= n/2 is read once and fills the slots of k = m
+  // and of its mirror n-m, which this workgroup holds too (row k1 pairs with
+  // row n1-k1; the partner index is the one pass B separates with).
   for (int e = threadIdx.x; e < S * n2; e += NTF) {
     const int k2 = fS.div(e), s = e - k2 * S;
     const int h = s / TILE, q = s - h * TILE;
     const int k1 = h ? k1b : k1a;
     const int k = k1 + n1 * k2;
+    if (2 * k > n) continue;  // filled from its mirror
     const int pair = tile * TILE + q;
     const int ia = 2 * pair + 1;
-    const bool upper = 2 * k > n;
-    const int m = upper ? n - k : k;
     double2 xa = make_double2(0., 0.), xb = make_double2(0., 0.);
-    if (ia <= g.itot) xa = psihat[(size_t)m * g.itot + (ia - 1)];
-    if (ia + 1 <= g.itot) xb = psihat[(size_t)m * g.itot + ia];
-    if (m == 0 || 2 * m == n) { xa.y = 0.; xb.y = 0.; }
-    buf[e] = upper ? make_double2(xa.x + xb.y, -xa.y + xb.x) : make_double2(xa.x - xb.y, xa.y + xb.x);
+    if (ia <= g.itot) xa = psihat[(size_t)k * g.itot + (ia - 1)];
+    if (ia + 1 <= g.itot) xb = psihat[(size_t)k * g.itot + ia];
+    if (k == 0 || 2 * k == n) { xa.y = 0.; xb.y = 0.; }
+    buf[e] = make_double2(xa.x - xb.y, xa.y + xb.x);
+    if (k != 0 && 2 * k != n) {
+      int k2p, hp;
+      if (k1 == 0) {
+        k2p = (n2 - k2) % n2;
+        hp = h;
+      } else {
+        k2p = n2 - 1 - k2;
+        hp = nh == 1 ? h : 1 - h;
+      }
+      buf[k2p * S + hp * TILE + q] = make_double2(xa.x + xb.y, -xa.y + xb.x);
+    }
   }
   __syncthreads();
   if constexpr (L > 0) fft_p2<NTF, EMAXF, L, +1>(buf, __builtin_ctz(S), tw, n / L);
