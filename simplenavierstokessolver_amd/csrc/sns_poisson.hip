@@ -257,7 +257,7 @@ __device__ __forceinline__ void pinv_b(const Geo &g, const Fft2Plan &pl,
 
 // ------------------------------------------------------------ inverse A'
 
-template <int L>
+template <int L, bool AL>
 __device__ __forceinline__ void pinv_a(const Geo &g, const Fft2Plan &pl,
                                        const double2 *__restrict__ tw,
                                        const double2 *__restrict__ Y, double *__restrict__ psi) {
@@ -275,15 +275,26 @@ __device__ __forceinline__ void pinv_a(const Geo &g, const Fft2Plan &pl,
   __syncthreads();
   if constexpr (L > 0) fft_p2<NTF, EMAXF, L, +1>(buf, __builtin_ctz(S), tw, pl.n / L);
   else fft_il<NTF, EMAXF, +1>(buf, S, n1, pl.np1, pl.code1, tw, pl.n / n1);
-  const double njtot = 1. * g.jtot;
+  // psi = q/jtot: the exact division of div_cr, or (jtot a power of two) the
+  // product with the exact reciprocal; both equal IEEE q/jtot bit for bit
+  const double njtot = 1. * g.jtot, rnj = 1. / njtot;
+  const bool jp2 = (g.jtot & (g.jtot - 1)) == 0;
   for (int e = threadIdx.x; e < S * n1; e += NTF) {
     const int j1 = fS.div(e), s = e - j1 * S;
     const int gg = s / TILE, q = s - gg * TILE;
     const int j = n2 * j1 + j20 + gg + 1;  // 1-based row
     const int ia = 2 * (tile * TILE + q) + 1;
     const double2 z = buf[e];
-    if (ia <= g.itot) put_psi(g, psi, ia, j, z.x / njtot);
-    if (ia + 1 <= g.itot) put_psi(g, psi, ia + 1, j, z.y / njtot);
+    const double va = jp2 ? z.x * rnj : div_cr(z.x, njtot, rnj);
+    const double vb = jp2 ? z.y * rnj : div_cr(z.y, njtot, rnj);
+    const bool edge = j == 1 || j == g.jtot || ia == 1 || ia + 1 >= g.itot;
+    if (AL && !edge) {
+      // both columns in one 16-byte store (no halo or wall copies here)
+      *reinterpret_cast<double2 *>(psi + c_idx(g, ia, j)) = make_double2(va, vb);
+    } else {
+      if (ia <= g.itot) put_psi(g, psi, ia, j, va);
+      if (ia + 1 <= g.itot) put_psi(g, psi, ia + 1, j, vb);
+    }
   }
 }
 
@@ -437,10 +448,17 @@ __global__ void __launch_bounds__(NTF) k_pinv_b(S src) {
   pinv_b<L>(a.g, a.pl, a.tw, a.psihat, a.Y);
 }
 
+// the same on the padded state layout (psi's column 1 on a 16-byte boundary)
+template <int L, class S>
+__global__ void __launch_bounds__(NTF) k_pinv_a_al(S src) {
+  const PoisArgs a = src.get();
+  pinv_a<L, true>(a.g, a.pl, a.tw, a.Y, a.psi);
+}
+
 template <int L, class S>
 __global__ void __launch_bounds__(NTF) k_pinv_a(S src) {
   const PoisArgs a = src.get();
-  pinv_a<L>(a.g, a.pl, a.tw, a.Y, a.psi);
+  pinv_a<L, false>(a.g, a.pl, a.tw, a.Y, a.psi);
 }
 
 static constexpr int kPxM = 8;
@@ -583,15 +601,17 @@ static void poisson_stage(hipStream_t s, const Geo &g, const Fft2Plan &pl, bool 
   }
   if (stage == 2 || stage < 0) {
     SNS_P2_LAUNCH(k_pinv_b, Lb, with_z(grid_b(pl), nz));
-    SNS_P2_LAUNCH(k_pinv_a, La, with_z(grid_a(pl), nz));
+    if (aligned) SNS_P2_LAUNCH(k_pinv_a_al, La, with_z(grid_a(pl), nz))
+    else SNS_P2_LAUNCH(k_pinv_a, La, with_z(grid_a(pl), nz));
   }
 }
 
-// The padded state layout: even row pitches, ux rows and uy's column 1 on
-// 16-byte boundaries (every instance of a batch is allocated alike).
+// The padded state layout: even row pitches, ux rows and the column 1 of uy
+// and psi on 16-byte boundaries (every instance of a batch is allocated
+// alike).
 static bool pair_layout(const PoisArgs &a) {
   auto al16 = [](const void *p) { return ((uintptr_t)p & 15u) == 0; };
-  return a.g.pu % 2 == 0 && a.g.pc % 2 == 0 && al16(a.ux) && al16(a.uy + 1);
+  return a.g.pu % 2 == 0 && a.g.pc % 2 == 0 && al16(a.ux) && al16(a.uy + 1) && al16(a.psi + 1);
 }
 
 void launch_poisson(hipStream_t s, const PoisArgs &a, int stage) {
