@@ -692,7 +692,7 @@ extern "C" int snsg_instance_bytes(const snsg_config *cfg, size_t *bytes) {
   if (cfg->implicity) {
     n += d * (2 * itot + itot - 1) * jtot;
     const int L = ysolve_chunk((int)jtot);
-    if (L) n += d * 9 * ysolve_table_doubles(L) + d * (jtot / L) * itot;
+    if (L) n += d * 9 * ysolve_table_doubles(L) + 3 * d * (jtot / L) * itot;
   }
   if (cfg->implicitx) n += d * 27 * (size_t)lu_table_entries((int)itot);
   Geo g{};
@@ -837,7 +837,8 @@ int step_prepare(snsg_ctx *c) {
     c->ychunk = ysolve_chunk(jtot);
     if (c->ychunk) {
       const size_t nt = 9 * ysolve_table_doubles(c->ychunk);
-      const size_t nz = (size_t)(jtot / c->ychunk) * itot;
+      // interface values and the first and last interior values of each chunk
+      const size_t nz = 3 * (size_t)(jtot / c->ychunk) * itot;
       if ((rc = ctx_alloc((void **)&c->d_yfac, sizeof(double) * nt, &c->mem))) return rc;
       if ((rc = ctx_alloc((void **)&c->d_qz, sizeof(double) * nz, &c->mem))) return rc;
     }

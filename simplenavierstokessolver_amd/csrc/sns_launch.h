@@ -70,8 +70,9 @@ struct YArgs {
   double *qf, *qy;
   double *out;
   // partitioned solve on the row-major increments (sns_ysolve.hip) when L > 0:
-  // S chunks of L rows, the factor table of this substage and field, and the
-  // interface values (S rows of n)
+  // S chunks of L rows, the factor table of this substage and field, and
+  // 3S rows of n: the interface values, then the first and the last interior
+  // value of each chunk
   int L, S;
   const double *yfac;
   double *qz;

@@ -18,10 +18,15 @@
 //     solved per column as the reference solves periodic systems
 //     (src/linalg.c:52-108): the (S-1)-row block with the right-hand side
 //     and with [-O, 0, .., 0, -O], then the last unknown, then the rest.
-// Three kernels, each a pass over the row-major increments: k_ylocal (the
-// interior solves, y in place), k_yreduced (the interface system, one wave
-// per column, into qz), k_yapply (the corrections and out = in + x with
-// boundary values).  The factors of T and of the reduced system depend on
+// Three kernels: k_ylocal (the interior solves; the first and last interior
+// values of each chunk go to qz rows S..3S-1), k_yreduced (the interface
+// system, one wave per column, into qz rows 0..S-1), k_yapply (the
+// corrections and out = in + x with boundary values).  When there are
+// enough chunk columns to fill the GPU, k_yapply solves the interiors again,
+// bit for bit the same, instead of reading them back: a few flops per cell
+// for 32 instead of 40 bytes per cell.  Otherwise k_ylocal also stores the
+// interior solution and k_yapply_rows applies it row by row (one thread per
+// cell, which keeps small instances busy).  The factors of T and of the reduced system depend on
 // dt; k_yfactor computes them once per step for the three substages and
 // fields.  Rounding differs from the reference's sequential sweep at the
 // ulp level.
@@ -148,19 +153,13 @@ __global__ void __launch_bounds__(64) k_yfactor(S src) {
 
 // ------------------------------------------------------ interior solves
 
-// Wave w of the workgroup: chunk blockIdx.y*4 + w; lane: column.  The L-1
-// interior rows of the chunk are solved in registers and written back.
-template <class S>
-__global__ void __launch_bounds__(256) k_ylocal(S src) {
-  const YArgs a = src.get();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int s = blockIdx.y * 4 + wave, col = blockIdx.x * 64 + lane;
-  if (s >= a.S || col >= a.n) return;
+// Interior solve of chunk s, column col: the L-1 rows in x (Dirichlet
+// block, shared factors), from the increments qf.
+__device__ __forceinline__ void ychunk_solve(const YArgs &a, const YTab &tb, int s, int col,
+                                             double (&x)[kYMaxL - 1]) {
   const int L1 = a.L - 1;
-  const YTab tb{a.yfac, L1};
   const double lam = tb.lambda();
-  double *q = a.qf + ((size_t)s * a.L + 1) * a.n + col;
-  double x[kYMaxL - 1];
+  const double *q = a.qf + ((size_t)s * a.L + 1) * a.n + col;
 #pragma unroll
   for (int m = 0; m < kYMaxL - 1; m++)
     if (m < L1) x[m] = q[(size_t)m * a.n];
@@ -174,9 +173,33 @@ __global__ void __launch_bounds__(256) k_ylocal(S src) {
 #pragma unroll
   for (int m = kYMaxL - 3; m >= 0; m--)
     if (m < L1 - 1) x[m] = x[m] - tb.beta()[m] * x[m + 1];
+}
+
+// Wave w of the workgroup: chunk blockIdx.y*4 + w; lane: column.  The
+// chunk's first and last interior values go to qz rows S+s and 2S+s; STORE:
+// the interior solution also replaces the increments.
+template <bool STORE, class S>
+__global__ void __launch_bounds__(256) k_ylocal(S src) {
+  const YArgs a = src.get();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int s = blockIdx.y * 4 + wave, col = blockIdx.x * 64 + lane;
+  if (s >= a.S || col >= a.n) return;
+  const int L1 = a.L - 1;
+  const YTab tb{a.yfac, L1};
+  double x[kYMaxL - 1];
+  ychunk_solve(a, tb, s, col, x);
+  double last = x[0];
 #pragma unroll
-  for (int m = 0; m < kYMaxL - 1; m++)
-    if (m < L1) q[(size_t)m * a.n] = x[m];
+  for (int m = 1; m < kYMaxL - 1; m++)
+    if (m == L1 - 1) last = x[m];
+  a.qz[(size_t)(a.S + s) * a.n + col] = x[0];
+  a.qz[(size_t)(2 * a.S + s) * a.n + col] = last;
+  if (STORE) {
+    double *q = a.qf + ((size_t)s * a.L + 1) * a.n + col;
+#pragma unroll
+    for (int m = 0; m < kYMaxL - 1; m++)
+      if (m < L1) q[(size_t)m * a.n] = x[m];
+  }
 }
 
 // -------------------------------------------------- interface system
@@ -193,11 +216,12 @@ __global__ void __launch_bounds__(kYRedNT) k_yreduced(S src) {
   const YTab tb{a.yfac, L - 1};
   const double lam = tb.lambda(), O = tb.O();
   const double *q = a.qf + col;
+  const double *yfirst = a.qz + (size_t)a.S * n + col, *ylast = a.qz + (size_t)2 * a.S * n + col;
   // right-hand sides of the interface rows (chunk k: row kL; its neighbours
   // are the last interior row of chunk k-1 and the first of chunk k)
   auto rhs = [&](int k) {
-    const int rl = k == 0 ? a.S * L - 1 : k * L - 1;
-    return q[(size_t)k * L * n] - lam * (q[(size_t)rl * n] + q[((size_t)k * L + 1) * n]);
+    const int kl = k == 0 ? a.S - 1 : k - 1;
+    return q[(size_t)k * L * n] - lam * (ylast[(size_t)kl * n] + yfirst[(size_t)k * n]);
   };
   double d[1][kYRedM];
 #pragma unroll
@@ -230,16 +254,27 @@ __global__ void __launch_bounds__(kYRedNT) k_yreduced(S src) {
 
 // ------------------------------------------------------------- apply
 
-// Row blockIdx.y (j = row+1): x = p_s on an interface row, else the interior
-// solution with the spikes of both interfaces; out = in + x with boundary
-// values.
-template <int KIND, class S>
-__global__ void __launch_bounds__(256) k_yapply(S src) {
-  const YArgs a = src.get();
+// Chunk s = blockIdx.y*4 + wave, column ii = lane: x = p_s on the interface
+// row, the interior solution (solved again) with the spikes of both
+// interfaces on the others; out = in + x with boundary values, row j = row+1.
+template <int KIND>
+__device__ __forceinline__ void yput(const YArgs &a, int ii, int row, double x) {
   const Geo &g = a.g;
+  const int i = ii + (KIND == KIND_UX ? 2 : 1), j = row + 1;
+  if (KIND == KIND_UX) put_ux(g, a.out, i, j, a.in[ux_idx(g, i, j)] + x);
+  else if (KIND == KIND_UY) put_c<0>(g, a.out, i, j, a.in[c_idx(g, i, j)] + x);
+  else put_c<2>(g, a.out, i, j, a.in[c_idx(g, i, j)] + x);
+}
+
+// Row blockIdx.y (j = row+1) after k_ylocal<true>: x = p_s on an interface
+// row, else the stored interior solution with the spikes of both
+// interfaces.
+template <int KIND, class S>
+__global__ void __launch_bounds__(256) k_yapply_rows(S src) {
+  const YArgs a = src.get();
   const int n = a.n;
   const int ii = blockIdx.x * blockDim.x + threadIdx.x;
-  const int row = blockIdx.y, j = row + 1;
+  const int row = blockIdx.y;
   if (ii >= n) return;
   const int s = row / a.L, m = row - s * a.L;
   const int s1 = s + 1 == a.S ? 0 : s + 1;
@@ -254,24 +289,61 @@ __global__ void __launch_bounds__(256) k_yapply(S src) {
     const double y = a.qf[(size_t)row * n + ii];
     x = y - lam * zs * tb.v()[m - 1] - lam * zn * tb.v()[a.L - 1 - m];
   }
-  const int i = ii + (KIND == KIND_UX ? 2 : 1);
-  if (KIND == KIND_UX) put_ux(g, a.out, i, j, a.in[ux_idx(g, i, j)] + x);
-  else if (KIND == KIND_UY) put_c<0>(g, a.out, i, j, a.in[c_idx(g, i, j)] + x);
-  else put_c<2>(g, a.out, i, j, a.in[c_idx(g, i, j)] + x);
+  yput<KIND>(a, ii, row, x);
+}
+
+template <int KIND, class S>
+__global__ void __launch_bounds__(256) k_yapply(S src) {
+  const YArgs a = src.get();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int s = blockIdx.y * 4 + wave, ii = blockIdx.x * 64 + lane;
+  if (s >= a.S || ii >= a.n) return;
+  const int n = a.n, L1 = a.L - 1;
+  const int s1 = s + 1 == a.S ? 0 : s + 1;
+  const YTab tb{a.yfac, L1};
+  const double lam = tb.lambda();
+  const double zs = a.qz[(size_t)s * n + ii], zn = a.qz[(size_t)s1 * n + ii];
+  double x[kYMaxL - 1];
+  ychunk_solve(a, tb, s, ii, x);
+  yput<KIND>(a, ii, s * a.L, zs);
+#pragma unroll
+  for (int m = 1; m < kYMaxL; m++)
+    if (m <= L1)
+      yput<KIND>(a, ii, s * a.L + m,
+                 x[m - 1] - lam * zs * tb.v()[m - 1] - lam * zn * tb.v()[a.L - 1 - m]);
 }
 
 // ------------------------------------------------------------- launch
 
+// Chunk columns (threads of the per-chunk kernels) from which the interiors
+// are solved twice rather than stored: 4096 waves.  SNSG_YSOLVE_RECOMPUTE=1
+// or =0 forces either way (both return the same bits).
+constexpr size_t kYRecompute = (size_t)4096 * 64;
+
+static bool y_recompute(size_t columns) {
+  const char *e = getenv("SNSG_YSOLVE_RECOMPUTE");
+  if (e && (e[0] == '0' || e[0] == '1') && e[1] == 0) return e[0] == '1';
+  return columns >= kYRecompute;
+}
+
 template <class S>
 static void ypart(hipStream_t s, int kind, const YArgs &shape, S src, int nz) {
   const int n = shape.n;
-  hipLaunchKernelGGL(k_ylocal<S>, dim3((n + 63) / 64, (shape.S + 3) / 4, nz), dim3(256), 0, s,
-                     src);
+  const bool recompute = y_recompute((size_t)n * shape.S * nz);
+  const dim3 gc((n + 63) / 64, (shape.S + 3) / 4, nz);
+  if (recompute) hipLaunchKernelGGL((k_ylocal<false, S>), gc, dim3(256), 0, s, src);
+  else hipLaunchKernelGGL((k_ylocal<true, S>), gc, dim3(256), 0, s, src);
   hipLaunchKernelGGL(k_yreduced<S>, dim3(n, 1, nz), dim3(kYRedNT), 0, s, src);
-  const dim3 gr((n + 255) / 256, shape.g.jtot, nz);
-  if (kind == KIND_UX) hipLaunchKernelGGL((k_yapply<KIND_UX, S>), gr, dim3(256), 0, s, src);
-  else if (kind == KIND_UY) hipLaunchKernelGGL((k_yapply<KIND_UY, S>), gr, dim3(256), 0, s, src);
-  else hipLaunchKernelGGL((k_yapply<KIND_TEMP, S>), gr, dim3(256), 0, s, src);
+  if (recompute) {
+    if (kind == KIND_UX) hipLaunchKernelGGL((k_yapply<KIND_UX, S>), gc, dim3(256), 0, s, src);
+    else if (kind == KIND_UY) hipLaunchKernelGGL((k_yapply<KIND_UY, S>), gc, dim3(256), 0, s, src);
+    else hipLaunchKernelGGL((k_yapply<KIND_TEMP, S>), gc, dim3(256), 0, s, src);
+  } else {
+    const dim3 gr((n + 255) / 256, shape.g.jtot, nz);
+    if (kind == KIND_UX) hipLaunchKernelGGL((k_yapply_rows<KIND_UX, S>), gr, dim3(256), 0, s, src);
+    else if (kind == KIND_UY) hipLaunchKernelGGL((k_yapply_rows<KIND_UY, S>), gr, dim3(256), 0, s, src);
+    else hipLaunchKernelGGL((k_yapply_rows<KIND_TEMP, S>), gr, dim3(256), 0, s, src);
+  }
 }
 
 void launch_ypart(hipStream_t s, int kind, const YArgs &a) {

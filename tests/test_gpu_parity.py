@@ -411,6 +411,33 @@ def test_pow2_division_is_exact(sns, monkeypatch, case):
         assert_bitwise(out[0][n], out[1][n], n)
 
 
+@pytest.mark.parametrize("itot,jtot,imx", [(64, 256, 1), (48, 960, 0), (130, 512, 1)])
+def test_ysolve_recompute_matches_store(sns, monkeypatch, itot, jtot, imx):
+    """The partitioned y solve either stores the chunk interiors and reads
+    them back, or (when there are enough chunk columns) solves them again in
+    the apply kernel; SNSG_YSOLVE_RECOMPUTE forces either way.  After three
+    steps every field agrees bit for bit."""
+    out = []
+    for force in ("0", "1"):
+        monkeypatch.setenv("SNSG_YSOLVE_RECOMPUTE", force)
+        o = po.Oracle(itot, jtot, stretch=3.0, Ra=1e9, Pr=0.7)
+        o.init_temperature(23, 1e-2, linear=False)
+        o.apply_boundaries()
+        inst = sns.Instance(itot, jtot, stretch=3.0, Ra=1e9, Pr=0.7, implicitx=imx, implicity=1)
+        inst.state_alloc()
+        rng = np.random.default_rng(6)
+        for n in ["ux", "uy", "p", "psi", "temp"]:
+            f = o.f[n]
+            if n in ("ux", "uy"):
+                f = f + 1e-2 * rng.standard_normal(f.shape)
+            inst.state_upload(n, f)
+        inst.advance(3)
+        out.append({n: inst.state_download(n) for n in ["ux", "uy", "p", "temp"]})
+        inst.close()
+    for n in out[0]:
+        assert_bitwise(out[0][n], out[1][n], n)
+
+
 @pytest.mark.parametrize("case", STEP_CASES)
 def test_advance_matches_oracle_time_loop(sns, pinned_gauge, case):
     """snsg_advance (time step decided on the device) against the oracle's
