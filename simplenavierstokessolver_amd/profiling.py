@@ -32,7 +32,7 @@ GROUPS = [
     ("k_dt", "dt"), ("k_lu_factor", "lu_factor"), ("k_copy", "copy"),
     ("k_transpose", "ysolve"), ("k_cny_rows", "ysolve"), ("k_add_out", "ysolve"),
     ("k_ytrans", "ysolve"), ("k_yfactor", "ysolve"), ("k_ylocal", "ysolve"),
-    ("k_yreduced", "ysolve"), ("k_yapply", "ysolve"),
+    ("k_yreduced", "ysolve"), ("k_yapply", "ysolve"), ("k_yapply_rows", "ysolve"),
 ]
 
 
@@ -42,9 +42,11 @@ def short_name(full):
 
 
 def group_of(full):
+    """Group of a kernel: its name, or a variant of it (name + "_" + suffix,
+    e.g. k_pfwd_a_p2, the power-of-two variant of k_pfwd_a)."""
     s = short_name(full)
     for k, g in GROUPS:
-        if s == k:
+        if s == k or s.startswith(k + "_"):
             return g
     return s
 
