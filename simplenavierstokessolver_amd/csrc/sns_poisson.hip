@@ -25,8 +25,8 @@
 namespace sns {
 
 static constexpr int TILE = 8;   // complex pairs per tile (16 real columns)
-static constexpr int NTF = 256;  // threads of the FFT passes
-static constexpr int EMAXF = 8;  // complex values per thread and pass
+static constexpr int NTF = 512;  // threads of the FFT passes
+static constexpr int EMAXF = 4;  // complex values per thread and pass
 
 // P2: every divisor a power of two (the quotients are products, see dv)
 template <bool P2>
