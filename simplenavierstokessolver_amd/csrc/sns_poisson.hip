@@ -431,7 +431,7 @@ __global__ void __launch_bounds__(NTF) k_pfwd_a_p2(S src) {
 }
 
 template <int L, class S>
-__global__ void __launch_bounds__(NTF) k_pfwd_b(S src) {
+__global__ void __launch_bounds__(NTF, 8) k_pfwd_b(S src) {
   const PoisArgs a = src.get();
   pfwd_b<L>(a.g, a.pl, a.tw, a.Y, a.psihat);
 }
@@ -450,13 +450,13 @@ __global__ void __launch_bounds__(NTF) k_pinv_b(S src) {
 
 // the same on the padded state layout (psi's column 1 on a 16-byte boundary)
 template <int L, class S>
-__global__ void __launch_bounds__(NTF) k_pinv_a_al(S src) {
+__global__ void __launch_bounds__(NTF, 8) k_pinv_a_al(S src) {
   const PoisArgs a = src.get();
   pinv_a<L, true>(a.g, a.pl, a.tw, a.Y, a.psi);
 }
 
 template <int L, class S>
-__global__ void __launch_bounds__(NTF) k_pinv_a(S src) {
+__global__ void __launch_bounds__(NTF, 8) k_pinv_a(S src) {
   const PoisArgs a = src.get();
   pinv_a<L, false>(a.g, a.pl, a.tw, a.Y, a.psi);
 }
