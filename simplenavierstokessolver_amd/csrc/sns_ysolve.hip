@@ -85,70 +85,89 @@ __device__ __forceinline__ double cny_pf(const Geo &g, double gamma, double dt) 
 // ------------------------------------------------------------- factors
 
 // Workgroup (substage blockIdx.x, field blockIdx.y); thread 0 runs the
-// sequential recurrences, the others clear the padding of the reduced tables.
+// sequential recurrences in LDS (each step reads what the previous one
+// wrote: an LDS round trip instead of an HBM one), then all threads copy
+// the tables out, zeros past the reduced block's last row included.
 template <class S>
 __global__ void __launch_bounds__(64) k_yfactor(S src) {
   const YFactorArgs a = src.get();
+  constexpr int NR = kYRedNT * kYRedM;
+  __shared__ double sr[kYMaxL], sbeta[kYMaxL], sv[kYMaxL];
+  __shared__ double srr[NR], sal[NR], sbe[NR], sq1[NR];
+  __shared__ double shead[kYHead];
   const int s = blockIdx.x, f = blockIdx.y;
   const int L1 = a.L - 1, SR = a.S - 1;  // interior rows; rows of the reduced block
   double *t = a.out + (size_t)(s * 3 + f) * a.stride;
-  double *rr = t + kYHead + 3 * L1, *al = rr + kYRedNT * kYRedM, *be = al + kYRedNT * kYRedM,
-         *q1 = be + kYRedNT * kYRedM;
-  for (int k = threadIdx.x; k < kYRedNT * kYRedM; k += 64) rr[k] = al[k] = be[k] = q1[k] = 0.;
+  for (int k = threadIdx.x; k < NR; k += 64) srr[k] = sal[k] = sbe[k] = sq1[k] = 0.;
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  const double gam = s == 0 ? a.gamma[0] : s == 1 ? a.gamma[1] : a.gamma[2];
-  const double dt = *a.dtp;
-  const double pf = f == 0 ? cny_pf<KIND_TEMP>(a.g, gam, dt) : cny_pf<KIND_UX>(a.g, gam, dt);
-  // coefficients as src/temperature/update.c:188-197
-  const double coef_ym = 1. / a.g.dy / a.g.dy, coef_yp = 1. / a.g.dy / a.g.dy;
-  const double coef_y0 = -coef_ym - coef_yp;
-  const double lam = -pf * coef_ym;
-  const double c = 1. - pf * coef_y0;
-  // interior Dirichlet block: r_m = 1/pivot, beta_m = lambda r_m (u')
+  if (threadIdx.x == 0) {
+    const double gam = s == 0 ? a.gamma[0] : s == 1 ? a.gamma[1] : a.gamma[2];
+    const double dt = *a.dtp;
+    const double pf = f == 0 ? cny_pf<KIND_TEMP>(a.g, gam, dt) : cny_pf<KIND_UX>(a.g, gam, dt);
+    // coefficients as src/temperature/update.c:188-197
+    const double coef_ym = 1. / a.g.dy / a.g.dy, coef_yp = 1. / a.g.dy / a.g.dy;
+    const double coef_y0 = -coef_ym - coef_yp;
+    const double lam = -pf * coef_ym;
+    const double c = 1. - pf * coef_y0;
+    // interior Dirichlet block: r_m = 1/pivot, beta_m = lambda r_m (u')
+    double prev_beta = 0.;
+    for (int m = 0; m < L1; m++) {
+      const double piv = m == 0 ? c : c - lam * prev_beta;
+      sr[m] = 1. / piv;
+      sbeta[m] = lam * sr[m];
+      prev_beta = sbeta[m];
+    }
+    // spike v = T^{-1} e_0
+    double y = 0.;
+    for (int m = 0; m < L1; m++) {
+      y = ((m == 0 ? 1. : 0.) - lam * y) * sr[m];
+      sv[m] = y;
+    }
+    for (int m = L1 - 2; m >= 0; m--) sv[m] = sv[m] - sbeta[m] * sv[m + 1];
+    const double lam2 = lam * lam;
+    const double D = c - lam2 * sv[0] - lam2 * sv[0];
+    const double O = -(lam2 * sv[L1 - 1]);
+    // reduced (S-1)-row block: interleaved factors (row k at lu_slot)
+    auto slot = [](int k) { return (k % kYRedM) * kYRedNT + k / kYRedM; };
+    double rp = 0., bp = 0.;
+    for (int k = 0; k < SR; k++) {
+      const double piv = k == 0 ? D : D - O * bp;
+      rp = 1. / piv;
+      bp = O * rp;
+      srr[slot(k)] = rp;
+      sal[slot(k)] = k == 0 ? 0. : -(rp * O);
+      sbe[slot(k)] = k == SR - 1 ? 0. : -bp;
+    }
+    // q1: the block's solution for the right-hand side [-O, 0, .., 0, -O]
+    double yq = 0.;
+    for (int k = 0; k < SR; k++) {
+      const double rhs = (k == 0 ? -O : 0.) + (k == SR - 1 ? -O : 0.);
+      yq = srr[slot(k)] * rhs + sal[slot(k)] * yq;
+      sq1[slot(k)] = yq;
+    }
+    for (int k = SR - 2; k >= 0; k--) sq1[slot(k)] = sq1[slot(k)] + sbe[slot(k)] * sq1[slot(k + 1)];
+    shead[0] = lam;
+    shead[1] = c;
+    shead[2] = D;
+    shead[3] = O;
+    shead[4] = D + O * sq1[slot(0)] + O * sq1[slot(SR - 1)];
+    shead[5] = shead[6] = shead[7] = 0.;
+  }
+  __syncthreads();
   double *r = t + kYHead, *beta = r + L1, *v = beta + L1;
-  double prev_beta = 0.;
-  for (int m = 0; m < L1; m++) {
-    const double piv = m == 0 ? c : c - lam * prev_beta;
-    r[m] = 1. / piv;
-    beta[m] = lam * r[m];
-    prev_beta = beta[m];
+  double *rr = t + kYHead + 3 * L1, *al = rr + NR, *be = al + NR, *q1 = be + NR;
+  if (threadIdx.x < kYHead) t[threadIdx.x] = shead[threadIdx.x];
+  for (int m = threadIdx.x; m < L1; m += 64) {
+    r[m] = sr[m];
+    beta[m] = sbeta[m];
+    v[m] = sv[m];
   }
-  // spike v = T^{-1} e_0
-  double y = 0.;
-  for (int m = 0; m < L1; m++) {
-    y = ((m == 0 ? 1. : 0.) - lam * y) * r[m];
-    v[m] = y;
+  for (int k = threadIdx.x; k < NR; k += 64) {
+    rr[k] = srr[k];
+    al[k] = sal[k];
+    be[k] = sbe[k];
+    q1[k] = sq1[k];
   }
-  for (int m = L1 - 2; m >= 0; m--) v[m] = v[m] - beta[m] * v[m + 1];
-  const double lam2 = lam * lam;
-  const double D = c - lam2 * v[0] - lam2 * v[0];
-  const double O = -(lam2 * v[L1 - 1]);
-  // reduced (S-1)-row block: interleaved factors (row k at lu_slot)
-  auto slot = [](int k) { return (k % kYRedM) * kYRedNT + k / kYRedM; };
-  double rp = 0., bp = 0.;
-  for (int k = 0; k < SR; k++) {
-    const double piv = k == 0 ? D : D - O * bp;
-    rp = 1. / piv;
-    bp = O * rp;
-    rr[slot(k)] = rp;
-    al[slot(k)] = k == 0 ? 0. : -(rp * O);
-    be[slot(k)] = k == SR - 1 ? 0. : -bp;
-  }
-  // q1: the block's solution for the right-hand side [-O, 0, .., 0, -O]
-  double yq = 0.;
-  for (int k = 0; k < SR; k++) {
-    const double rhs = (k == 0 ? -O : 0.) + (k == SR - 1 ? -O : 0.);
-    yq = rr[slot(k)] * rhs + al[slot(k)] * yq;
-    q1[slot(k)] = yq;
-  }
-  for (int k = SR - 2; k >= 0; k--) q1[slot(k)] = q1[slot(k)] + be[slot(k)] * q1[slot(k + 1)];
-  t[0] = lam;
-  t[1] = c;
-  t[2] = D;
-  t[3] = O;
-  t[4] = D + O * q1[slot(0)] + O * q1[slot(SR - 1)];
-  t[5] = t[6] = t[7] = 0.;
 }
 
 // ------------------------------------------------------ interior solves
@@ -252,6 +271,100 @@ __global__ void __launch_bounds__(kYRedNT) k_yreduced(S src) {
   if (threadIdx.x == 0) z[(size_t)SR * n] = zl;
 }
 
+// The same systems for 64 columns per workgroup with the lanes over
+// columns, so every load and store is coalesced: wave w holds rows
+// w*M .. w*M+M-1 of the (S-1)-row block (kYColWaves*M >= S-1), runs the two
+// substitutions of lu_solve_shared over them with zero carries, and the
+// carries are composed across the waves in LDS (the chunks are the waves
+// here, not the lanes, so the rounding differs from k_yreduced at the ulp
+// level).  Rows past S-2 meet the zero padding of the factor tables.
+constexpr int kYColWaves = 16;
+
+template <int M, class S>
+__global__ void __launch_bounds__(64 * kYColWaves) k_yreduced_c(S src) {
+  const YArgs a = src.get();
+  constexpr int NW = kYColWaves;
+  __shared__ double sA[NW][64], sB[NW][64];
+  __shared__ double sends[2][64];
+  // the wave index is uniform: the factor rows of a chunk are scalar loads
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int n = a.n, L = a.L, SR = a.S - 1;
+  const int col = blockIdx.x * 64 + lane;
+  const int cc = col < n ? col : n - 1;  // lanes past the last column repeat it
+  const YTab tb{a.yfac, L - 1};
+  const double lam = tb.lambda(), O = tb.O();
+  const double *q = a.qf + cc;
+  const double *yfirst = a.qz + (size_t)a.S * n + cc, *ylast = a.qz + (size_t)2 * a.S * n + cc;
+  auto rhs = [&](int k) {
+    const int kl = k == 0 ? a.S - 1 : k - 1;
+    return q[(size_t)k * L * n] - lam * (ylast[(size_t)kl * n] + yfirst[(size_t)k * n]);
+  };
+  auto slot = [](int k) { return (k % kYRedM) * kYRedNT + k / kYRedM; };
+  const double *rr = tb.red(0), *al = tb.red(1), *be = tb.red(2), *q1 = tb.red(3);
+  // the running products of al and be are formed again where they are used
+  // (same factors, same order) instead of being held per row
+  double d[M];
+  // forward substitution of the chunk with a zero carry
+  double y = 0., pr = 1.;
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    const int k = w * M + m;
+    const double r = rr[slot(k)], l = al[slot(k)];
+    pr *= l;
+    const double dk = k < SR ? rhs(k) : 0.;
+    y = r * dk + l * y;
+    d[m] = y;
+  }
+  sA[w][lane] = pr;
+  sB[w][lane] = y;
+  __syncthreads();
+  double cin = 0.;
+  for (int v = 0; v < w; v++) cin = sA[v][lane] * cin + sB[v][lane];
+  pr = 1.;
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    pr *= al[slot(w * M + m)];
+    d[m] += pr * cin;
+  }
+  __syncthreads();
+  // back substitution with a zero carry from the right
+  double x = 0.;
+  pr = 1.;
+#pragma unroll
+  for (int m = M - 1; m >= 0; m--) {
+    const double b = be[slot(w * M + m)];
+    pr *= b;
+    x = d[m] + b * x;
+    d[m] = x;
+  }
+  sA[w][lane] = pr;
+  sB[w][lane] = x;
+  __syncthreads();
+  cin = 0.;
+  for (int v = NW - 1; v > w; v--) cin = sA[v][lane] * cin + sB[v][lane];
+  pr = 1.;
+#pragma unroll
+  for (int m = M - 1; m >= 0; m--) {
+    pr *= be[slot(w * M + m)];
+    d[m] += pr * cin;
+  }
+  // the last unknown from the first and last rows of the block
+  if (w == 0) sends[0][lane] = d[0];
+#pragma unroll
+  for (int m = 0; m < M; m++)
+    if (w * M + m == SR - 1) sends[1][lane] = d[m];
+  __syncthreads();
+  const double zl = (rhs(SR) - O * sends[0][lane] - O * sends[1][lane]) / tb.den();
+  if (col >= n) return;
+  double *z = a.qz + col;
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    const int k = w * M + m;
+    if (k < SR) z[(size_t)k * n] = d[m] + zl * q1[slot(k)];
+  }
+  if (w == 0) z[(size_t)SR * n] = zl;
+}
+
 // ------------------------------------------------------------- apply
 
 // Chunk s = blockIdx.y*4 + wave, column ii = lane: x = p_s on the interface
@@ -333,7 +446,14 @@ static void ypart(hipStream_t s, int kind, const YArgs &shape, S src, int nz) {
   const dim3 gc((n + 63) / 64, (shape.S + 3) / 4, nz);
   if (recompute) hipLaunchKernelGGL((k_ylocal<false, S>), gc, dim3(256), 0, s, src);
   else hipLaunchKernelGGL((k_ylocal<true, S>), gc, dim3(256), 0, s, src);
-  hipLaunchKernelGGL(k_yreduced<S>, dim3(n, 1, nz), dim3(kYRedNT), 0, s, src);
+  const int SR = shape.S - 1;
+  const dim3 gq((n + 63) / 64, 1, nz), bq(64 * kYColWaves);
+  if (SR <= 1 * kYColWaves) hipLaunchKernelGGL((k_yreduced_c<1, S>), gq, bq, 0, s, src);
+  else if (SR <= 2 * kYColWaves) hipLaunchKernelGGL((k_yreduced_c<2, S>), gq, bq, 0, s, src);
+  else if (SR <= 4 * kYColWaves) hipLaunchKernelGGL((k_yreduced_c<4, S>), gq, bq, 0, s, src);
+  else if (SR <= 8 * kYColWaves) hipLaunchKernelGGL((k_yreduced_c<8, S>), gq, bq, 0, s, src);
+  else if (SR <= 16 * kYColWaves) hipLaunchKernelGGL((k_yreduced_c<16, S>), gq, bq, 0, s, src);
+  else hipLaunchKernelGGL(k_yreduced<S>, dim3(n, 1, nz), dim3(kYRedNT), 0, s, src);
   if (recompute) {
     if (kind == KIND_UX) hipLaunchKernelGGL((k_yapply<KIND_UX, S>), gc, dim3(256), 0, s, src);
     else if (kind == KIND_UY) hipLaunchKernelGGL((k_yapply<KIND_UY, S>), gc, dim3(256), 0, s, src);

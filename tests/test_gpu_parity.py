@@ -262,6 +262,7 @@ STEP_CASES = [
     (16, 4096, 3.0, 1e8, 1.0, 1, 1, 7, 1e-2, 5),      # y solve: 128 chunks of 32 rows
     (20, 960, 3.0, 1e7, 2.0, 0, 1, 8, 1e-2, 8),       # 30 chunks of 32
     (24, 60, 3.0, 1e8, 1.0, 1, 1, 9, 1e-2, 10),       # 4 chunks of 15
+    (16, 8192, 3.0, 1e8, 1.0, 1, 1, 10, 1e-2, 5),     # 256 chunks: interface block of 255 rows
 ]
 
 
