@@ -127,25 +127,28 @@ __global__ void __launch_bounds__(64) k_yfactor(S src) {
     const double lam2 = lam * lam;
     const double D = c - lam2 * sv[0] - lam2 * sv[0];
     const double O = -(lam2 * sv[L1 - 1]);
-    // reduced (S-1)-row block: interleaved factors (row k at lu_slot)
+    // reduced (S-1)-row block: interleaved factors (row k at lu_slot), and
+    // q1, the block's solution for the right-hand side [-O, 0, .., 0, -O]:
+    // its forward sweep in the same loop, both carried in registers
     auto slot = [](int k) { return (k % kYRedM) * kYRedNT + k / kYRedM; };
-    double rp = 0., bp = 0.;
+    double rp = 0., bp = 0., yq = 0.;
     for (int k = 0; k < SR; k++) {
       const double piv = k == 0 ? D : D - O * bp;
       rp = 1. / piv;
       bp = O * rp;
+      const double alk = k == 0 ? 0. : -(rp * O);
       srr[slot(k)] = rp;
-      sal[slot(k)] = k == 0 ? 0. : -(rp * O);
+      sal[slot(k)] = alk;
       sbe[slot(k)] = k == SR - 1 ? 0. : -bp;
-    }
-    // q1: the block's solution for the right-hand side [-O, 0, .., 0, -O]
-    double yq = 0.;
-    for (int k = 0; k < SR; k++) {
       const double rhs = (k == 0 ? -O : 0.) + (k == SR - 1 ? -O : 0.);
-      yq = srr[slot(k)] * rhs + sal[slot(k)] * yq;
+      yq = rp * rhs + alk * yq;
       sq1[slot(k)] = yq;
     }
-    for (int k = SR - 2; k >= 0; k--) sq1[slot(k)] = sq1[slot(k)] + sbe[slot(k)] * sq1[slot(k + 1)];
+    double xq = sq1[slot(SR - 1)];
+    for (int k = SR - 2; k >= 0; k--) {
+      xq = sq1[slot(k)] + sbe[slot(k)] * xq;
+      sq1[slot(k)] = xq;
+    }
     shead[0] = lam;
     shead[1] = c;
     shead[2] = D;
