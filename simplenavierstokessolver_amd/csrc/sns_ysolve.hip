@@ -274,25 +274,29 @@ __global__ void __launch_bounds__(kYRedNT) k_yreduced(S src) {
   if (threadIdx.x == 0) z[(size_t)SR * n] = zl;
 }
 
-// The same systems for 64 columns per workgroup with the lanes over
-// columns, so every load and store is coalesced: wave w holds rows
-// w*M .. w*M+M-1 of the (S-1)-row block (kYColWaves*M >= S-1), runs the two
-// substitutions of lu_solve_shared over them with zero carries, and the
-// carries are composed across the waves in LDS (the chunks are the waves
-// here, not the lanes, so the rounding differs from k_yreduced at the ulp
-// level).  Rows past S-2 meet the zero padding of the factor tables.
+// The same systems for CW columns per workgroup with the lanes over
+// columns, so loads and stores are coalesced in runs of CW: lane l of wave w
+// holds column l % CW and chunk c = w*(64/CW) + l/CW of the (S-1)-row block,
+// rows c*M .. c*M+M-1.  Each chunk runs the two substitutions of
+// lu_solve_shared with zero carries, and the carries are composed across the
+// chunks in LDS (the chunking differs from k_yreduced, so the rounding
+// differs at the ulp level).  Rows past S-2 meet the zero padding of the
+// factor tables.  Fewer columns per workgroup make more, shorter chunks,
+// at the price of a longer carry scan over the chunks.
 constexpr int kYColWaves = 16;
 
-template <int M, class S>
+template <int CW, int M, class S>
 __global__ void __launch_bounds__(64 * kYColWaves) k_yreduced_c(S src) {
   const YArgs a = src.get();
-  constexpr int NW = kYColWaves;
-  __shared__ double sA[NW][64], sB[NW][64];
-  __shared__ double sends[2][64];
-  // the wave index is uniform: the factor rows of a chunk are scalar loads
+  constexpr int SUB = 64 / CW, NCH = kYColWaves * SUB;
+  __shared__ double sA[NCH][CW], sB[NCH][CW];
+  __shared__ double sends[2][CW];
+  // the wave index is uniform: with one chunk per wave the factor-table
+  // reads become scalar loads
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int cl = lane % CW, c = w * SUB + lane / CW;  // column in the group; chunk
   const int n = a.n, L = a.L, SR = a.S - 1;
-  const int col = blockIdx.x * 64 + lane;
+  const int col = blockIdx.x * CW + cl;
   const int cc = col < n ? col : n - 1;  // lanes past the last column repeat it
   const YTab tb{a.yfac, L - 1};
   const double lam = tb.lambda(), O = tb.O();
@@ -311,22 +315,22 @@ __global__ void __launch_bounds__(64 * kYColWaves) k_yreduced_c(S src) {
   double y = 0., pr = 1.;
 #pragma unroll
   for (int m = 0; m < M; m++) {
-    const int k = w * M + m;
+    const int k = c * M + m;
     const double r = rr[slot(k)], l = al[slot(k)];
     pr *= l;
     const double dk = k < SR ? rhs(k) : 0.;
     y = r * dk + l * y;
     d[m] = y;
   }
-  sA[w][lane] = pr;
-  sB[w][lane] = y;
+  sA[c][cl] = pr;
+  sB[c][cl] = y;
   __syncthreads();
   double cin = 0.;
-  for (int v = 0; v < w; v++) cin = sA[v][lane] * cin + sB[v][lane];
+  for (int v = 0; v < c; v++) cin = sA[v][cl] * cin + sB[v][cl];
   pr = 1.;
 #pragma unroll
   for (int m = 0; m < M; m++) {
-    pr *= al[slot(w * M + m)];
+    pr *= al[slot(c * M + m)];
     d[m] += pr * cin;
   }
   __syncthreads();
@@ -335,37 +339,52 @@ __global__ void __launch_bounds__(64 * kYColWaves) k_yreduced_c(S src) {
   pr = 1.;
 #pragma unroll
   for (int m = M - 1; m >= 0; m--) {
-    const double b = be[slot(w * M + m)];
+    const double b = be[slot(c * M + m)];
     pr *= b;
     x = d[m] + b * x;
     d[m] = x;
   }
-  sA[w][lane] = pr;
-  sB[w][lane] = x;
+  sA[c][cl] = pr;
+  sB[c][cl] = x;
   __syncthreads();
   cin = 0.;
-  for (int v = NW - 1; v > w; v--) cin = sA[v][lane] * cin + sB[v][lane];
+  for (int v = NCH - 1; v > c; v--) cin = sA[v][cl] * cin + sB[v][cl];
   pr = 1.;
 #pragma unroll
   for (int m = M - 1; m >= 0; m--) {
-    pr *= be[slot(w * M + m)];
+    pr *= be[slot(c * M + m)];
     d[m] += pr * cin;
   }
   // the last unknown from the first and last rows of the block
-  if (w == 0) sends[0][lane] = d[0];
+  if (c == 0) sends[0][cl] = d[0];
 #pragma unroll
   for (int m = 0; m < M; m++)
-    if (w * M + m == SR - 1) sends[1][lane] = d[m];
+    if (c * M + m == SR - 1) sends[1][cl] = d[m];
   __syncthreads();
-  const double zl = (rhs(SR) - O * sends[0][lane] - O * sends[1][lane]) / tb.den();
+  const double zl = (rhs(SR) - O * sends[0][cl] - O * sends[1][cl]) / tb.den();
   if (col >= n) return;
   double *z = a.qz + col;
 #pragma unroll
   for (int m = 0; m < M; m++) {
-    const int k = w * M + m;
+    const int k = c * M + m;
     if (k < SR) z[(size_t)k * n] = d[m] + zl * q1[slot(k)];
   }
-  if (w == 0) z[(size_t)SR * n] = zl;
+  if (c == 0) z[(size_t)SR * n] = zl;
+}
+
+// Launch of k_yreduced_c for blocks of up to 256 rows: the chunk length M
+// from the number of chunks a workgroup holds
+template <int CW, class S>
+static void yreduced_c_launch(hipStream_t s, int n, int SR, S src, int nz) {
+  constexpr int NCH = kYColWaves * (64 / CW);
+  const dim3 g((n + CW - 1) / CW, 1, nz), b(64 * kYColWaves);
+  if (SR <= NCH) hipLaunchKernelGGL((k_yreduced_c<CW, 1, S>), g, b, 0, s, src);
+  else if (SR <= 2 * NCH) hipLaunchKernelGGL((k_yreduced_c<CW, 2, S>), g, b, 0, s, src);
+  else if (SR <= 4 * NCH) hipLaunchKernelGGL((k_yreduced_c<CW, 4, S>), g, b, 0, s, src);
+  else if constexpr (CW >= 32) {
+    if (SR <= 8 * NCH) hipLaunchKernelGGL((k_yreduced_c<CW, 8, S>), g, b, 0, s, src);
+    else if constexpr (CW == 64) hipLaunchKernelGGL((k_yreduced_c<CW, 16, S>), g, b, 0, s, src);
+  }
 }
 
 // ------------------------------------------------------------- apply
@@ -442,6 +461,21 @@ static bool y_recompute(size_t columns) {
   return columns >= kYRecompute;
 }
 
+// Columns per workgroup of the reduced solve: the widest whose chunks stay
+// at 4 rows or fewer (measured at 4096 columns: 17 us with 16 columns and
+// chunks of 4 rows against 24 us with 64 and chunks of 16; at 1024 columns
+// and 63 rows: 7.5 us with 64 against 9.3 us with 16, although 16 columns
+// give four times the workgroups).  SNSG_YSOLVE_COLS=16, 32 or 64 forces a
+// width (the chunking, and so the rounding, differs with it).
+static int y_cols(int SR) {
+  const char *e = getenv("SNSG_YSOLVE_COLS");
+  if (e) {
+    const int v = atoi(e);
+    if (v == 16 || v == 32 || v == 64) return v;
+  }
+  return SR <= 4 * kYColWaves ? 64 : SR <= 8 * kYColWaves ? 32 : 16;
+}
+
 template <class S>
 static void ypart(hipStream_t s, int kind, const YArgs &shape, S src, int nz) {
   const int n = shape.n;
@@ -450,13 +484,12 @@ static void ypart(hipStream_t s, int kind, const YArgs &shape, S src, int nz) {
   if (recompute) hipLaunchKernelGGL((k_ylocal<false, S>), gc, dim3(256), 0, s, src);
   else hipLaunchKernelGGL((k_ylocal<true, S>), gc, dim3(256), 0, s, src);
   const int SR = shape.S - 1;
-  const dim3 gq((n + 63) / 64, 1, nz), bq(64 * kYColWaves);
-  if (SR <= 1 * kYColWaves) hipLaunchKernelGGL((k_yreduced_c<1, S>), gq, bq, 0, s, src);
-  else if (SR <= 2 * kYColWaves) hipLaunchKernelGGL((k_yreduced_c<2, S>), gq, bq, 0, s, src);
-  else if (SR <= 4 * kYColWaves) hipLaunchKernelGGL((k_yreduced_c<4, S>), gq, bq, 0, s, src);
-  else if (SR <= 8 * kYColWaves) hipLaunchKernelGGL((k_yreduced_c<8, S>), gq, bq, 0, s, src);
-  else if (SR <= 16 * kYColWaves) hipLaunchKernelGGL((k_yreduced_c<16, S>), gq, bq, 0, s, src);
-  else hipLaunchKernelGGL(k_yreduced<S>, dim3(n, 1, nz), dim3(kYRedNT), 0, s, src);
+  const int cols = y_cols(SR);
+  if (SR > 16 * kYColWaves)
+    hipLaunchKernelGGL(k_yreduced<S>, dim3(n, 1, nz), dim3(kYRedNT), 0, s, src);
+  else if (cols == 64) yreduced_c_launch<64>(s, n, SR, src, nz);
+  else if (cols == 32) yreduced_c_launch<32>(s, n, SR, src, nz);
+  else yreduced_c_launch<16>(s, n, SR, src, nz);
   if (recompute) {
     if (kind == KIND_UX) hipLaunchKernelGGL((k_yapply<KIND_UX, S>), gc, dim3(256), 0, s, src);
     else if (kind == KIND_UY) hipLaunchKernelGGL((k_yapply<KIND_UY, S>), gc, dim3(256), 0, s, src);

@@ -439,6 +439,23 @@ def test_ysolve_recompute_matches_store(sns, monkeypatch, itot, jtot, imx):
         assert_bitwise(out[0][n], out[1][n], n)
 
 
+@pytest.mark.parametrize("cols", ["16", "32", "64"])
+@pytest.mark.parametrize("case", [c for c in STEP_CASES if c[6] == 1 and c[1] >= 60])
+def test_ysolve_widths_match_oracle(sns, pinned_gauge, monkeypatch, cols, case):
+    """The reduced interface solve of the partitioned y solve takes 16, 32 or
+    64 columns per workgroup (more, shorter chunks for narrow instances);
+    SNSG_YSOLVE_COLS forces each.  All three stay within 1e-12 of the oracle
+    over whole fused steps, interface blocks of 3 to 255 rows included."""
+    monkeypatch.setenv("SNSG_YSOLVE_COLS", cols)
+    itot, jtot, stretch, Ra, Pr, imx, imy, seed, amp, steps = case
+    o, inst = seeded_pair(sns, itot, jtot, stretch, Ra, Pr, imx, imy, seed, amp)
+    run_both(sns, o, inst, steps, imx, imy, fused=True)
+    for n in ["ux", "uy", "temp"]:
+        assert_close(interior(inst.state_download(n)), interior(o.f[n]), 1e-12, n)
+    assert_close(interior(inst.state_download("p")), interior(o.f["p"]), 1e-12, "p", demean=True)
+    inst.close()
+
+
 @pytest.mark.parametrize("case", STEP_CASES)
 def test_advance_matches_oracle_time_loop(sns, pinned_gauge, case):
     """snsg_advance (time step decided on the device) against the oracle's
