@@ -67,7 +67,7 @@ def parse():
     ap.add_argument("--itot", type=int, default=WORKLOAD["itot"])
     ap.add_argument("--jtot", type=int, default=WORKLOAD["jtot"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--per-kernel", action="store_true", help="print per-kernel timings to stderr")
     ap.add_argument("--ensemble-steps", type=int, default=300,
                     help="steps per job of the mixed ensemble (0: skip it)")
