@@ -184,6 +184,39 @@ int snsg_instance_bytes(const snsg_config *cfg, size_t *bytes);
 int snsg_uniform_grid(const snsg_ctx *ctx, int *uniform);
 /* Device bytes an instance holds now (its state and scratch included) */
 int snsg_memory(const snsg_ctx *ctx, size_t *bytes);
+/* ---- launch shapes (no counterpart in the reference, whose FFTW plans and
+ * sequential solves have none; for the autotuner and callers that tune per
+ * grid size).  A field left 0 keeps the built-in rule.  An instance takes
+ * its shapes from snsg_set_launch, else from the process-wide table
+ * (snsg_tuning_add, keyed by itot and jtot) as it stands at snsg_create,
+ * else from the rules.  The shapes reorder floating-point operations, not
+ * the algorithm: every choice stays within the parity tolerance of the
+ * reference, and instances with equal shapes return the same bits alone,
+ * batched or served (a batch's shape classes include the shapes). */
+typedef struct {
+  int fft_n1;           /* length of the first factor of the four-step y FFT
+                           (a divisor of jtot; jtot/fft_n1 the second) */
+  int ysolve_chunk;     /* rows per chunk of the partitioned CN-y solve: a
+                           divisor of jtot in 4..32, with 4..513 chunks */
+  int ysolve_cols;      /* columns per workgroup of its interface solve: 16,
+                           32 or 64 */
+  int ysolve_recompute; /* its chunk interiors: 1 solved again in the apply
+                           kernel, 2 stored and read back */
+} snsg_launch;
+/* SNSG_E_ARG for a value outside the ranges above, SNSG_E_SIZE for one the
+ * grid does not admit (a factor that does not divide jtot or whose FFT pass
+ * does not fit the kernels' LDS tile).  Host-only: needs no GPU. */
+int snsg_launch_check(int itot, int jtot, const snsg_launch *l);
+/* Before the instance's first fused step, and not while it is in a batch
+ * (the partitioned y solve's scratch is sized by the chunk length). */
+int snsg_set_launch(snsg_ctx *ctx, const snsg_launch *l);
+/* The shapes in effect, rules resolved; ysolve_cols and ysolve_recompute
+ * are 0 where the rule decides at launch (it depends on the batch size). */
+int snsg_get_launch(const snsg_ctx *ctx, snsg_launch *l);
+/* Adds (or replaces) the table entry of (itot, jtot) after
+ * snsg_launch_check; snsg_tuning_clear empties the table.  Host-only. */
+int snsg_tuning_add(int itot, int jtot, const snsg_launch *l);
+int snsg_tuning_clear(void);
 /* ---- batches (no counterpart in the reference, which runs one instance per
  * process).  Many instances advance together: each kernel of a step is
  * launched once per shape class (the members with the same itot, jtot,

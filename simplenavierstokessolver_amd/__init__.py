@@ -54,6 +54,16 @@ class SnsgServerStats(C.Structure):
         ("mem_peak", C.c_size_t), ("lanes", C.c_int)]
 
 
+class SnsgLaunch(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("fft_n1", "ysolve_chunk", "ysolve_cols",
+                                       "ysolve_recompute")]
+
+
+LAUNCH_FIELDS = tuple(n for n, _ in SnsgLaunch._fields_)
+# launch-shape table written by `python -m simplenavierstokessolver_amd.autotune`
+TUNING_FILE = os.path.join(_PKG, "tuning", "launch_gfx950.json")
+
+
 # field ids of snsg_state_upload / snsg_state_download (include/sns_hip.h)
 STATE_FIELDS = {"ux": 0, "uy": 1, "p": 2, "psi": 3, "temp": 4, "ux_star": 5, "uy_star": 6,
                 "srcuxa": 7, "srcuya": 8, "srctempa": 9}
@@ -109,6 +119,11 @@ def _load_hip():
         L.snsg_instance_bytes.argtypes = [C.POINTER(SnsgConfig), C.POINTER(C.c_size_t)]
         L.snsg_memory.argtypes = [_vp, C.POINTER(C.c_size_t)]
         L.snsg_uniform_grid.argtypes = [_vp, C.POINTER(C.c_int)]
+        L.snsg_launch_check.argtypes = [C.c_int, C.c_int, C.POINTER(SnsgLaunch)]
+        L.snsg_set_launch.argtypes = [_vp, C.POINTER(SnsgLaunch)]
+        L.snsg_get_launch.argtypes = [_vp, C.POINTER(SnsgLaunch)]
+        L.snsg_tuning_add.argtypes = [C.c_int, C.c_int, C.POINTER(SnsgLaunch)]
+        L.snsg_tuning_clear.argtypes = []
         L.snsg_server_create.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(_vp)]
         L.snsg_server_submit.argtypes = [_vp, C.POINTER(SnsgJob), C.POINTER(C.c_int)]
         L.snsg_server_wait.argtypes = [_vp, C.c_int, C.c_double]
@@ -243,6 +258,49 @@ def instance_bytes(cfg):
     return n.value
 
 
+def _launch(shapes):
+    bad = set(shapes) - set(LAUNCH_FIELDS)
+    if bad:
+        raise ValueError("unknown launch shapes: %s" % ", ".join(sorted(bad)))
+    return SnsgLaunch(**{k: int(shapes.get(k, 0)) for k in LAUNCH_FIELDS})
+
+
+def launch_check(itot, jtot, **shapes):
+    """Raises GpuError unless the kernels take these launch shapes for an
+    itot x jtot grid (snsg_launch_check; host-only, needs no GPU)."""
+    _check(_load_hip().snsg_launch_check(int(itot), int(jtot), C.byref(_launch(shapes))),
+           "snsg_launch_check")
+
+
+def tuning_add(itot, jtot, **shapes):
+    """Table entry of (itot, jtot): instances created afterwards take these
+    launch shapes (snsg_tuning_add)."""
+    _check(_load_hip().snsg_tuning_add(int(itot), int(jtot), C.byref(_launch(shapes))),
+           "snsg_tuning_add")
+
+
+def tuning_clear():
+    _check(_load_hip().snsg_tuning_clear(), "snsg_tuning_clear")
+
+
+def read_tuning(path=TUNING_FILE):
+    """Entries of a launch-shape table file: [(itot, jtot, shapes), ...]."""
+    import json
+    with open(path) as f:
+        doc = json.load(f)
+    return [(int(e["itot"]), int(e["jtot"]), {k: int(e.get(k, 0)) for k in LAUNCH_FIELDS})
+            for e in doc["entries"]]
+
+
+def load_tuning(path=TUNING_FILE):
+    """Registers every entry of a launch-shape table file; returns their
+    number."""
+    entries = read_tuning(path)
+    for itot, jtot, shapes in entries:
+        tuning_add(itot, jtot, **shapes)
+    return len(entries)
+
+
 def _state_shape(name):
     """Reference-layout shape name of a state field."""
     return {"ux_star": "ux", "uy_star": "uy"}.get(name, name)
@@ -342,6 +400,17 @@ class Instance:
     def set_treatment(self, implicitx, implicity):
         _check(self.L.snsg_set_treatment(self.ctx, int(implicitx), int(implicity)),
                "snsg_set_treatment")
+
+    def set_launch(self, **shapes):
+        """Launch shapes of this instance (all of them; 0 or absent: the
+        rule), before its first fused step (snsg_set_launch)."""
+        _check(self.L.snsg_set_launch(self.ctx, C.byref(_launch(shapes))), "snsg_set_launch")
+
+    def launch(self):
+        """The launch shapes in effect, rules resolved (snsg_get_launch)."""
+        l = SnsgLaunch()
+        _check(self.L.snsg_get_launch(self.ctx, C.byref(l)), "snsg_get_launch")
+        return {k: getattr(l, k) for k in LAUNCH_FIELDS}
 
     # ---- reference functions (host-layer conventions: B <- A by pointer exchange)
     def update_boundaries(self, name):

@@ -52,16 +52,21 @@ struct Lane {
   hipGraphExec_t exec[2] = {};
 };
 
-using LaneKey = std::tuple<int, int, int, int, int>;
+// shape class: size, treatment, temperature equation and launch shapes
+using LaneKey = std::tuple<int, int, int, int, int, int, int, int, int>;
 
 LaneKey key_of(const snsg_ctx *c) {
+  const LaunchKey l = launch_key(c);
   return LaneKey(c->geo.itot, c->geo.jtot, c->treat.implicitx, c->treat.implicity,
-                 c->cfg.with_temperature != 0);
+                 c->cfg.with_temperature != 0, l.n1, l.chunk, l.cols, l.recompute);
 }
 
+// the class of a job's instance: its shapes come from the table as snsg_create
+// reads it
 LaneKey key_of(const snsg_config &cfg) {
+  const LaunchKey l = launch_key(cfg.itot, cfg.jtot, tuned_launch(cfg.itot, cfg.jtot));
   return LaneKey(cfg.itot, cfg.jtot, cfg.implicitx != 0, cfg.implicity != 0,
-                 cfg.with_temperature != 0);
+                 cfg.with_temperature != 0, l.n1, l.chunk, l.cols, l.recompute);
 }
 
 int lane_init(Lane &k, const snsg_ctx *c) {
@@ -612,7 +617,7 @@ static int engine_collect(snsg_server *s, bool &progressed) {
     rel.swap(s->released);
   }
   for (Job *j : rel) {
-    if (j->c) s->pool.emplace(key_of(j->cfg), Pooled{j->c, j->st});
+    if (j->c) s->pool.emplace(key_of(j->c), Pooled{j->c, j->st});
     j->c = nullptr;
     progressed = true;
   }

@@ -111,11 +111,11 @@ extern "C" const char *snsg_last_error(void) { return g_last_error.c_str(); }
 
 __global__ void k_set_f64(double *p, double v) { *p = v; }
 
-static int check_sizes(const snsg_config *cfg, Fft2Plan &pl) {
+static int check_sizes(const snsg_config *cfg, Fft2Plan &pl, int n1) {
   const int itot = cfg->itot, jtot = cfg->jtot;
   if (itot < 4 || jtot < 4) return set_err(SNSG_E_SIZE, "snsg_create: itot and jtot must be >= 4");
   if (itot > 8192) return set_err(SNSG_E_SIZE, "snsg_create: itot > 8192 is not supported");
-  if (jtot > 16384 || !fft2_plan(jtot, itot, pl))
+  if (jtot > 16384 || !fft2_plan(jtot, itot, pl, n1))
     return set_err(SNSG_E_SIZE,
                    "snsg_create: jtot must be <= 16384, not prime, with prime factors 2, 3 "
                    "and 5 only");
@@ -266,13 +266,15 @@ extern "C" int snsg_create(int device, const snsg_config *cfg, const double *dxf
   if (!cfg || !dxf || !dxc || !out) return set_err(SNSG_E_ARG, "snsg_create: null argument");
   const int itot = cfg->itot, jtot = cfg->jtot;
   Fft2Plan pl;
-  int rc = check_sizes(cfg, pl);
+  const snsg_launch lt = tuned_launch(itot, jtot);
+  int rc = check_sizes(cfg, pl, lt.fft_n1);
   if (rc) return rc;
   HIPCHECK(hipSetDevice(device));
   snsg_ctx *c = new snsg_ctx();
   c->device = device;
   c->cfg = *cfg;
   c->plan = pl;
+  c->launch = lt;
   auto fail = [&](int code) {
     snsg_destroy(c);
     return code;
@@ -337,6 +339,89 @@ extern "C" int snsg_set_treatment(snsg_ctx *c, int implicitx, int implicity) {
   return 0;
 }
 
+
+// ------------------------------------------------------- launch shapes
+
+static std::mutex g_tune_mu;
+static std::map<std::pair<int, int>, snsg_launch> g_tune;
+
+snsg_launch tuned_launch(int itot, int jtot) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find({itot, jtot});
+  return it == g_tune.end() ? snsg_launch{} : it->second;
+}
+
+LaunchKey launch_key(int itot, int jtot, const snsg_launch &req) {
+  LaunchKey k{};
+  Fft2Plan pl;
+  k.n1 = fft2_plan(jtot, itot, pl, req.fft_n1) ? pl.n1 : 0;
+  k.chunk = req.ysolve_chunk ? req.ysolve_chunk : ysolve_chunk(jtot);
+  // the width applies to interface systems of up to 256 rows
+  const int SR = k.chunk ? jtot / k.chunk - 1 : 0;
+  k.cols = k.chunk && SR <= 256 ? (req.ysolve_cols ? req.ysolve_cols : ysolve_cols_rule(SR)) : 0;
+  k.recompute = k.chunk ? req.ysolve_recompute : 0;
+  return k;
+}
+
+LaunchKey launch_key(const snsg_ctx *c) {
+  LaunchKey k = launch_key(c->geo.itot, c->geo.jtot, c->launch);
+  k.n1 = c->plan.n1;
+  return k;
+}
+
+extern "C" int snsg_launch_check(int itot, int jtot, const snsg_launch *l) {
+  if (!l) return set_err(SNSG_E_ARG, "snsg_launch_check: null argument");
+  if (itot < 4 || jtot < 4) return set_err(SNSG_E_SIZE, "snsg_launch_check: itot, jtot < 4");
+  const int w = l->ysolve_cols;
+  if (l->fft_n1 < 0 || l->ysolve_chunk < 0 || (w && w != 16 && w != 32 && w != 64) ||
+      l->ysolve_recompute < 0 || l->ysolve_recompute > 2)
+    return set_err(SNSG_E_ARG, "snsg_launch_check: value out of range");
+  Fft2Plan pl;
+  if (l->fft_n1 && !fft2_plan(jtot, itot, pl, l->fft_n1))
+    return set_err(SNSG_E_SIZE, "snsg_launch_check: fft_n1 does not give a four-step plan of jtot");
+  if (l->ysolve_chunk && !ysolve_chunk_ok(jtot, l->ysolve_chunk))
+    return set_err(SNSG_E_SIZE, "snsg_launch_check: ysolve_chunk must divide jtot into 4..513 "
+                                "chunks of 4..32 rows");
+  return 0;
+}
+
+extern "C" int snsg_set_launch(snsg_ctx *c, const snsg_launch *l) {
+  if (!c) return set_err(SNSG_E_ARG, "null ctx");
+  const int itot = c->geo.itot, jtot = c->geo.jtot;
+  int rc = snsg_launch_check(itot, jtot, l);
+  if (rc) return rc;
+  Fft2Plan pl;
+  if (!fft2_plan(jtot, itot, pl, l->fft_n1))
+    return set_err(SNSG_E_SIZE, "snsg_set_launch: no four-step plan");
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  c->plan = pl;
+  c->launch = *l;  // the chunk length takes effect at the next fused step
+  return 0;
+}
+
+extern "C" int snsg_get_launch(const snsg_ctx *c, snsg_launch *l) {
+  if (!c || !l) return set_err(SNSG_E_ARG, "snsg_get_launch: null argument");
+  const LaunchKey k = launch_key(c);
+  l->fft_n1 = k.n1;
+  l->ysolve_chunk = k.chunk;
+  l->ysolve_cols = k.cols;
+  l->ysolve_recompute = k.recompute;
+  return 0;
+}
+
+extern "C" int snsg_tuning_add(int itot, int jtot, const snsg_launch *l) {
+  int rc = snsg_launch_check(itot, jtot, l);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[{itot, jtot}] = *l;
+  return 0;
+}
+
+extern "C" int snsg_tuning_clear(void) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune.clear();
+  return 0;
+}
 
 extern "C" int snsg_set_dt(snsg_ctx *c, double dt) {
   if (!c) return set_err(SNSG_E_ARG, "null ctx");
@@ -673,7 +758,7 @@ extern "C" int snsg_state_init_temperature(snsg_ctx *c, snsg_state *st, uint64_t
 extern "C" int snsg_instance_bytes(const snsg_config *cfg, size_t *bytes) {
   if (!cfg || !bytes) return set_err(SNSG_E_ARG, __func__);
   Fft2Plan pl;
-  int rc = check_sizes(cfg, pl);
+  int rc = check_sizes(cfg, pl, tuned_launch(cfg->itot, cfg->jtot).fft_n1);
   if (rc) return rc;
   const size_t itot = cfg->itot, jtot = cfg->jtot, d = sizeof(double), z = sizeof(double2);
   const size_t stride = itot + 2;
@@ -691,7 +776,7 @@ extern "C" int snsg_instance_bytes(const snsg_config *cfg, size_t *bytes) {
   // first fused step
   if (cfg->implicity) {
     n += d * (2 * itot + itot - 1) * jtot;
-    const int L = ysolve_chunk((int)jtot);
+    const int L = launch_key((int)itot, (int)jtot, tuned_launch((int)itot, (int)jtot)).chunk;
     if (L) n += d * 9 * ysolve_table_doubles(L) + 3 * d * (jtot / L) * itot;
   }
   if (cfg->implicitx) n += d * 27 * (size_t)lu_table_entries((int)itot);
@@ -834,13 +919,31 @@ int step_prepare(snsg_ctx *c) {
     const size_t nb[3] = {(size_t)itot * jtot, (size_t)(itot - 1) * jtot, (size_t)itot * jtot};
     for (int k = 0; k < 3; k++)
       if ((rc = ctx_alloc((void **)&c->d_qf[k], sizeof(double) * nb[k], &c->mem))) return rc;
-    c->ychunk = ysolve_chunk(jtot);
-    if (c->ychunk) {
-      const size_t nt = 9 * ysolve_table_doubles(c->ychunk);
-      // interface values and the first and last interior values of each chunk
-      const size_t nz = 3 * (size_t)(jtot / c->ychunk) * itot;
-      if ((rc = ctx_alloc((void **)&c->d_yfac, sizeof(double) * nt, &c->mem))) return rc;
-      if ((rc = ctx_alloc((void **)&c->d_qz, sizeof(double) * nz, &c->mem))) return rc;
+  }
+  // factor tables and interface rows of the partitioned y solve, sized by
+  // the chunk length (the launch shapes may have changed it)
+  auto ybytes = [&](int L, size_t &nt, size_t &nz) {
+    nt = sizeof(double) * 9 * ysolve_table_doubles(L);
+    // interface values and the first and last interior values of each chunk
+    nz = sizeof(double) * 3 * (size_t)(jtot / L) * itot;
+  };
+  const int L = c->treat.implicity ? launch_key(c).chunk : c->ychunk;
+  if (L != c->ychunk || (L && !c->d_qz)) {
+    if (c->d_qz) {
+      HIPCHECK(hipStreamSynchronize(c->stream));
+      size_t nt, nz;
+      ybytes(c->ychunk, nt, nz);
+      (void)hipFree(c->d_yfac);
+      (void)hipFree(c->d_qz);
+      c->d_yfac = c->d_qz = nullptr;
+      c->mem -= nt + nz;
+    }
+    c->ychunk = L;
+    if (L) {
+      size_t nt, nz;
+      ybytes(L, nt, nz);
+      if ((rc = ctx_alloc((void **)&c->d_yfac, nt, &c->mem))) return rc;
+      if ((rc = ctx_alloc((void **)&c->d_qz, nz, &c->mem))) return rc;
     }
   }
   if (c->treat.implicitx && !c->d_lu) {
@@ -925,6 +1028,8 @@ void step_args(const snsg_ctx *c, const snsg_state *st, bool reduce, StepArgs &s
       y.S = yL ? g.jtot / yL : 0;
       y.yfac = yL ? c->d_yfac + (size_t)(rk * 3 + f) * ystride : nullptr;
       y.qz = c->d_qz;
+      y.cols = c->launch.ysolve_cols;
+      y.recompute = c->launch.ysolve_recompute;
     }
     sa.pois[rk] = pois_args(c, g, c->cfg.gamma[rk], st->ux_star, st->uy_star, st->psi);
     CorrArgs &ca = sa.corr[rk];

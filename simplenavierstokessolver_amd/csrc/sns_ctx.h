@@ -34,6 +34,7 @@ struct snsg_ctx {
   double *h_diag = nullptr;             // pinned copy of the results
   int nparts = 0;
   Fft2Plan plan{};
+  snsg_launch launch{};  // requested launch shapes (0: the rule)
   double2 *d_Y = nullptr;  // four-step FFT intermediate, jtot rows of (itot+1)/2 pairs
   // per-column tables of the fused kernels (one allocation)
   double *d_tab = nullptr;
@@ -102,4 +103,15 @@ int state_reset(snsg_ctx *c, snsg_state *st);
 
 // Scratch of the fused step, allocated on first use
 int step_prepare(snsg_ctx *c);
+
+// Launch shapes: the table entry of (itot, jtot) (zeros when there is none),
+// and the shapes that decide a batch's shape classes, rules resolved where
+// they are static: FFT factor n1, chunk length, interface-solve width,
+// recompute choice (0: decided at launch).
+snsg_launch tuned_launch(int itot, int jtot);
+struct LaunchKey {
+  int n1, chunk, cols, recompute;
+};
+LaunchKey launch_key(int itot, int jtot, const snsg_launch &req);
+LaunchKey launch_key(const snsg_ctx *c);
 void step_args(const snsg_ctx *c, const snsg_state *st, bool reduce, StepArgs &sa);

@@ -67,6 +67,8 @@ struct YFactorArgs {
 };
 // chunk length of the partitioned y solve for jtot rows (0: not applicable)
 int ysolve_chunk(int jtot);
+bool ysolve_chunk_ok(int jtot, int L);
+int ysolve_cols_rule(int SR);
 size_t ysolve_table_doubles(int L);
 void launch_yfactor(hipStream_t s, const YFactorArgs &a);
 void launch_yfactor_batch(hipStream_t s, const YFactorArgs *dev, int nz);

@@ -76,6 +76,9 @@ struct YArgs {
   int L, S;
   const double *yfac;
   double *qz;
+  // launch shapes of the partitioned solve (host side; 0: the rule):
+  // columns per workgroup of the interface solve, 1 recompute / 2 store
+  int cols, recompute;
 };
 
 // ---- boundary values (src/fluid/boundary_conditions.c, src/temperature/boundary_conditions.c)

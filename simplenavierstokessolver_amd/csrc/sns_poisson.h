@@ -14,7 +14,9 @@ struct Fft2Plan {
   int G;                         // rows j2 per workgroup in passes A and A'
 };
 
-bool fft2_plan(int n, int itot, Fft2Plan &pl);
+// four-step plan of length n for itot columns; n1 > 0 fixes the first
+// factor (false when the factors or the LDS tiles do not fit)
+bool fft2_plan(int n, int itot, Fft2Plan &pl, int n1 = 0);
 
 // Thomas factors of the x systems of all wavenumbers (once per instance):
 // fac holds (jtot/2+1) tables of poisson_factor_entries(itot) reciprocal

@@ -526,15 +526,25 @@ static bool radix_code(int m, int &np, unsigned long long &code) {
   return rest == 1;
 }
 
-bool fft2_plan(int n, int itot, Fft2Plan &pl) {
+bool fft2_plan(int n, int itot, Fft2Plan &pl, int n1) {
   if (n < 4) return false;
-  // n1 = the largest divisor of n not above sqrt(n) that the radices cover
+  // n1 = the given factor, else the largest divisor of n not above sqrt(n)
+  // that the radices cover
   int best = -1;
-  for (int d = 1; d * d <= n; d++) {
-    if (n % d) continue;
+  if (n1 > 0) {
     int np1, np2;
     unsigned long long c1, c2;
-    if (radix_code(d, np1, c1) && radix_code(n / d, np2, c2)) best = d;
+    if (n % n1 || n1 < 2 || n / n1 < 2 || !radix_code(n1, np1, c1) ||
+        !radix_code(n / n1, np2, c2))
+      return false;
+    best = n1;
+  } else {
+    for (int d = 1; d * d <= n; d++) {
+      if (n % d) continue;
+      int np1, np2;
+      unsigned long long c1, c2;
+      if (radix_code(d, np1, c1) && radix_code(n / d, np2, c2)) best = d;
+    }
   }
   if (best < 1) return false;
   pl.n = n;

@@ -52,6 +52,14 @@ int ysolve_chunk(int jtot) {
   return 0;
 }
 
+// A chunk length the kernels take: 4..32 rows (the interior solves hold a
+// chunk in registers), dividing jtot into 4..513 chunks (the reduced
+// system's tables hold up to 512 rows).
+bool ysolve_chunk_ok(int jtot, int L) {
+  return L >= 4 && L <= kYMaxL && jtot % L == 0 && jtot / L >= 4 &&
+         jtot / L - 1 <= kYRedNT * kYRedM;
+}
+
 size_t ysolve_table_doubles(int L) {
   return kYHead + 3 * (size_t)(L - 1) + 4 * (size_t)(kYRedNT * kYRedM);
 }
@@ -451,40 +459,48 @@ __global__ void __launch_bounds__(256) k_yapply(S src) {
 // ------------------------------------------------------------- launch
 
 // Chunk columns (threads of the per-chunk kernels) from which the interiors
-// are solved twice rather than stored: 4096 waves.  SNSG_YSOLVE_RECOMPUTE=1
-// or =0 forces either way (both return the same bits).
+// are solved twice rather than stored: 4096 waves, unless the instance's
+// launch shapes say (1 recompute, 2 store).  SNSG_YSOLVE_RECOMPUTE=1 or =0
+// forces either way (both return the same bits).
 constexpr size_t kYRecompute = (size_t)4096 * 64;
 
-static bool y_recompute(size_t columns) {
+static bool y_recompute(size_t columns, int chosen) {
   const char *e = getenv("SNSG_YSOLVE_RECOMPUTE");
   if (e && (e[0] == '0' || e[0] == '1') && e[1] == 0) return e[0] == '1';
+  if (chosen == 1 || chosen == 2) return chosen == 1;
   return columns >= kYRecompute;
 }
 
-// Columns per workgroup of the reduced solve: the widest whose chunks stay
-// at 4 rows or fewer (measured at 4096 columns: 17 us with 16 columns and
-// chunks of 4 rows against 24 us with 64 and chunks of 16; at 1024 columns
-// and 63 rows: 7.5 us with 64 against 9.3 us with 16, although 16 columns
-// give four times the workgroups).  SNSG_YSOLVE_COLS=16, 32 or 64 forces a
-// width (the chunking, and so the rounding, differs with it).
-static int y_cols(int SR) {
+// Columns per workgroup of the reduced solve: the instance's launch shape,
+// else the widest whose chunks stay at 4 rows or fewer (measured at 4096
+// columns: 17 us with 16 columns and chunks of 4 rows against 24 us with 64
+// and chunks of 16; at 1024 columns and 63 rows: 7.5 us with 64 against
+// 9.3 us with 16, although 16 columns give four times the workgroups).
+// SNSG_YSOLVE_COLS=16, 32 or 64 forces a width (the chunking, and so the
+// rounding, differs with it).
+int ysolve_cols_rule(int SR) {
+  return SR <= 4 * kYColWaves ? 64 : SR <= 8 * kYColWaves ? 32 : 16;
+}
+
+static int y_cols(int SR, int chosen) {
   const char *e = getenv("SNSG_YSOLVE_COLS");
   if (e) {
     const int v = atoi(e);
     if (v == 16 || v == 32 || v == 64) return v;
   }
-  return SR <= 4 * kYColWaves ? 64 : SR <= 8 * kYColWaves ? 32 : 16;
+  if (chosen == 16 || chosen == 32 || chosen == 64) return chosen;
+  return ysolve_cols_rule(SR);
 }
 
 template <class S>
 static void ypart(hipStream_t s, int kind, const YArgs &shape, S src, int nz) {
   const int n = shape.n;
-  const bool recompute = y_recompute((size_t)n * shape.S * nz);
+  const bool recompute = y_recompute((size_t)n * shape.S * nz, shape.recompute);
   const dim3 gc((n + 63) / 64, (shape.S + 3) / 4, nz);
   if (recompute) hipLaunchKernelGGL((k_ylocal<false, S>), gc, dim3(256), 0, s, src);
   else hipLaunchKernelGGL((k_ylocal<true, S>), gc, dim3(256), 0, s, src);
   const int SR = shape.S - 1;
-  const int cols = y_cols(SR);
+  const int cols = y_cols(SR, shape.cols);
   if (SR > 16 * kYColWaves)
     hipLaunchKernelGGL(k_yreduced<S>, dim3(n, 1, nz), dim3(kYRedNT), 0, s, src);
   else if (cols == 64) yreduced_c_launch<64>(s, n, SR, src, nz);
