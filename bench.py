@@ -306,7 +306,11 @@ def main():
         return
     itot, jtot = a.itot, a.jtot
     cells = itot * jtot
+    # launch shapes from the autotuner's table (every instance below, the
+    # ensemble's and the served jobs' included, is created after this)
+    ntuned = sns.load_tuning() if os.path.exists(sns.TUNING_FILE) else 0
     inst = make_instance(sns, itot, jtot)
+    launch = inst.launch()
     step(sns, inst, a.warmup)
     inst.sync()
     # timed region, with the dominant-kernel events recorded live on the stream
@@ -328,7 +332,8 @@ def main():
         "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": 1e3 * el / a.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": dict(WORKLOAD, itot=itot, jtot=jtot),
+        "config": dict(WORKLOAD, itot=itot, jtot=jtot, launch=launch,
+                       launch_table_entries=ntuned),
     }
     if dominant and nlaunch:
         avg_s = kms / nlaunch * 1e-3
