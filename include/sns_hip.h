@@ -197,7 +197,9 @@ typedef struct {
   int fft_n1;           /* length of the first factor of the four-step y FFT
                            (a divisor of jtot; jtot/fft_n1 the second) */
   int ysolve_chunk;     /* rows per chunk of the partitioned CN-y solve: a
-                           divisor of jtot in 4..32, with 4..513 chunks */
+                           divisor of jtot in 4..32, with 4..513 chunks;
+                           1: no partition (column solves on the transposed
+                           increments) */
   int ysolve_cols;      /* columns per workgroup of its interface solve: 16,
                            32 or 64 */
   int ysolve_recompute; /* its chunk interiors: 1 solved again in the apply

@@ -2,11 +2,12 @@
 
 Per grid size it times, on the GPU, the launch shapes the kernels take
 (``snsg_launch`` in include/sns_hip.h: the first factor of the four-step y
-FFT, the chunk length of the partitioned Crank-Nicolson y solve, the width
-of its interface solve, and whether its chunk interiors are solved again or
-stored) and writes the fastest into a table file.  ``load_tuning`` registers
-that table (``snsg_tuning_add``); instances created afterwards take its
-shapes, alone, batched and served alike.
+FFT, the chunk length of the partitioned Crank-Nicolson y solve or no
+partition at all, the width of its interface solve, and whether its chunk
+interiors are solved again or stored) and writes the fastest into a table
+file.  ``load_tuning`` registers that table (``snsg_tuning_add``);
+instances created afterwards take its shapes, alone, batched and served
+alike.
 
 Every candidate passes ``snsg_launch_check``, and every shape keeps the
 results within the parity tolerance of the reference (the GPU tests run
@@ -55,7 +56,7 @@ def candidates(sns, itot, jtot, field):
     if field == "fft_n1":
         vals = [d for d in divisors(jtot) if 2 <= d <= jtot // 2]
     elif field == "ysolve_chunk":
-        vals = [d for d in divisors(jtot) if 4 <= d <= 32]
+        vals = [1] + [d for d in divisors(jtot) if 4 <= d <= 32]  # 1: no partition
     elif field == "ysolve_cols":
         vals = [16, 32, 64]
     elif field == "ysolve_recompute":
@@ -141,7 +142,7 @@ def main(argv=None):
     ap.add_argument("--sizes", nargs="*", default=None, help="itot x jtot, e.g. 1024x2048")
     ap.add_argument("--out", default=sns.TUNING_FILE)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--margin", type=float, default=0.02)
     a = ap.parse_args(argv)
     sizes = SIZES if not a.sizes else [tuple(int(x) for x in s.split("x")) for s in a.sizes]

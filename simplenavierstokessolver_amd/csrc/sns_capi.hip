@@ -355,11 +355,15 @@ LaunchKey launch_key(int itot, int jtot, const snsg_launch &req) {
   LaunchKey k{};
   Fft2Plan pl;
   k.n1 = fft2_plan(jtot, itot, pl, req.fft_n1) ? pl.n1 : 0;
+  // chunk 1: the column solves on the transposed increments (also where
+  // jtot has no chunk length the partitioned solve takes)
   k.chunk = req.ysolve_chunk ? req.ysolve_chunk : ysolve_chunk(jtot);
+  if (k.chunk == 0) k.chunk = 1;
+  const bool part = k.chunk > 1;
   // the width applies to interface systems of up to 256 rows
-  const int SR = k.chunk ? jtot / k.chunk - 1 : 0;
-  k.cols = k.chunk && SR <= 256 ? (req.ysolve_cols ? req.ysolve_cols : ysolve_cols_rule(SR)) : 0;
-  k.recompute = k.chunk ? req.ysolve_recompute : 0;
+  const int SR = part ? jtot / k.chunk - 1 : 0;
+  k.cols = part && SR <= 256 ? (req.ysolve_cols ? req.ysolve_cols : ysolve_cols_rule(SR)) : 0;
+  k.recompute = part ? req.ysolve_recompute : 0;
   return k;
 }
 
@@ -379,7 +383,7 @@ extern "C" int snsg_launch_check(int itot, int jtot, const snsg_launch *l) {
   Fft2Plan pl;
   if (l->fft_n1 && !fft2_plan(jtot, itot, pl, l->fft_n1))
     return set_err(SNSG_E_SIZE, "snsg_launch_check: fft_n1 does not give a four-step plan of jtot");
-  if (l->ysolve_chunk && !ysolve_chunk_ok(jtot, l->ysolve_chunk))
+  if (l->ysolve_chunk > 1 && !ysolve_chunk_ok(jtot, l->ysolve_chunk))
     return set_err(SNSG_E_SIZE, "snsg_launch_check: ysolve_chunk must divide jtot into 4..513 "
                                 "chunks of 4..32 rows");
   return 0;
@@ -777,7 +781,7 @@ extern "C" int snsg_instance_bytes(const snsg_config *cfg, size_t *bytes) {
   if (cfg->implicity) {
     n += d * (2 * itot + itot - 1) * jtot;
     const int L = launch_key((int)itot, (int)jtot, tuned_launch((int)itot, (int)jtot)).chunk;
-    if (L) n += d * 9 * ysolve_table_doubles(L) + 3 * d * (jtot / L) * itot;
+    if (L > 1) n += d * 9 * ysolve_table_doubles(L) + 3 * d * (jtot / L) * itot;
   }
   if (cfg->implicitx) n += d * 27 * (size_t)lu_table_entries((int)itot);
   Geo g{};
@@ -927,7 +931,8 @@ int step_prepare(snsg_ctx *c) {
     // interface values and the first and last interior values of each chunk
     nz = sizeof(double) * 3 * (size_t)(jtot / L) * itot;
   };
-  const int L = c->treat.implicity ? launch_key(c).chunk : c->ychunk;
+  int L = c->treat.implicity ? launch_key(c).chunk : c->ychunk;
+  if (L == 1) L = 0;  // transposed solves
   if (L != c->ychunk || (L && !c->d_qz)) {
     if (c->d_qz) {
       HIPCHECK(hipStreamSynchronize(c->stream));

@@ -1,5 +1,5 @@
 """Launch shapes (snsg_launch: the first factor of the four-step y FFT, the
-chunk length of the partitioned Crank-Nicolson y solve, the width of its
+chunk length of the partitioned Crank-Nicolson y solve or none, the width of its
 interface solve, recompute or store) against the oracle, and the launch-shape
 table (snsg_tuning_add) through the batch and the serving engine.
 
@@ -79,6 +79,8 @@ SHAPE_CASES = [
     (20, 960, 3.0, 1e7, 2.0, 0, 1, 6, dict(ysolve_chunk=24, ysolve_recompute=2)),
     (16, 8192, 3.0, 1e8, 1.0, 1, 1, 4, dict(ysolve_chunk=16)),     # 511-row interface block
     (16, 4096, 3.0, 1e8, 1.0, 1, 1, 4, dict(ysolve_chunk=32, ysolve_cols=64, fft_n1=128)),
+    (64, 256, 3.0, 1e8, 1.0, 1, 1, 10, dict(ysolve_chunk=1)),      # no partition
+    (48, 96, 3.0, 1e6, 2.0, 0, 1, 10, dict(ysolve_chunk=1, fft_n1=8)),
 ]
 
 

@@ -17,6 +17,8 @@ from simplenavierstokessolver_amd import autotune
     (4096, 8192, dict(fft_n1=128)),
     (64, 256, dict(ysolve_chunk=4, ysolve_cols=16, ysolve_recompute=2)),
     (16, 16384, dict(ysolve_chunk=32)),      # 512 chunks
+    (256, 512, dict(ysolve_chunk=1)),        # no partition
+    (40, 8, dict(ysolve_chunk=1)),
 ])
 def test_launch_check_accepts(itot, jtot, shapes):
     sns.launch_check(itot, jtot, **shapes)
