@@ -209,8 +209,9 @@ typedef struct {
  * grid does not admit (a factor that does not divide jtot or whose FFT pass
  * does not fit the kernels' LDS tile).  Host-only: needs no GPU. */
 int snsg_launch_check(int itot, int jtot, const snsg_launch *l);
-/* Before the instance's first fused step, and not while it is in a batch
- * (the partitioned y solve's scratch is sized by the chunk length). */
+/* Replaces all of the instance's shapes (0: the rule).  A new chunk length
+ * takes effect at the next fused step, which sizes the y solve's scratch
+ * again.  SNSG_E_ARG while the instance is in a batch. */
 int snsg_set_launch(snsg_ctx *ctx, const snsg_launch *l);
 /* The shapes in effect, rules resolved; ysolve_cols and ysolve_recompute
  * are 0 where the rule decides at launch (it depends on the batch size). */

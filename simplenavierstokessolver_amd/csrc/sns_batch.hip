@@ -268,11 +268,14 @@ struct snsg_batch {
   std::vector<Lane> lanes;
   std::vector<hipEvent_t> done;   // one per lane
   std::vector<std::vector<int>> ids;  // member indices of each lane
+  bool counted = false;               // members' in_batch counts raised
 };
 
 static void batch_free(snsg_batch *b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
+  if (b->counted)
+    for (snsg_ctx *c : b->ctx) c->in_batch--;
   for (Lane &k : b->lanes) lane_free(k);
   for (hipEvent_t e : b->ready)
     if (e) (void)hipEventDestroy(e);
@@ -327,6 +330,8 @@ extern "C" int snsg_batch_create(int n, snsg_ctx *const *ctx, snsg_state *const 
     for (hipEvent_t &e : *v)
       if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
         return fail(set_err(SNSG_E_ARG, "snsg_batch_create: event"));
+  for (snsg_ctx *c : b->ctx) c->in_batch++;
+  b->counted = true;
   *out = b;
   return 0;
 }

@@ -394,6 +394,9 @@ extern "C" int snsg_set_launch(snsg_ctx *c, const snsg_launch *l) {
   const int itot = c->geo.itot, jtot = c->geo.jtot;
   int rc = snsg_launch_check(itot, jtot, l);
   if (rc) return rc;
+  if (c->in_batch)
+    return set_err(SNSG_E_ARG, "snsg_set_launch: the instance is in a batch (its shape class "
+                               "and recorded graphs depend on the shapes)");
   Fft2Plan pl;
   if (!fft2_plan(jtot, itot, pl, l->fft_n1))
     return set_err(SNSG_E_SIZE, "snsg_set_launch: no four-step plan");

@@ -35,6 +35,7 @@ struct snsg_ctx {
   int nparts = 0;
   Fft2Plan plan{};
   snsg_launch launch{};  // requested launch shapes (0: the rule)
+  int in_batch = 0;      // batches that hold this instance
   double2 *d_Y = nullptr;  // four-step FFT intermediate, jtot rows of (itot+1)/2 pairs
   // per-column tables of the fused kernels (one allocation)
   double *d_tab = nullptr;

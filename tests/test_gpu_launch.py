@@ -192,12 +192,15 @@ def test_table_batch_matches_alone(table):
     other.set_launch()  # the rules
     b = sns.Batch(insts + [other])
     assert b.classes() == 3
+    with pytest.raises(sns.GpuError):  # shapes are fixed while in a batch
+        other.set_launch(fft_n1=8)
     b.advance(6)
     b.sync()
     for m, inst, ref in zip(MEMBERS, insts, alone):
         for n in ref:
             assert same(inst.state_download(n), ref[n]), (m, n)
     b.close()
+    other.set_launch(fft_n1=8)  # free again once the batch is gone
     for i in insts + [other]:
         i.close()
 
