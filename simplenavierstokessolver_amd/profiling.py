@@ -6,6 +6,9 @@
         --write CSV [CSV ...] --copy-bytes N --cells N [--out FILE] [--json FILE]
         [--command TEXT ...]
 
+  python -m simplenavierstokessolver_amd.profiling counters CSV [CSV ...]
+        [--out FILE] [--command TEXT ...]
+
 `stats` condenses rocprofv3's kernel_stats.csv into a table of kernel groups
 (the names bench.py and the C-ABI timing use).  `traffic` averages the
 FETCH_SIZE and WRITE_SIZE counters (KiB per dispatch) of every kernel over
@@ -13,6 +16,9 @@ the given counter_collection.csv files and scales each by the factor that
 makes the plain copy kernel (k_copy, which reads and writes --copy-bytes
 bytes) read exactly its size; the scaled bytes per dispatch are compared
 with the kernel's algorithmic bytes (bench.KERNEL_BYTES_PER_CELL x cells).
+`counters` tabulates every counter of the given counter_collection.csv
+files as its mean per dispatch of each kernel, with the L2 hit rate
+TCC_HIT / (TCC_HIT + TCC_MISS) when both are there.
 """
 import argparse
 import collections
@@ -100,6 +106,21 @@ def traffic(fetch_paths, write_paths, copy_bytes, cells):
             "cells": cells, "kernels": rows}
 
 
+def counter_table(paths):
+    """(counter names, {kernel: {counter: (dispatches, mean per dispatch)}})."""
+    names = []
+    with_kernels = collections.defaultdict(dict)
+    for p in paths:
+        with open(p) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] not in names:
+                    names.append(r["Counter_Name"])
+    for n in names:
+        for k, v in counter_avgs(paths, n).items():
+            with_kernels[k][n] = v
+    return names, with_kernels
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -115,6 +136,10 @@ def main(argv=None):
     s2.add_argument("--out")
     s2.add_argument("--json")
     s2.add_argument("--command", action="append", default=[])
+    s3 = sub.add_parser("counters")
+    s3.add_argument("csv", nargs="+")
+    s3.add_argument("--out")
+    s3.add_argument("--command", action="append", default=[])
     a = ap.parse_args(argv)
     lines = []
     if a.command:
@@ -125,6 +150,22 @@ def main(argv=None):
                   "|---|---|---:|---:|---:|---:|"]
         for k, g, n, avg, tot, pct in rows:
             lines.append("| %s | %s | %d | %.1f | %.3f | %.2f |" % (k, g, n, avg, tot, pct))
+    elif a.cmd == "counters":
+        names, tab = counter_table(a.csv)
+        hit = "TCC_HIT_sum" in names and "TCC_MISS_sum" in names
+        lines += ["Mean per dispatch of each counter.", "",
+                  "| kernel | group | dispatches | " + " | ".join(names) +
+                  (" | L2 hit % |" if hit else " |"),
+                  "|---|---|---:|" + "---:|" * (len(names) + (1 if hit else 0))]
+        for k in sorted(tab, key=lambda k: group_of("sns::" + k + "(")):
+            row = tab[k]
+            nd = max(v[0] for v in row.values())
+            vals = ["%.4g" % row[n][1] if n in row else "-" for n in names]
+            line = "| %s | %s | %d | %s |" % (k, group_of("sns::" + k + "("), nd, " | ".join(vals))
+            if hit:
+                h, m = row.get("TCC_HIT_sum", (0, 0.))[1], row.get("TCC_MISS_sum", (0, 0.))[1]
+                line += " %s |" % ("%.1f" % (100. * h / (h + m)) if h + m > 0 else "-")
+            lines.append(line)
     else:
         t = traffic(a.fetch, a.write, a.copy_bytes, a.cells)
         lines += ["Calibration: k_copy of %d bytes read %.0f KiB (FETCH_SIZE) and wrote %.0f KiB "
