@@ -34,6 +34,24 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contra
 CC_FLAGS = ["-O2", "-std=gnu11", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
             "-Wno-unused-parameter", "-I", INC]
 
+# include/parallel.h keeps the reference's MPI-typed prototypes, so the host
+# layer compiles against an MPI implementation's <mpi.h> (it links no MPI
+# library: one rank needs no communication)
+MPI_INCLUDE_CANDIDATES = ["/opt/conda/include", "/usr/include/x86_64-linux-gnu/mpich",
+                          "/usr/include/mpich", "/usr/lib/x86_64-linux-gnu/openmpi/include",
+                          "/usr/include/openmpi"]
+
+
+def mpi_include():
+    """Directory holding mpi.h: $SNS_MPI_INCLUDE, else the first candidate
+    that has one."""
+    env = os.environ.get("SNS_MPI_INCLUDE")
+    for d in ([env] if env else []) + MPI_INCLUDE_CANDIDATES:
+        if os.path.exists(os.path.join(d, "mpi.h")):
+            return d
+    raise RuntimeError("build failed: no mpi.h found (set SNS_MPI_INCLUDE to the directory "
+                       "of an MPI implementation's mpi.h)")
+
 
 def _run(cmd):
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -60,12 +78,13 @@ def build(verbose=False):
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", hip_so + ".tmp"] + objs)
     os.replace(hip_so + ".tmp", hip_so)
     host_so = os.path.join(LIB, "libsns.so")
-    _run(["gcc"] + CC_FLAGS + ["-shared", "-o", host_so + ".tmp"]
+    cc_flags = CC_FLAGS + ["-I", mpi_include()]
+    _run(["gcc"] + cc_flags + ["-shared", "-o", host_so + ".tmp"]
          + [os.path.join(HOST, s) for s in HOST_SOURCES]
          + ["-L" + LIB, "-lsns_hip", "-lm", "-Wl,-rpath,$ORIGIN"])
     os.replace(host_so + ".tmp", host_so)
     exe = os.path.join(LIB, "sns_run")
-    _run(["gcc"] + CC_FLAGS + ["-o", exe + ".tmp", os.path.join(HOST, "main.c"),
+    _run(["gcc"] + cc_flags + ["-o", exe + ".tmp", os.path.join(HOST, "main.c"),
                                "-L" + LIB, "-lsns", "-lsns_hip", "-lm", "-Wl,-rpath,$ORIGIN"])
     os.replace(exe + ".tmp", exe)
     return hip_so, host_so, exe

@@ -36,7 +36,7 @@ static int integrate(const param_t *param, const parallel_t *parallel, fluid_t *
 
 int main(void){
   double wtimes[2] = {0., 0.};
-  wtimes[0] = parallel_get_wtime();
+  wtimes[0] = parallel_get_wtime(MPI_MIN);
   mkdir("output", 0777);
   mkdir(FILEIO_SAVE, 0777);
   mkdir(FILEIO_LOG, 0777);
@@ -67,7 +67,7 @@ int main(void){
     if(param->time > param->timemax){
       break;
     }
-    wtimes[1] = parallel_get_wtime();
+    wtimes[1] = parallel_get_wtime(MPI_MAX);
     if(wtimes[1]-wtimes[0] > param->wtimemax){
       break;
     }

@@ -2,6 +2,7 @@
  * Run parameters, grid, RK coefficients, time step and treatment choice
  * (reference: src/param/init.c, decide_dt.c, estimate_cost.c).
  */
+#include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -33,109 +34,133 @@ static int verbose(void){
 
 #define PRINTF_MAIN(...) do { if(verbose()) printf(__VA_ARGS__); } while(0)
 
-static char *load_env_as_string(const char varname[]){
-  char *buf = getenv(varname);
-  if(buf == NULL) return NULL;
-  size_t nchar = strlen(buf);
-  char *string = common_calloc(nchar+1, sizeof(char));
-  memcpy(string, buf, sizeof(char)*nchar);
-  return string;
+/* ---------------------------------------------------------------------------
+ * Configuration from environment variables.  The variable names, defaults
+ * and printed values are the reference's interface (src/param/init.c:37-121);
+ * the loader is one table walked by one function.  Each entry names a
+ * param_t member by its offset; every default is held as a double.
+ * ------------------------------------------------------------------------- */
+
+typedef enum { CFG_BOOL, CFG_INT, CFG_SEED, CFG_DOUBLE } cfg_kind;
+
+typedef struct {
+  const char *name;   /* environment variable */
+  cfg_kind kind;
+  size_t member;      /* offsetof(param_t, ...) */
+  double fallback;    /* value when the variable is unset (bool: 0 / 1) */
+} cfg_entry;
+
+#define CFG(name, kind, member, fallback) {name, kind, offsetof(param_t, member), fallback}
+
+static const cfg_entry g_config[] = {
+  CFG("with_temperature",     CFG_BOOL,   with_temperature,     1.),
+  CFG("with_thermal_forcing", CFG_BOOL,   with_thermal_forcing, 1.),
+  CFG("timemax",              CFG_DOUBLE, timemax,              2.0e+2),
+  CFG("wtimemax",             CFG_DOUBLE, wtimemax,             6.0e+2),
+  CFG("log_rate",             CFG_DOUBLE, log.rate,             2.0e-1),
+  CFG("log_after",            CFG_DOUBLE, log.after,            0.0e+0),
+  CFG("save_rate",            CFG_DOUBLE, save.rate,            2.0e+1),
+  CFG("save_after",           CFG_DOUBLE, save.after,           0.0e+0),
+  CFG("stat_rate",            CFG_DOUBLE, stat.rate,            1.0e-1),
+  CFG("stat_after",           CFG_DOUBLE, stat.after,           1.0e+2),
+  CFG("itot",                 CFG_INT,    itot,                 128),
+  CFG("jtot",                 CFG_INT,    jtot,                 192),
+  CFG("ly",                   CFG_DOUBLE, ly,                   2.0e+0),
+  CFG("stretch",              CFG_DOUBLE, stretch,              3.0e+0),
+  CFG("Ra",                   CFG_DOUBLE, Ra,                   1.0e+8),
+  CFG("Pr",                   CFG_DOUBLE, Pr,                   1.0e+1),
+  /* additions of this build: initial condition and device */
+  CFG("ic_seed",              CFG_SEED,   ic_seed,              0),
+  CFG("ic_amplitude",         CFG_DOUBLE, ic_amplitude,         5.0e-2),
+  CFG("ic_linear",            CFG_BOOL,   ic_linear,            1.),
+  CFG("sns_device",           CFG_INT,    device,               0),
+};
+
+/* a boolean variable counts when it starts with "true" or "false"; anything
+ * else leaves the default */
+static int parse_bool(const char *text, bool *out){
+  if(text == NULL) return 0;
+  if(0 == strncmp(text, "true", 4)){ *out = true; return 1; }
+  if(0 == strncmp(text, "false", 5)){ *out = false; return 1; }
+  return 0;
 }
 
-/* src/param/init.c:37-56 */
-static bool load_bool(const char varname[], const bool default_val){
-  bool retval = default_val;
-  char *string = load_env_as_string(varname);
-  if(string != NULL && 0 == strncmp(string, "true", 4)){
-    retval = true;
-    PRINTF_MAIN("  %s: %s\n", varname, "true");
-  }else if(string != NULL && 0 == strncmp(string, "false", 5)){
-    retval = false;
-    PRINTF_MAIN("  %s: %s\n", varname, "false");
-  }else{
-    PRINTF_MAIN("  %s: %s (default)\n", varname, retval ? "true" : "false");
+/* decimal integer, saturated to the range of int */
+static int parse_int(const char *text){
+  long v = strtol(text, NULL, 10);
+  if(v < INT_MIN) return INT_MIN;
+  if(v > INT_MAX) return INT_MAX;
+  return (int)v;
+}
+
+static void load_entry(param_t *param, const cfg_entry *e){
+  const char *text = getenv(e->name);
+  void *dst = (char *)param+e->member;
+  const char *note = " (default)";
+  switch(e->kind){
+    case CFG_BOOL: {
+      bool v = e->fallback != 0.;
+      if(parse_bool(text, &v)) note = "";
+      *(bool *)dst = v;
+      PRINTF_MAIN("  %s: %s%s\n", e->name, v ? "true" : "false", note);
+      break;
+    }
+    case CFG_INT:
+    case CFG_SEED: {
+      int v = (int)e->fallback;
+      if(text != NULL){ v = parse_int(text); note = ""; }
+      if(e->kind == CFG_INT) *(int *)dst = v;
+      else *(uint64_t *)dst = (uint64_t)v;
+      PRINTF_MAIN("  %s: %d%s\n", e->name, v, note);
+      break;
+    }
+    case CFG_DOUBLE: {
+      double v = e->fallback;
+      if(text != NULL){ v = strtod(text, NULL); note = ""; }
+      *(double *)dst = v;
+      PRINTF_MAIN("  %s: %.3e%s\n", e->name, v, note);
+      break;
+    }
   }
-  common_free(string);
-  return retval;
 }
 
-/* src/param/init.c:58-72 */
-static int load_int(const char varname[], const int default_val){
-  int retval = default_val;
-  char *string = load_env_as_string(varname);
-  if(string != NULL){
-    long tmp = strtol(string, NULL, 10);
-    tmp = INT_MIN > tmp ? INT_MIN : tmp;
-    tmp = INT_MAX < tmp ? INT_MAX : tmp;
-    retval = (int)tmp;
-    PRINTF_MAIN("  %s: %d\n", varname, retval);
-  }else{
-    PRINTF_MAIN("  %s: %d (default)\n", varname, retval);
-  }
-  common_free(string);
-  return retval;
+/* a copy of an environment variable's text, or NULL when it is unset */
+static char *env_copy(const char *name){
+  const char *text = getenv(name);
+  if(text == NULL) return NULL;
+  const size_t n = strlen(text)+1;
+  char *copy = common_calloc(n, sizeof(char));
+  memcpy(copy, text, n);
+  return copy;
 }
 
-/* src/param/init.c:74-85 */
-static double load_double(const char varname[], const double default_val){
-  double retval = default_val;
-  char *string = load_env_as_string(varname);
-  if(string != NULL){
-    retval = strtod(string, NULL);
-    PRINTF_MAIN("  %s: %.3e\n", varname, retval);
-  }else{
-    PRINTF_MAIN("  %s: %.3e (default)\n", varname, retval);
-  }
-  common_free(string);
-  return retval;
-}
-
-/* src/param/init.c:87-122, plus the GPU build's additions */
 static int load_config(param_t *param){
   PRINTF_MAIN("------- parameters are loaded -------\n");
-  param->dirname_restart = load_env_as_string("dirname_restart");
-  if(param->dirname_restart == NULL){
-    param->load_flow_field = false;
-    PRINTF_MAIN("  flow fields are initialised\n");
-  }else{
-    param->load_flow_field = true;
+  param->dirname_restart = env_copy("dirname_restart");
+  param->load_flow_field = param->dirname_restart != NULL;
+  if(param->load_flow_field){
     PRINTF_MAIN("  flow fields are loaded: %s\n", param->dirname_restart);
+  }else{
+    PRINTF_MAIN("  flow fields are initialised\n");
   }
-  param->with_temperature     = load_bool("with_temperature",     true);
-  param->with_thermal_forcing = load_bool("with_thermal_forcing", true);
-  param->timemax    = load_double("timemax",    2.0e+2);
-  param->wtimemax   = load_double("wtimemax",   6.0e+2);
-  param->log.rate   = load_double("log_rate",   2.0e-1);
-  param->log.after  = load_double("log_after",  0.0e+0);
-  param->save.rate  = load_double("save_rate",  2.0e+1);
-  param->save.after = load_double("save_after", 0.0e+0);
-  param->stat.rate  = load_double("stat_rate",  1.0e-1);
-  param->stat.after = load_double("stat_after", 1.0e+2);
-  param->itot    = load_int("itot", 128);
-  param->jtot    = load_int("jtot", 192);
-  param->lx      = LX;
-  param->ly      = load_double("ly", 2.0e+0);
-  param->stretch = load_double("stretch", 3.0e+0);
-  param->Ra      = load_double("Ra", 1.0e+8);
-  param->Pr      = load_double("Pr", 1.0e+1);
-  /* additions: forced diffusive treatment, initial condition, device */
+  for(size_t k = 0; k < sizeof g_config/sizeof g_config[0]; k++){
+    load_entry(param, &g_config[k]);
+  }
+  param->lx = LX;
+  /* addition of this build: a diffusive treatment forced by implicitx /
+   * implicity ("true" or "1" selects implicit; either variable forces both) */
   {
-    char *sx = load_env_as_string("implicitx");
-    char *sy = load_env_as_string("implicity");
+    const char *sx = getenv("implicitx");
+    const char *sy = getenv("implicity");
+    bool vx = false, vy = false;
     param->force_treatment = (sx != NULL || sy != NULL);
-    param->forced_implicitx = (sx != NULL && (0 == strncmp(sx, "true", 4) || 0 == strcmp(sx, "1")));
-    param->forced_implicity = (sy != NULL && (0 == strncmp(sy, "true", 4) || 0 == strcmp(sy, "1")));
+    param->forced_implicitx = (parse_bool(sx, &vx) && vx) || (sx != NULL && 0 == strcmp(sx, "1"));
+    param->forced_implicity = (parse_bool(sy, &vy) && vy) || (sy != NULL && 0 == strcmp(sy, "1"));
     if(param->force_treatment){
       PRINTF_MAIN("  treatment forced: x %s, y %s\n",
           param->forced_implicitx ? "IMP" : "EXP", param->forced_implicity ? "IMP" : "EXP");
     }
-    common_free(sx);
-    common_free(sy);
   }
-  param->ic_seed      = (uint64_t)load_int("ic_seed", 0);
-  param->ic_amplitude = load_double("ic_amplitude", 5.0e-2);
-  param->ic_linear    = load_bool("ic_linear", true);
-  param->device       = load_int("sns_device", 0);
   PRINTF_MAIN("-------------------------------------\n");
   return 0;
 }
@@ -343,7 +368,7 @@ static int emulate(param_t *param, const parallel_t *parallel, const double wtim
   temperature_t *temperature = temperature_init(param, parallel);
   common_check_gpu(snsg_set_treatment(param->gpu, param->implicitx, param->implicity), "set_treatment");
   common_check_gpu(snsg_set_dt(param->gpu, param->dt), "set_dt");
-  const double t0 = parallel_get_wtime();
+  const double t0 = parallel_get_wtime(MPI_MAX);
   for(niter=0; ; niter++){
     const int rkstep = 0;
     fluid_update_boundaries_ux(param, parallel, fluid->ux);
@@ -359,7 +384,7 @@ static int emulate(param_t *param, const parallel_t *parallel, const double wtim
     fluid_correct_velocity(param, parallel, rkstep, fluid);
     fluid_update_pressure(param, parallel, rkstep, fluid);
     common_check_gpu(snsg_synchronize(param->gpu), "emulate");
-    if(parallel_get_wtime()-t0 > wtimemax) break;
+    if(parallel_get_wtime(MPI_MIN)-t0 > wtimemax) break;
   }
   fluid_finalise(fluid);
   temperature_finalise(temperature);
