@@ -63,7 +63,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--itot", type=int, default=WORKLOAD["itot"])
     ap.add_argument("--jtot", type=int, default=WORKLOAD["jtot"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -130,25 +130,40 @@ def cpu_baseline(itot, jtot, nsteps):
             "seconds": el}
 
 
-def copy_calibration(sns, gib, reps=5):
-    """Device copy of `gib` GiB, `reps` times after one warm-up: prints the
-    copy's read+write bandwidth (one JSON line)."""
+COPY_SHAPES = [(0, False)] + [(u, nt) for nt in (False, True) for u in (1, 2, 4, 8)]
+
+
+def copy_sweep(sns, gib, reps=5):
+    """Device copies of `gib` GiB: the plain grid-stride kernel (loads 0,
+    the one that calibrates the traffic counters) and the tiled kernel at
+    each shape, `reps` timed launches after one warm-up each; read+write
+    GB/s per shape and the best, the practical copy ceiling."""
     nbytes = int(gib * (1 << 30)) // 16 * 16
     inst = sns.Instance(64, 64)
     src, dst = inst.malloc(nbytes), inst.malloc(nbytes)
-    inst.copy(dst, src, nbytes)
-    inst.sync()
-    inst.timing_enable("copy")
-    for _ in range(reps):
-        inst.copy(dst, src, nbytes)
-    inst.sync()
-    n, ms = inst.timing_read()
-    inst.timing_disable()
+    rows = []
+    for loads, nt in COPY_SHAPES:
+        inst.copy(dst, src, nbytes, loads=loads, nontemporal=nt)
+        inst.sync()
+        inst.timing_enable("copy")
+        for _ in range(reps):
+            inst.copy(dst, src, nbytes, loads=loads, nontemporal=nt)
+        inst.sync()
+        n, ms = inst.timing_read()
+        inst.timing_disable()
+        avg = ms / max(n, 1)
+        rows.append({"loads_per_thread": loads, "nontemporal": nt, "avg_launch_ms": avg,
+                     "GB/s": 2 * nbytes / (avg * 1e-3) / 1e9})
     inst.close()
-    avg = ms / max(n, 1)
-    print(json.dumps({"metric": "device copy bandwidth", "bytes_read": nbytes,
-                      "bytes_written": nbytes, "launches": n, "avg_launch_ms": avg,
-                      "GB/s": 2 * nbytes / (avg * 1e-3) / 1e9}))
+    best = max(rows, key=lambda r: r["GB/s"])
+    return {"bytes_read": nbytes, "bytes_written": nbytes, "launches_per_shape": reps,
+            "shapes": rows, "best": best, "GB/s": best["GB/s"]}
+
+
+def copy_calibration(sns, gib, reps=5):
+    """Prints the copy sweep as one JSON line."""
+    res = copy_sweep(sns, gib, reps)
+    print(json.dumps(dict({"metric": "device copy bandwidth"}, **res)))
 
 
 def log(msg):
@@ -344,6 +359,14 @@ def main():
                            "traffic": measured_traffic(dominant, cells),
                            "avg_launch_ms": kms / nlaunch,
                            "bytes_per_launch": bytes_launch}
+    # the practical copy ceiling, measured in the same run
+    ceiling = copy_sweep(sns, 1.0, reps=3)
+    res["copy_ceiling"] = {"GB/s": ceiling["GB/s"], "best_shape": ceiling["best"],
+                           "bytes": ceiling["bytes_read"],
+                           "what": "best read+write bandwidth of a device copy over the "
+                                   "shapes of the tiled copy kernel and the plain one"}
+    if "roofline" in res:
+        res["roofline"]["frac_of_copy"] = res["roofline"]["achieved"] / ceiling["GB/s"]
     step_gbs = value * B_ALG_STEP / 1e9
     res["step_roofline"] = {"bytes_per_cell_step": B_ALG_STEP, "achieved": step_gbs,
                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS}

@@ -96,6 +96,13 @@ int snsg_download(snsg_ctx *ctx, void *dst, const void *src, size_t bytes);
  * reference: state snapshots, and the known-size copy that calibrates
  * hardware traffic counters. */
 int snsg_copy(snsg_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* the same copy by a tiled kernel: each thread moves `loads` (1, 2, 4 or 8)
+ * 16-byte words, all loads issued before the stores; nontemporal != 0 marks
+ * them nontemporal.  No reference counterpart: its shape sweep measures the
+ * device's practical copy bandwidth, the ceiling quoted beside the HBM
+ * peak. */
+int snsg_copy_shaped(snsg_ctx *ctx, void *dst, const void *src, size_t bytes, int loads,
+                     int nontemporal);
 
 /* ---- boundary values */
 int snsg_fluid_update_boundaries_ux(snsg_ctx *ctx, double *ux);

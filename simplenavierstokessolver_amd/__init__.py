@@ -93,6 +93,7 @@ def _load_hip():
         L.snsg_device_count.argtypes = [C.POINTER(C.c_int)]
         L.snsg_malloc.argtypes = [_vp, C.c_size_t, C.POINTER(_vp)]
         L.snsg_copy.argtypes = [_vp, _vp, _vp, C.c_size_t]
+        L.snsg_copy_shaped.argtypes = [_vp, _vp, _vp, C.c_size_t, C.c_int, C.c_int]
         L.snsg_free.argtypes = [_vp, _vp]
         L.snsg_upload.argtypes = [_vp, _vp, _vp, C.c_size_t]
         L.snsg_download.argtypes = [_vp, _vp, _vp, C.c_size_t]
@@ -330,12 +331,12 @@ class Instance:
     def close(self):
         if self.ctx:
             if self.state is not None:
-                self.L.snsg_state_free(self.ctx, C.byref(self.state))
+                _check(self.L.snsg_state_free(self.ctx, C.byref(self.state)), "snsg_state_free")
                 self.state = None
             for v in self.buf.values():
                 self.L.snsg_free(self.ctx, v)
             self.buf = {}
-            self.L.snsg_destroy(self.ctx)
+            _check(self.L.snsg_destroy(self.ctx), "snsg_destroy")
             self.ctx = _vp()
 
     def __del__(self):
@@ -388,9 +389,14 @@ class Instance:
         self.buf.pop("_raw%d" % ptr.value, None)
         _check(self.L.snsg_free(self.ctx, ptr), "snsg_free")
 
-    def copy(self, dst, src, nbytes):
-        """Device-to-device copy on the instance's stream (snsg_copy)."""
-        _check(self.L.snsg_copy(self.ctx, dst, src, int(nbytes)), "snsg_copy")
+    def copy(self, dst, src, nbytes, loads=0, nontemporal=False):
+        """Device-to-device copy on the instance's stream: snsg_copy, or the
+        tiled snsg_copy_shaped when loads (per thread: 1, 2, 4, 8) is set."""
+        if loads:
+            _check(self.L.snsg_copy_shaped(self.ctx, dst, src, int(nbytes), int(loads),
+                                           int(bool(nontemporal))), "snsg_copy_shaped")
+        else:
+            _check(self.L.snsg_copy(self.ctx, dst, src, int(nbytes)), "snsg_copy")
 
     # ---- parameters
     def set_dt(self, dt):

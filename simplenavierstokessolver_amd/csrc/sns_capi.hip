@@ -315,6 +315,9 @@ extern "C" int snsg_create(int device, const snsg_config *cfg, const double *dxf
 
 extern "C" int snsg_destroy(snsg_ctx *c) {
   if (!c) return 0;
+  if (c->in_batch)
+    return set_err(SNSG_E_ARG, "snsg_destroy: the instance is in a batch (destroy the batch "
+                               "first: its graphs hold the instance's buffers)");
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *bufs[] = {c->d_dxf, c->d_dxc, c->d_dt, c->d_eigen, c->d_tw, c->d_psihat, c->d_Y, c->d_q,
@@ -333,6 +336,9 @@ extern "C" int snsg_destroy(snsg_ctx *c) {
 
 extern "C" int snsg_set_treatment(snsg_ctx *c, int implicitx, int implicity) {
   if (!c) return set_err(SNSG_E_ARG, "null ctx");
+  if (c->in_batch)
+    return set_err(SNSG_E_ARG, "snsg_set_treatment: the instance is in a batch (the treatment "
+                               "is part of its shape class and recorded graphs)");
   c->cfg.implicitx = implicitx ? 1 : 0;
   c->cfg.implicity = implicity ? 1 : 0;
   c->treat = make_treat(c->cfg.implicitx, c->cfg.implicity);
@@ -478,6 +484,22 @@ extern "C" int snsg_copy(snsg_ctx *c, void *dst, const void *src, size_t bytes) 
   if (bytes % 16 || (uintptr_t)dst % 16 || (uintptr_t)src % 16)
     return set_err(SNSG_E_ARG, "snsg_copy: size and addresses must be multiples of 16 bytes");
   return run(c, "copy", [&] { launch_copy(c->stream, src, dst, bytes / 16); });
+}
+
+extern "C" int snsg_copy_shaped(snsg_ctx *c, void *dst, const void *src, size_t bytes,
+                                int loads, int nontemporal) {
+  if (!c || !dst || !src) return set_err(SNSG_E_ARG, "snsg_copy_shaped: null argument");
+  if (bytes % 16 || (uintptr_t)dst % 16 || (uintptr_t)src % 16)
+    return set_err(SNSG_E_ARG, "snsg_copy_shaped: size and addresses must be multiples of 16 "
+                               "bytes");
+  if (loads != 1 && loads != 2 && loads != 4 && loads != 8)
+    return set_err(SNSG_E_ARG, "snsg_copy_shaped: loads must be 1, 2, 4 or 8");
+  bool ok = true;
+  int rc = run(c, "copy", [&] {
+    ok = launch_copy_shaped(c->stream, src, dst, bytes / 16, loads, nontemporal != 0);
+  });
+  if (!ok) return set_err(SNSG_E_SIZE, "snsg_copy_shaped: too many blocks");
+  return rc;
 }
 
 // ------------------------------------------------------ boundary values
@@ -707,6 +729,9 @@ extern "C" int snsg_state_alloc(snsg_ctx *c, snsg_state *st) {
 
 extern "C" int snsg_state_free(snsg_ctx *c, snsg_state *st) {
   if (!c || !st) return set_err(SNSG_E_ARG, __func__);
+  if (c->in_batch)
+    return set_err(SNSG_E_ARG, "snsg_state_free: the instance is in a batch (destroy the batch "
+                               "first: its graphs hold the state's fields)");
   HIPCHECK(hipStreamSynchronize(c->stream));
   double *plain[] = {st->ux, st->ux_star, st->srcuxa, st->srcuya, st->srctempa};
   double *cell[] = {st->uy, st->p, st->psi, st->temp, st->temp_alt, st->uy_star};

@@ -135,5 +135,8 @@ void launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v,
 // seeded initial temperature noise in the interior, then boundary values
 void launch_init_temp(hipStream_t s, const Geo &g, double *temp, uint64_t seed, double amplitude);
 void launch_copy(hipStream_t s, const void *src, void *dst, size_t n16);
+// tiled copy, loads in {1, 2, 4, 8} 16-byte words per thread; false: bad shape
+bool launch_copy_shaped(hipStream_t s, const void *src, void *dst, size_t n16, int loads,
+                        bool nontemporal);
 
 }  // namespace sns

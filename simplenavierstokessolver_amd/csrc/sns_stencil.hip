@@ -396,6 +396,61 @@ void launch_copy(hipStream_t s, const void *src, void *dst, size_t n16) {
                      (const double2 *)src, (double2 *)dst, n16);
 }
 
+// Tiled device copy: each 256-thread block moves 256*U 16-byte words, every
+// thread issuing its U loads before its U stores (U independent loads in
+// flight per thread); NT selects nontemporal loads and stores.  The shape
+// sweep of this kernel measures the device's practical copy bandwidth.
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+template <int U, bool NT>
+__global__ void __launch_bounds__(256) k_copy_tile(const v2d *__restrict__ src,
+                                                  v2d *__restrict__ dst, size_t n) {
+  const size_t base = (size_t)blockIdx.x * (256 * U) + threadIdx.x;
+  if (base + (size_t)(U - 1) * 256 < n) {
+    v2d v[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      if constexpr (NT) v[k] = __builtin_nontemporal_load(src + base + k * 256);
+      else v[k] = src[base + k * 256];
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      if constexpr (NT) __builtin_nontemporal_store(v[k], dst + base + k * 256);
+      else dst[base + k * 256] = v[k];
+    }
+  } else {
+    for (int k = 0; k < U; k++) {
+      const size_t e = base + (size_t)k * 256;
+      if (e < n) dst[e] = src[e];
+    }
+  }
+}
+
+template <int U, bool NT>
+static void copy_tile_launch(hipStream_t s, const void *src, void *dst, size_t n16) {
+  const size_t blocks = (n16 + 256 * U - 1) / (256 * U);
+  hipLaunchKernelGGL((k_copy_tile<U, NT>), dim3((unsigned)(blocks ? blocks : 1)), dim3(256), 0,
+                     s, (const v2d *)src, (v2d *)dst, n16);
+}
+
+bool launch_copy_shaped(hipStream_t s, const void *src, void *dst, size_t n16, int loads,
+                        bool nontemporal) {
+  if ((n16 + 255) / 256 > 0x7fffffffull) return false;
+#define SNS_COPY_CASE(U)                                                       \
+  case U:                                                                      \
+    if (nontemporal) copy_tile_launch<U, true>(s, src, dst, n16);             \
+    else copy_tile_launch<U, false>(s, src, dst, n16);                        \
+    return true;
+  switch (loads) {
+    SNS_COPY_CASE(1)
+    SNS_COPY_CASE(2)
+    SNS_COPY_CASE(4)
+    SNS_COPY_CASE(8)
+    default: return false;
+  }
+#undef SNS_COPY_CASE
+}
+
 void launch_dt_reduce(hipStream_t s, const Geo &g, const double *ux, const double *uy,
                       unsigned long long *out) {
   const long long total = (long long)g.itot * g.jtot;

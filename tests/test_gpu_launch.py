@@ -194,6 +194,11 @@ def test_table_batch_matches_alone(table):
     assert b.classes() == 3
     with pytest.raises(sns.GpuError):  # shapes are fixed while in a batch
         other.set_launch(fft_n1=8)
+    with pytest.raises(sns.GpuError):  # so is the treatment
+        other.set_treatment(0, 1)
+    with pytest.raises(sns.GpuError):  # the batch's graphs hold its fields
+        other.close()
+    assert other.ctx and other.state is not None
     b.advance(6)
     b.sync()
     for m, inst, ref in zip(MEMBERS, insts, alone):

@@ -503,6 +503,28 @@ def test_device_copy(sns):
     inst.close()
 
 
+@pytest.mark.parametrize("loads", [1, 2, 4, 8])
+@pytest.mark.parametrize("nontemporal", [False, True])
+def test_device_copy_shaped(sns, loads, nontemporal):
+    """snsg_copy_shaped: bitwise copies of whole tiles and of a ragged tail
+    (a word count that is no multiple of the tile), nothing written past
+    the end; a load count other than 1, 2, 4, 8 is rejected."""
+    inst = sns.Instance(16, 16)
+    n16 = (1 << 18) + 37                       # 16-byte words
+    n = 2 * n16 + 2                            # doubles in each buffer
+    src, dst = inst.malloc(8 * n), inst.malloc(8 * n)
+    x = np.random.default_rng(loads).standard_normal(n)
+    assert inst.L.snsg_upload(inst.ctx, src, x.ctypes.data, x.nbytes) == 0
+    inst.copy(dst, src, 16 * n16, loads=loads, nontemporal=nontemporal)
+    y = np.ones(n)
+    assert inst.L.snsg_download(inst.ctx, y.ctypes.data, dst, y.nbytes) == 0
+    assert np.array_equal(x[:2 * n16].view(np.uint64), y[:2 * n16].view(np.uint64))
+    assert np.array_equal(y[2 * n16:], np.zeros(2))   # snsg_malloc zero-fills
+    with pytest.raises(sns.GpuError):
+        inst.copy(dst, src, 16 * n16, loads=3)
+    inst.close()
+
+
 # ------------------------------------------------------------ diagnostics
 
 DIAG_CASES = [
