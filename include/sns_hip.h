@@ -165,6 +165,19 @@ int snsg_state_download(snsg_ctx *ctx, const snsg_state *st, int field, double *
 int snsg_diagnostics(snsg_ctx *ctx, const double *ux, const double *uy, const double *temp,
                      double *out);
 int snsg_state_diagnostics(snsg_ctx *ctx, const snsg_state *st, double *out);
+/* snsg_diagnostics without allocating: no counterpart in the reference;
+ * for callers that cannot allocate device or pinned host memory (code under
+ * HIP graph capture, tight serving loops).  Same inputs and the same 11
+ * results bit for bit, written to out_dev, a device array of 11 doubles;
+ * asynchronous on the instance's stream (no host synchronisation).  The
+ * per-workgroup partial sums live in a file-scope device array of the
+ * library, 2048 x 11 doubles (176 KiB) per GPU, held until the process
+ * ends; a grid needing more partial slots is refused with SNSG_E_SIZE (the
+ * monitors use at most 2048, so none is today).  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other streams, other
+ * threads) race and must be serialised by the caller. */
+int snsg_diagnostics_noalloc(snsg_ctx *ctx, const double *ux, const double *uy,
+                             const double *temp, double *out_dev);
 /* ---- statistics (src/statistics/collect.c:10-75): ux1 += ux, ux2 += ux^2
  * (I = 1..itot+1), uy1/uy2 and temp1/temp2 likewise (I = 0..itot+1), rows
  * 1..jtot; accumulators in the reference layouts (include/arrays/statistics.h

@@ -114,6 +114,7 @@ def _load_hip():
         L.snsg_state_decide_dt.argtypes = [_vp, C.POINTER(SnsgState), _dp]
         L.snsg_diagnostics.argtypes = [_vp, _vp, _vp, _vp, _dp]
         L.snsg_state_diagnostics.argtypes = [_vp, C.POINTER(SnsgState), _dp]
+        L.snsg_diagnostics_noalloc.argtypes = [_vp, _vp, _vp, _vp, _vp]
         L.snsg_statistics_collect.argtypes = [_vp] * 10
         L.snsg_state_init_temperature.argtypes = [_vp, C.POINTER(SnsgState), C.c_uint64,
                                                   C.c_double]

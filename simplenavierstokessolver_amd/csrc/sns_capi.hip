@@ -261,6 +261,42 @@ int ctx_configure(snsg_ctx *c, const snsg_config *cfg, const double *dxf, const 
   return 0;
 }
 
+// ---------------------------------------- scratch of the no-allocation calls
+//
+// snsg_diagnostics_noalloc keeps the monitors' per-workgroup partial sums in
+// this file-scope device array instead of instance memory.  Each GPU holds
+// its own copy for the life of the process, at its own address: the host
+// takes that address per device with hipGetSymbolAddress, once, while the
+// device is current (at the first snsg_create on it), and uses only that
+// address.  The array's name in host code is a host-side stand-in, never a
+// device address.
+constexpr int kScratchPartials = 2048;  // partial slots (diag_partials <= 2048)
+constexpr size_t kScratchDoubles = (size_t)kScratchPartials * kDiagSlots;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+static std::mutex g_scratch_mu;
+static std::map<int, double *> g_scratch_addr;  // device -> address of scratch
+
+// With `device` current: look up (once per device) the device address of
+// scratch.
+static int scratch_setup(int device) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch_addr.count(device)) return 0;
+  double *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || ptr == nullptr)
+    return set_err(e != hipSuccess ? (int)e : SNSG_E_ARG,
+                   "hipGetSymbolAddress(scratch) during instance set-up");
+  g_scratch_addr[device] = ptr;
+  return 0;
+}
+
+static double *scratch_addr(int device) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  auto it = g_scratch_addr.find(device);
+  return it == g_scratch_addr.end() ? nullptr : it->second;
+}
+
 extern "C" int snsg_create(int device, const snsg_config *cfg, const double *dxf,
                            const double *dxc, snsg_ctx **out) {
   if (!cfg || !dxf || !dxc || !out) return set_err(SNSG_E_ARG, "snsg_create: null argument");
@@ -270,6 +306,7 @@ extern "C" int snsg_create(int device, const snsg_config *cfg, const double *dxf
   int rc = check_sizes(cfg, pl, lt.fft_n1);
   if (rc) return rc;
   HIPCHECK(hipSetDevice(device));
+  if ((rc = scratch_setup(device))) return rc;
   snsg_ctx *c = new snsg_ctx();
   c->device = device;
   c->cfg = *cfg;
@@ -900,6 +937,32 @@ extern "C" int snsg_diagnostics(snsg_ctx *c, const double *ux, const double *uy,
 extern "C" int snsg_state_diagnostics(snsg_ctx *c, const snsg_state *st, double *out) {
   if (!c || !st || !st->ux || !out) return set_err(SNSG_E_ARG, __func__);
   return diagnostics(c, c->geo_st, st->ux, st->uy, st->temp, out);
+}
+
+// snsg_diagnostics with no allocation and no host synchronisation: the
+// same kernels and partial count (so the same bits), the partials in the
+// device's scratch, the 11 results written to the caller's device array.
+// Nothing in scratch needs to start at zero: every partial slot the final
+// reduction reads is written by the first kernel of the same call.
+extern "C" int snsg_diagnostics_noalloc(snsg_ctx *c, const double *ux, const double *uy,
+                                        const double *temp, double *out_dev) {
+  if (!c || !ux || !uy || !out_dev || (c->cfg.with_temperature && !temp))
+    return set_err(SNSG_E_ARG, __func__);
+  const int nb = diag_partials(c->geo);
+  if (nb > kScratchPartials) {
+    char msg[160];
+    snprintf(msg, sizeof msg,
+             "snsg_diagnostics_noalloc: %d partial slots exceed the scratch's %d", nb,
+             kScratchPartials);
+    return set_err(SNSG_E_SIZE, msg);
+  }
+  double *part = scratch_addr(c->device);
+  if (!part)
+    return set_err(SNSG_E_ARG, "snsg_diagnostics_noalloc: no scratch address for the "
+                               "instance's device");
+  const DiagArgs a{ux, uy, c->cfg.with_temperature ? temp : nullptr};
+  launch_diag(c->stream, c->geo, a, part, 1., c->cfg.ly, out_dev);
+  return check_launch(__func__);
 }
 
 extern "C" int snsg_statistics_collect(snsg_ctx *c, const double *ux, const double *uy,
