@@ -35,6 +35,25 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 B_ALG_STEP = 600.0  # bytes per cell-step, explicit-y treatments
 
+
+def bytes_per_cell_step(implicity):
+    """Compulsory HBM bytes per cell and RK3 step: 75 FP64 words, plus 27
+    for the implicit-y solve (write, read the increment, read the field, per
+    field and substage)."""
+    return 816.0 if implicity else 600.0
+
+
+def mix_roofline(value, jobs_weights):
+    """HBM roofline of a mix of jobs: jobs_weights = [(job, cell-steps)];
+    the cell-step-weighted bytes per cell-step times the measured rate."""
+    tot = sum(w for _, w in jobs_weights)
+    if tot <= 0:
+        return None
+    b = sum(bytes_per_cell_step(j["implicity"]) * w for j, w in jobs_weights) / tot
+    ach = value * b / 1e9
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "bytes_per_cell_step": b}
+
 # Algorithmic (compulsory) HBM bytes per interior cell for one launch of each
 # kernel of the step: each array the kernel must read or write, once.
 KERNEL_BYTES_PER_CELL = {
@@ -111,7 +130,40 @@ def per_kernel_times(sns, inst, steps):
     return out
 
 
+def host_cpu():
+    """CPU model, logical CPUs (all, and those this process may run on) and
+    physical cores of the host, from /proc/cpuinfo."""
+    model, cores = None, set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(),
+            "cpus_available": len(os.sched_getaffinity(0)),
+            "physical_cores": len(cores) or None}
+
+
 def cpu_baseline(itot, jtot, nsteps):
+    """The oracle (this build's C restatement of the reference algorithm,
+    one rank, OpenMP over independent rows and columns) timed on the host
+    on `nsteps` steps of the workload.  The reference itself cannot be
+    built here: its FFTW3 dependency is absent, so the oracle's transforms
+    are its own recursive mixed-radix FFT with FFTW's r2c/c2r conventions."""
     from oracle import pyoracle as po
     w = WORKLOAD
     o = po.Oracle(itot, jtot, ly=w["ly"], stretch=float(itot), Ra=w["Ra"], Pr=w["Pr"])
@@ -122,12 +174,16 @@ def cpu_baseline(itot, jtot, nsteps):
     o.run(nsteps)
     el = time.perf_counter() - t
     nthr = po.num_threads()
-    return {"value": itot * jtot * nsteps / el, "unit": "cell-steps/s", "cores": nthr,
-            "kind": "port",
-            "sample": "%d RK3 step(s) (time-step decision + 3 substages) of the same %dx%d "
-                      "workload from the same initial state, CPU oracle (C, gcc -O2, OpenMP "
-                      "%d threads)" % (nsteps, itot, jtot, nthr),
-            "seconds": el}
+    res = {"value": itot * jtot * nsteps / el, "unit": "cell-steps/s", "cores": nthr,
+           "kind": "port",
+           "sample": "%d RK3 step(s) (time-step decision + 3 substages) of the same %dx%d "
+                     "workload from the same initial state, CPU oracle (C, gcc -O2, OpenMP "
+                     "%d threads)" % (nsteps, itot, jtot, nthr),
+           "threads": nthr, "steps": nsteps, "seconds": el,
+           "fft": "the oracle's recursive mixed-radix FFT (FFTW3 is absent from the image)",
+           "decomposition": "one rank (the reference's y slabs at mpisize 1), no MPI"}
+    res.update(host_cpu())
+    return res
 
 
 COPY_SHAPES = [(0, False)] + [(u, nt) for nt in (False, True) for u in (1, 2, 4, 8)]
@@ -234,7 +290,10 @@ def ensemble_run(sns, steps, warmup=2):
         k = ens.TREATMENTS[(j["implicitx"], j["implicity"])]
         mix[k] = mix.get(k, 0) + 1
     v_b, v_p = cells * steps / t_batch, cells * steps / t_plain
+    weights = [(j, j["nx"] * j["ny"] * steps) for j in jobs]
     return {"config": ens.ENSEMBLE, "value": v_b, "unit": "cell-steps/s",
+            "roofline": mix_roofline(v_b, weights),
+            "plain_roofline": mix_roofline(v_p, weights),
             "ms_per_ensemble_step": 1e3 * t_batch / steps,
             "plain_value": v_p, "plain_ms_per_ensemble_step": 1e3 * t_plain / steps,
             "speedup_vs_plain": v_b / v_p, "shape_classes": nclass, "treatments": mix,
@@ -264,9 +323,9 @@ def served_run(sns, window, slots=0):
             break
         while k < len(jobs) and jobs[k]["arrival"] <= now:
             j = jobs[k]
-            srv.submit(j["nx"], j["ny"], j["steps"], Ra=j["Ra"], Pr=j["Pr"],
-                       implicitx=j["implicitx"], implicity=j["implicity"], seed=j["seed"],
-                       amplitude=j["amplitude"])
+            j["id"] = srv.submit(j["nx"], j["ny"], j["steps"], Ra=j["Ra"], Pr=j["Pr"],
+                                 implicitx=j["implicitx"], implicity=j["implicity"],
+                                 seed=j["seed"], amplitude=j["amplitude"])
             k += 1
         if now >= next_log:
             st = srv.stats()
@@ -276,13 +335,15 @@ def served_run(sns, window, slots=0):
         nxt = jobs[k]["arrival"] if k < len(jobs) else window
         time.sleep(max(0.0, min(0.005, nxt - now)))
     st = srv.stats()
+    weights = [(j, srv.status(j["id"])[1] * j["nx"] * j["ny"]) for j in jobs[:k]]
     srv.close()
     mix = {}
     for j in jobs[:k]:
         key = ens.TREATMENTS[(j["implicitx"], j["implicity"])]
         mix[key] = mix.get(key, 0) + 1
-    return {"config": ens.SERVED, "window_s": window, "value": st["cell_steps"] / st["busy_s"]
-            if st["busy_s"] > 0 else 0.0, "unit": "cell-steps/s",
+    value = st["cell_steps"] / st["busy_s"] if st["busy_s"] > 0 else 0.0
+    return {"config": ens.SERVED, "window_s": window, "value": value, "unit": "cell-steps/s",
+            "roofline": mix_roofline(value, weights),
             "p50_step_ms": st["p50_ms"], "p99_step_ms": st["p99_ms"], "max_step_ms": st["max_ms"],
             "submitted": st["submitted"], "admitted": st["admitted"],
             "completed": st["completed"], "job_steps": st["steps"], "lanes": st["lanes"],

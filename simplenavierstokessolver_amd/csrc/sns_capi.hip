@@ -226,6 +226,9 @@ int ctx_configure(snsg_ctx *c, const snsg_config *cfg, const double *dxf, const 
                  same(t[5], 0, itot - 2, cxp) && memcmp(&cxp, &tb.u_cT, sizeof cxp) == 0 &&
                  same(t[6], 1, itot - 1, tb.u_cU) && same(t[7], 1, itot - 1, uxp) &&
                  memcmp(&uxp, &tb.u_cU, sizeof uxp) == 0;
+    // the Poisson substitutions read l for rows 1..itot-1 and u for rows
+    // 0..itot-2 (al of row 0 and be of the last row are 0)
+    tb.plu_const = itot >= 3 && same(t[12], 1, itot - 1, tb.plc) && same(t[13], 0, itot - 2, tb.puc);
   }
   // every divisor (DXF(1..itot), DXC(1..itot+1), dy) a power of two with an
   // exact reciprocal: divisions become multiplications, bit for bit
@@ -848,7 +851,7 @@ extern "C" int snsg_instance_bytes(const snsg_config *cfg, size_t *bytes) {
     const int L = launch_key((int)itot, (int)jtot, tuned_launch((int)itot, (int)jtot)).chunk;
     if (L > 1) n += d * 9 * ysolve_table_doubles(L) + 3 * d * (jtot / L) * itot;
   }
-  if (cfg->implicitx) n += d * 27 * (size_t)lu_table_entries((int)itot);
+  if (cfg->implicitx) n += d * lu_table_doubles(lu_table_entries((int)itot));
   Geo g{};
   g.itot = (int)itot;
   g.jtot = (int)jtot;
@@ -1044,7 +1047,7 @@ int step_prepare(snsg_ctx *c) {
   }
   if (c->treat.implicitx && !c->d_lu) {
     c->lu_ntm = lu_table_entries(itot);
-    if ((rc = ctx_alloc((void **)&c->d_lu, sizeof(double) * 27 * (size_t)c->lu_ntm, &c->mem))) return rc;
+    if ((rc = ctx_alloc((void **)&c->d_lu, sizeof(double) * lu_table_doubles(c->lu_ntm), &c->mem))) return rc;
   }
   if (!c->d_part) {
     c->nparts = corr_partials(g);
@@ -1107,6 +1110,7 @@ void step_args(const snsg_ctx *c, const snsg_state *st, bool reduce, StepArgs &s
     pa.with_temp = with_temp;
     pa.forcing = with_temp && c->cfg.with_thermal_forcing;
     pa.lu = c->d_lu ? c->d_lu + (size_t)(rk * 3 * 3) * c->lu_ntm : nullptr;
+    pa.lu_hdr = c->d_lu ? c->d_lu + 27 * (size_t)c->lu_ntm + rk * 3 * kLuHdr : nullptr;
     pa.ntm = c->lu_ntm;
     const double *ins[3] = {temp, st->ux, st->uy};
     double *outs[3] = {temp_alt, st->ux_star, st->uy_star};

@@ -81,6 +81,11 @@ struct Tables {
   // is a power of two: the kernels then divide by multiplying with the
   // tabulated reciprocal (see dv).
   int pow2;
+  // Set when the Poisson row coefficients the x substitutions use are each
+  // one value bit for bit: pl(I) for rows 1..itot-1 (plc) and pu(I) for rows
+  // 0..itot-2 (puc), 0-based; the x solve then takes them as constants.
+  int plu_const;
+  double plc, puc;
   __host__ __device__ const double *tab(int k) const { return t + k * stride; }
   __host__ __device__ const double *dxf() const { return tab(0); }    // DXF(I)
   __host__ __device__ const double *dxc() const { return tab(1); }    // DXC(I)

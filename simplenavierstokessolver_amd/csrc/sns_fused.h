@@ -19,6 +19,7 @@ struct PredArgs {
   int with_temp;                          // temperature equation integrated
   int forcing;                            // buoyancy term from T
   const double *lu;                       // implicit x: LU tables of the substage (3 fields)
+  const double *lu_hdr;                   // their headers (kLuHdr doubles per field)
   int ntm;                                // entries per LU table
 };
 
@@ -45,7 +46,11 @@ struct DtArgs {
 
 // Thomas factorisation tables of the three CN-x operators for the three
 // substages: out holds 9 groups (substage s, field f at (s*3+f)) of 3 tables
-// of ntm entries each (see sns_lu.h)
+// of ntm entries each (see sns_lu.h), then 9 headers of kLuHdr doubles:
+// the row P from which the rows up to n-2 all repeat one (r, al, be) triple
+// bit for bit, and that triple (lu_table_doubles in all)
+constexpr int kLuHdr = 4;
+inline size_t lu_table_doubles(int ntm) { return 27 * (size_t)ntm + 9 * kLuHdr; }
 struct LuFactorArgs {
   Geo g;
   Tables tb;

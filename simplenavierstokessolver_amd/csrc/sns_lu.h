@@ -30,6 +30,30 @@ struct LuTab {
   __device__ __forceinline__ double get_be(int k, int nt) const { return be[k * nt + threadIdx.x]; }
 };
 
+// LuTab with the repeated rows folded: rows P..n-2 all hold the triple
+// (rc, alc, bec) bit for bit (k_lu_factor's header), so only rows below P,
+// the last row and the padding rows (zeros) are read from the tables; a
+// thread whose chunk lies in P..n-2 reads nothing.
+template <int M>
+struct LuTabC {
+  const double *r, *al, *be;  // each NT*M entries, interleaved
+  int P, n;
+  double rc, alc, bec;
+  __device__ __forceinline__ bool in_table(int k) const {
+    const int row = threadIdx.x * M + k;
+    return row < P || row >= n - 1;
+  }
+  __device__ __forceinline__ double get_r(int k, int nt) const {
+    return in_table(k) ? r[k * nt + threadIdx.x] : rc;
+  }
+  __device__ __forceinline__ double get_al(int k, int nt) const {
+    return in_table(k) ? al[k * nt + threadIdx.x] : alc;
+  }
+  __device__ __forceinline__ double get_be(int k, int nt) const {
+    return in_table(k) ? be[k * nt + threadIdx.x] : bec;
+  }
+};
+
 // The same factors when al_i = -(r_i*l_i) and be_i = -(r_i*u_i) hold for
 // 0 < i < n-1 with row coefficients l, u that do not depend on the system
 // (only r does): l and u are interleaved tables like r.  Row 0 has al = 0 and
@@ -55,15 +79,20 @@ struct LuTabLU {
 // pivot from row pfx on equal, bit for bit, up to row n-2), so only rows up
 // to pfx and the last row are read from the table: a thread whose chunk
 // holds none of them reads nothing.
+// With cst set, l of rows 1..n-1 and u of rows 0..n-2 are the constants lc,
+// uc (the table entries themselves), so the solve reads no coefficient table.
 template <int M>
 struct LuTabLUr {
   double rr[M];
   const double *l, *u;
   int n;
   double up0;
+  bool cst;
+  double lc, uc;
   __device__ __forceinline__ LuTabLUr(const double *r, const double *l_, const double *u_,
-                                      int n_, double up0_, int pfx, double rinf, int nt)
-      : l(l_), u(u_), n(n_), up0(up0_) {
+                                      int n_, double up0_, int pfx, double rinf, int nt,
+                                      bool cst_ = false, double lc_ = 0., double uc_ = 0.)
+      : l(l_), u(u_), n(n_), up0(up0_), cst(cst_), lc(lc_), uc(uc_) {
     const int row0 = threadIdx.x * M;
     if (row0 <= pfx || row0 + M - 1 >= n - 1) {
 #pragma unroll
@@ -79,11 +108,12 @@ struct LuTabLUr {
   __device__ __forceinline__ double get_r(int k, int) const { return rr[k]; }
   __device__ __forceinline__ double get_al(int k, int nt) const {
     const int row = threadIdx.x * M + k;
-    return row == 0 ? 0. : -(rr[k] * l[k * nt + threadIdx.x]);
+    return row == 0 ? 0. : -(rr[k] * (cst ? lc : l[k * nt + threadIdx.x]));
   }
   __device__ __forceinline__ double get_be(int k, int nt) const {
     const int row = threadIdx.x * M + k;
-    return row == n - 1 ? 0. : (row == 0 ? -up0 : -(rr[k] * u[k * nt + threadIdx.x]));
+    return row == n - 1 ? 0.
+                        : (row == 0 ? -up0 : -(rr[k] * (cst ? uc : u[k * nt + threadIdx.x])));
   }
 };
 

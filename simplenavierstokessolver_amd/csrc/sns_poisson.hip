@@ -375,7 +375,8 @@ struct PxLds {
 // One workgroup per wavenumber row; the row is staged through LDS (coalesced
 // global access, chunk order for the substitutions).
 template <int NT, int M>
-__device__ __forceinline__ void poisson_x3(const Geo &g, const double *__restrict__ fac,
+__device__ __forceinline__ void poisson_x3(const Geo &g, const Tables &tb,
+                                           const double *__restrict__ fac,
                                            const double *__restrict__ plu, int ntm,
                                            double2 *__restrict__ psihat) {
   extern __shared__ double lds_raw[];
@@ -396,7 +397,7 @@ __device__ __forceinline__ void poisson_x3(const Geo &g, const double *__restric
   const int nJ = g.jtot / 2 + 1;
   const double *tail = fac + (size_t)nJ * ntm;
   const LuTabLUr<M> tab(fac + (size_t)J * ntm, plu, plu + ntm, n, tail[J], (int)tail[nJ + J],
-                        tail[2 * nJ + J], NT);
+                        tail[2 * nJ + J], NT, tb.plu_const != 0, tb.plc, tb.puc);
   lu_solve_shared<NT, M, 2>(d, tab, L.scr);
 #pragma unroll
   for (int k = 0; k < M; k++) {
@@ -439,7 +440,7 @@ __global__ void __launch_bounds__(NTF, 8) k_pfwd_b(S src) {
 template <int NT, int M, class S>
 __global__ void __launch_bounds__(NT) k_poisson_x3(S src) {
   const PoisArgs a = src.get();
-  poisson_x3<NT, M>(a.g, a.fac, a.plu, NT * M, a.psihat);
+  poisson_x3<NT, M>(a.g, a.tb, a.fac, a.plu, NT * M, a.psihat);
 }
 
 template <int L, class S>

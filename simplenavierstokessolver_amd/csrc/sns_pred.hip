@@ -142,6 +142,54 @@ __global__ void __launch_bounds__(NT) k_lu_factor(S src) {
     base[1 * a.ntm + k * NT + t] = al[k];
     base[2 * a.ntm + k * NT + t] = be[k];
   }
+  // Header: P = the last row in 1..n-2 whose (r, al, be) differs bit for bit
+  // from the row before (0 if none), so that rows P..n-2 all hold the
+  // triple of row n-2; on a grid whose interior rows are alike the pivot
+  // recurrence reaches that fixed point after a few rows.  Small systems
+  // (n < 4) keep every row in the table (P = n).
+  __shared__ double wlast[NW][3];
+  __shared__ int wmax[NW];
+  auto bits = [](double v) { return __double_as_longlong(v); };
+  if (lane == 63) {
+    wlast[wave][0] = rr[M - 1];
+    wlast[wave][1] = al[M - 1];
+    wlast[wave][2] = be[M - 1];
+  }
+  __syncthreads();
+  double pr = __shfl_up(rr[M - 1], 1), pa = __shfl_up(al[M - 1], 1), pb = __shfl_up(be[M - 1], 1);
+  if (lane == 0 && wave > 0) {
+    pr = wlast[wave - 1][0];
+    pa = wlast[wave - 1][1];
+    pb = wlast[wave - 1][2];
+  }
+  int pt = 0;
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const int r = t * M + k;
+    const double qr = k ? rr[k - 1] : pr, qa = k ? al[k - 1] : pa, qb = k ? be[k - 1] : pb;
+    if (r >= 1 && r <= n - 2 &&
+        (bits(rr[k]) != bits(qr) || bits(al[k]) != bits(qa) || bits(be[k]) != bits(qb)))
+      pt = r;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) pt = max(pt, __shfl_xor(pt, off));
+  if (lane == 0) wmax[wave] = pt;
+  double *hdr = a.out + 27 * (size_t)a.ntm + (s * 3 + f) * kLuHdr;
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    if (t * M + k == n - 2) {
+      hdr[1] = rr[k];
+      hdr[2] = al[k];
+      hdr[3] = be[k];
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+    int P = 0;
+    for (int w = 0; w < NW; w++) P = max(P, wmax[w]);
+    hdr[0] = (double)(n < 4 ? n : P);
+    if (n < 2) hdr[1] = hdr[2] = hdr[3] = 0.;
+  }
 }
 
 // -------------------------------------------------------- row predictor
@@ -441,7 +489,8 @@ __device__ __forceinline__ void pred_rows_body(const PredArgs &a, PredRowsLds<NT
       dd[s][k] = r < n ? L.d[s][pad_idx<M>(r)] : 0.;
     }
   const double *b = a.lu + (size_t)(F * 3) * a.ntm;
-  const LuTab tab = {b, b + a.ntm, b + 2 * a.ntm};
+  const double *h = a.lu_hdr + F * kLuHdr;
+  const LuTabC<M> tab = {b, b + a.ntm, b + 2 * a.ntm, (int)h[0], n, h[1], h[2], h[3]};
   lu_solve_shared<NT, M, 2>(dd, tab, L.scr);
 #pragma unroll
   for (int s = 0; s < 2; s++)
