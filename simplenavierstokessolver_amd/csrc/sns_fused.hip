@@ -155,13 +155,25 @@ __global__ void __launch_bounds__(kCorrThreads) k_corr(S src) {
   const double gdt = a.gamma * dt;
   const double fac = -gdt / g.two_Re;
   const size_t wc = (size_t)g.pc, wu = (size_t)g.pu;
-  const double dxc_i = ldr(tb.dxc(), ti), rdxc_i = ldr(tb.rdxc(), ti);  // DXC(i), 1./DXC(i)
-  double dxc_p = 0., rdxc_p = 0., dxf_i = 0., rdxf_i = 0.;
-  if (a.tr.implicitx) {
-    dxc_p = from_right(dxc_i);  // DXC(i+1)
-    rdxc_p = from_right(rdxc_i);
-    dxf_i = ldr(tb.dxf(), ti);
-    rdxf_i = ldr(tb.rdxf(), ti);
+  // column metrics; on a uniform grid a wave whose 64 columns all lie in
+  // 2..itot-1 takes them as constants (the table entries themselves)
+  const int c0 = i - lane;
+  const bool interior = tb.uniform && c0 >= 2 && c0 + 63 <= itot - 1;
+  double dxc_i, rdxc_i, dxc_p = 0., rdxc_p = 0., dxf_i = 0., rdxf_i = 0.;
+  if (interior) {
+    dxc_i = dxc_p = tb.u_dxc;
+    rdxc_i = rdxc_p = tb.u_rdxc;
+    dxf_i = tb.u_dxf;
+    rdxf_i = tb.u_rdxf;
+  } else {
+    dxc_i = ldr(tb.dxc(), ti);  // DXC(i), 1./DXC(i)
+    rdxc_i = ldr(tb.rdxc(), ti);
+    if (a.tr.implicitx) {
+      dxc_p = from_right(dxc_i);  // DXC(i+1)
+      rdxc_p = from_right(rdxc_i);
+      dxf_i = ldr(tb.dxf(), ti);
+      rdxf_i = ldr(tb.rdxf(), ti);
+    }
   }
   double m = 5.e-2;
   double psi_s = ldr(a.psi + (size_t)(j0 - 1) * wc, ci);  // psi(i, j-1)

@@ -12,6 +12,12 @@ the GPU by `measure_costs`, as src/param/estimate_cost.c measures it for one
 run, and recorded in tuning/treatment_costs_gfx950.json:
 
     python -m simplenavierstokessolver_amd.ensemble tune [--out FILE]
+
+`trace` runs the mixed ensemble as one batch and brackets a few batched
+steps with two marker copies, for rocprofv3 --kernel-trace and the
+`timeline` summary of simplenavierstokessolver_amd.profiling:
+
+    python -m simplenavierstokessolver_amd.ensemble trace [--steps N]
 """
 import argparse
 import json
@@ -134,6 +140,44 @@ def load_costs(path=COSTS_FILE):
         return json.load(f)["seconds_per_step"]
 
 
+def trace_batch(sns, steps=1, warmup=2):
+    """The mixed ensemble as one batch, `steps` batched steps bracketed by
+    two 16-byte device copies (marker dispatches of k_copy, each followed by
+    a synchronisation) for a kernel trace; returns the host wall time of the
+    bracketed steps in seconds."""
+    jobs = ensemble_jobs()
+    costs = load_costs()
+    for j in jobs:
+        j["implicitx"], j["implicity"] = job_treatment(costs, j, sns.grid)
+    sns.load_tuning()
+    insts = []
+    for j in jobs:
+        inst = sns.Instance(j["nx"], j["ny"], ly=2.0, stretch=float(j["nx"]), Ra=j["Ra"],
+                            Pr=j["Pr"], implicitx=j["implicitx"], implicity=j["implicity"])
+        inst.state_alloc()
+        inst.state_init_temperature(j["seed"], j["amplitude"])
+        insts.append(inst)
+    batch = sns.Batch(insts)
+    batch.advance(warmup)
+    batch.sync()
+    m = insts[0]
+    src, dst = m.malloc(16), m.malloc(16)
+    m.copy(dst, src, 16)
+    m.sync()
+    t = time.perf_counter()
+    batch.advance(steps)
+    batch.sync()
+    el = time.perf_counter() - t
+    m.copy(dst, src, 16)
+    m.sync()
+    print(json.dumps({"jobs": len(jobs), "classes": batch.classes(), "steps": steps,
+                      "host_s": el}), flush=True)
+    batch.close()
+    for i in insts:
+        i.close()
+    return el
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -141,8 +185,14 @@ def main(argv=None):
     t.add_argument("--out", default=COSTS_FILE)
     t.add_argument("--sizes", type=int, nargs="+", default=[128, 256, 512, 1024, 2048])
     t.add_argument("--steps", type=int, default=10)
+    tr = sub.add_parser("trace", help="batched ensemble steps between two marker copies")
+    tr.add_argument("--steps", type=int, default=1)
+    tr.add_argument("--warmup", type=int, default=2)
     a = ap.parse_args(argv)
     import simplenavierstokessolver_amd as sns
+    if a.cmd == "trace":
+        trace_batch(sns, a.steps, a.warmup)
+        return
     costs = measure_costs(sns, a.sizes, a.steps)
     os.makedirs(os.path.dirname(a.out), exist_ok=True)
     with open(a.out, "w") as f:

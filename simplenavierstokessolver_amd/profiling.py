@@ -9,6 +9,9 @@
   python -m simplenavierstokessolver_amd.profiling counters CSV [CSV ...]
         [--out FILE] [--command TEXT ...]
 
+  python -m simplenavierstokessolver_amd.profiling timeline KERNEL_TRACE_CSV
+        [--marker NAME] [--steps N] [--out FILE] [--command TEXT ...]
+
 `stats` condenses rocprofv3's kernel_stats.csv into a table of kernel groups
 (the names bench.py and the C-ABI timing use).  `traffic` averages the
 FETCH_SIZE and WRITE_SIZE counters (KiB per dispatch) of every kernel over
@@ -19,6 +22,12 @@ with the kernel's algorithmic bytes (bench.KERNEL_BYTES_PER_CELL x cells).
 `counters` tabulates every counter of the given counter_collection.csv
 files as its mean per dispatch of each kernel, with the L2 hit rate
 TCC_HIT / (TCC_HIT + TCC_MISS) when both are there.
+`timeline` reads a rocprofv3 kernel_trace.csv and takes the window between
+the last two dispatches of the marker kernel (a 16-byte k_copy the traced
+program issues before and after the steps it wants measured): the share of
+the window in which no kernel runs, and per shape class (the dispatches'
+grid z, which the batched kernels set to the class's member count, plus
+their stream) the kernel time and the time any of its kernels runs.
 """
 import argparse
 import collections
@@ -121,6 +130,58 @@ def counter_table(paths):
     return names, with_kernels
 
 
+def union_length(intervals):
+    """Total length covered by a set of [start, end) intervals."""
+    tot, cur_s, cur_e = 0, None, None
+    for s, e in sorted(intervals):
+        if cur_e is None or s > cur_e:
+            if cur_e is not None:
+                tot += cur_e - cur_s
+            cur_s, cur_e = s, e
+        else:
+            cur_e = max(cur_e, e)
+    if cur_e is not None:
+        tot += cur_e - cur_s
+    return tot
+
+
+def timeline(path, marker="k_copy"):
+    """Window between the last two marker dispatches: its length, the time
+    covered by kernels, and per (stream, grid z) class the summed kernel
+    time, the covered time and the dispatch count (times in ns)."""
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            rows.append(r)
+    marks = [r for r in rows if short_name(r["Kernel_Name"]) == marker]
+    if len(marks) < 2:
+        raise SystemExit("timeline: fewer than two %s dispatches in %s" % (marker, path))
+    marks.sort(key=lambda r: int(r["Start_Timestamp"]))
+    t0, t1 = int(marks[-2]["End_Timestamp"]), int(marks[-1]["Start_Timestamp"])
+    inside = [r for r in rows if int(r["Start_Timestamp"]) >= t0 and int(r["End_Timestamp"]) <= t1
+              and short_name(r["Kernel_Name"]) != marker]
+    if not inside:
+        raise SystemExit("timeline: no dispatch between the markers")
+    stream_key = "Stream_Id" if "Stream_Id" in rows[0] else "Queue_Id"
+    classes = collections.defaultdict(list)
+    for r in inside:
+        z = int(r.get("Grid_Size_Z", 1)) // max(1, int(r.get("Workgroup_Size_Z", 1)))
+        classes[(r.get(stream_key, "0"), z)].append(r)
+    span = max(int(r["End_Timestamp"]) for r in inside) - min(int(r["Start_Timestamp"])
+                                                               for r in inside)
+    iv = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in inside]
+    out = {"window_ns": t1 - t0, "span_ns": span, "busy_ns": union_length(iv),
+           "kernel_ns": sum(e - s for s, e in iv), "dispatches": len(inside), "classes": []}
+    for (stream, z), rs in sorted(classes.items(), key=lambda kv: -sum(
+            int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in kv[1])):
+        civ = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in rs]
+        names = collections.Counter(group_of(r["Kernel_Name"]) for r in rs)
+        out["classes"].append({"stream": stream, "members": z, "dispatches": len(rs),
+                               "kernel_ns": sum(e - s for s, e in civ),
+                               "busy_ns": union_length(civ), "groups": dict(names)})
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -140,6 +201,13 @@ def main(argv=None):
     s3.add_argument("csv", nargs="+")
     s3.add_argument("--out")
     s3.add_argument("--command", action="append", default=[])
+    s4 = sub.add_parser("timeline")
+    s4.add_argument("csv")
+    s4.add_argument("--marker", default="k_copy")
+    s4.add_argument("--steps", type=int, default=1, help="steps inside the window")
+    s4.add_argument("--out")
+    s4.add_argument("--json")
+    s4.add_argument("--command", action="append", default=[])
     a = ap.parse_args(argv)
     lines = []
     if a.command:
@@ -166,6 +234,29 @@ def main(argv=None):
                 h, m = row.get("TCC_HIT_sum", (0, 0.))[1], row.get("TCC_MISS_sum", (0, 0.))[1]
                 line += " %s |" % ("%.1f" % (100. * h / (h + m)) if h + m > 0 else "-")
             lines.append(line)
+    elif a.cmd == "timeline":
+        t = timeline(a.csv, a.marker)
+        w = t["window_ns"]
+        lines += ["Window between the marker dispatches: %.3f ms for %d step(s), %.3f ms per "
+                  "step; %d dispatches." % (w / 1e6, a.steps, w / 1e6 / a.steps, t["dispatches"]),
+                  "",
+                  "- time with at least one kernel running: %.3f ms (%.1f%% of the window)"
+                  % (t["busy_ns"] / 1e6, 100. * t["busy_ns"] / w),
+                  "- time with no kernel running (launch gaps, host work): %.3f ms (%.1f%%)"
+                  % ((w - t["busy_ns"]) / 1e6, 100. * (w - t["busy_ns"]) / w),
+                  "- summed kernel durations: %.3f ms (%.2f kernels running on average "
+                  "while busy)" % (t["kernel_ns"] / 1e6, t["kernel_ns"] / max(t["busy_ns"], 1)),
+                  "",
+                  "| stream | members (grid z) | dispatches | kernel time (ms) | share of kernel "
+                  "time | covered (ms) | kernels |", "|---|---:|---:|---:|---:|---:|---|"]
+        for c in t["classes"]:
+            lines.append("| %s | %d | %d | %.3f | %.1f%% | %.3f | %s |" % (
+                c["stream"], c["members"], c["dispatches"], c["kernel_ns"] / 1e6,
+                100. * c["kernel_ns"] / t["kernel_ns"], c["busy_ns"] / 1e6,
+                ", ".join("%s %d" % kv for kv in sorted(c["groups"].items()))))
+        if a.json:
+            with open(a.json, "w") as f:
+                json.dump(t, f, indent=1)
     else:
         t = traffic(a.fetch, a.write, a.copy_bytes, a.cells)
         lines += ["Calibration: k_copy of %d bytes read %.0f KiB (FETCH_SIZE) and wrote %.0f KiB "

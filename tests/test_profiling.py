@@ -52,3 +52,38 @@ def test_counter_table_and_hit_rate(tmp_path, capsys):
     prof.main(["counters", str(p)])
     out = capsys.readouterr().out
     assert "| k_pred_rows | pred | 2 | 80 | 20 | 80.0 |" in out
+
+
+def write_trace(path, rows):
+    with open(path, "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["Kernel_Name", "Stream_Id", "Grid_Size_Z", "Workgroup_Size_Z",
+                    "Start_Timestamp", "End_Timestamp"])
+        w.writerows(rows)
+
+
+def test_timeline_window_gaps_and_classes(tmp_path, capsys):
+    """Window between the last two markers; kernels of two classes overlap,
+    one gap of 10 ns; a dispatch outside the window is ignored."""
+    p = tmp_path / "t.csv"
+    write_trace(p, [(COPY, 0, 1, 1, 0, 5),                 # earlier marker, ignored
+                    (PRED, 1, 19, 1, 6, 50),               # before the window
+                    (COPY, 0, 1, 1, 100, 110),             # marker: window opens at 110
+                    (PRED, 1, 19, 1, 110, 150),
+                    (PRED, 2, 15, 1, 120, 140),
+                    (PRED, 1, 19, 1, 160, 190),
+                    (COPY, 0, 1, 1, 200, 205)])            # marker: window closes at 200
+    t = prof.timeline(str(p))
+    assert t["window_ns"] == 90 and t["dispatches"] == 3
+    assert t["busy_ns"] == 70 and t["kernel_ns"] == 90
+    (a, b) = t["classes"]
+    assert (a["members"], a["kernel_ns"], a["busy_ns"]) == (19, 70, 70)
+    assert (b["members"], b["kernel_ns"]) == (15, 20)
+    prof.main(["timeline", str(p), "--steps", "1"])
+    out = capsys.readouterr().out
+    assert "no kernel running" in out and "(22.2%)" in out
+
+
+def test_union_length():
+    assert prof.union_length([(0, 10), (5, 15), (20, 30)]) == 25
+    assert prof.union_length([]) == 0
