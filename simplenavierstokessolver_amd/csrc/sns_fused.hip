@@ -126,7 +126,7 @@ void launch_pred_batch(hipStream_t s, const PredArgs &shape, int implicitx, cons
 // ------------------------------------------------- correction + pressure
 
 // rows per workgroup of the correction kernel
-constexpr int kCorrRows = 4;
+constexpr int kCorrRows = 2;
 // output columns per wave: lanes 1..62; lanes 0 and 63 hold the neighbours
 constexpr int kCorrCols = 62;
 constexpr int kCorrThreads = 256;
