@@ -16,10 +16,11 @@ Printed JSON (one line): value = cell-steps/s, plus
                    8 TB/s HBM peak, using that kernel's algorithmic bytes;
   step_roofline -- the whole step against 600 B/cell-step of compulsory
                    traffic (75 FP64 words per cell and step);
-  cpu_baseline  -- the CPU oracle (a C port of the reference algorithm,
-                   OpenMP over rows and columns, OMP_NUM_THREADS threads)
-                   timed on a bounded sample of the same workload: --cpu-steps
-                   steps of the same grid from the same initial state.
+  cpu_baseline  -- the reference algorithm restated in C (the oracle) with
+                   the reference's y-slab MPI decomposition, one MPICH rank
+                   per available CPU, timed on a bounded sample of the same
+                   workload: --cpu-steps steps of the same grid from the
+                   same initial state.
 """
 import argparse
 import json
@@ -159,30 +160,50 @@ def host_cpu():
 
 
 def cpu_baseline(itot, jtot, nsteps):
-    """The oracle (this build's C restatement of the reference algorithm,
-    one rank, OpenMP over independent rows and columns) timed on the host
-    on `nsteps` steps of the workload.  The reference itself cannot be
-    built here: its FFTW3 dependency is absent, so the oracle's transforms
-    are its own recursive mixed-radix FFT with FFTW's r2c/c2r conventions."""
+    """The reference algorithm on the host cores, on `nsteps` steps of the
+    workload: the oracle's y-slab MPI restatement (the reference's
+    decomposition: rows split over the ranks, halo exchanges, MPI_Alltoallv
+    transposes for the y solves and the Poisson transforms; one rank per
+    available CPU, one thread each) under MPICH, or, where MPI is missing,
+    the single-rank oracle with OpenMP threads.  The reference itself cannot
+    be built here: its FFTW3 dependency is absent, so the transforms are the
+    oracle's recursive mixed-radix FFT with FFTW's r2c/c2r conventions."""
     from oracle import pyoracle as po
     w = WORKLOAD
-    o = po.Oracle(itot, jtot, ly=w["ly"], stretch=float(itot), Ra=w["Ra"], Pr=w["Pr"])
-    o.set_treatment(w["implicitx"], w["implicity"])
-    o.init_temperature(w["seed"], w["amplitude"], linear=False)
-    o.apply_boundaries()
-    t = time.perf_counter()
-    o.run(nsteps)
-    el = time.perf_counter() - t
-    nthr = po.num_threads()
-    res = {"value": itot * jtot * nsteps / el, "unit": "cell-steps/s", "cores": nthr,
-           "kind": "port",
-           "sample": "%d RK3 step(s) (time-step decision + 3 substages) of the same %dx%d "
-                     "workload from the same initial state, CPU oracle (C, gcc -O2, OpenMP "
-                     "%d threads)" % (nsteps, itot, jtot, nthr),
-           "threads": nthr, "steps": nsteps, "seconds": el,
-           "fft": "the oracle's recursive mixed-radix FFT (FFTW3 is absent from the image)",
-           "decomposition": "one rank (the reference's y slabs at mpisize 1), no MPI"}
-    res.update(host_cpu())
+    host = host_cpu()
+    ranks = host["cpus_available"]
+    fft = "the oracle's recursive mixed-radix FFT (FFTW3 is absent from the image)"
+    if os.path.exists(po.MPI_EXE) and po.mpiexec():
+        summ, _ = po.mpi_run(ranks, itot, jtot, w["Ra"], w["Pr"], w["implicitx"],
+                             w["implicity"], w["seed"], w["amplitude"], nsteps,
+                             stretch=float(itot), timeout=600)
+        res = {"value": summ["cell_steps_per_s"], "unit": "cell-steps/s", "cores": ranks,
+               "kind": "port",
+               "sample": "%d RK3 step(s) (time-step decision + 3 substages) of the same %dx%d "
+                         "workload from the same initial state, y-slab MPI restatement of the "
+                         "reference algorithm, %d MPICH ranks x 1 thread, step loop timed by "
+                         "MPI_Wtime" % (nsteps, itot, jtot, ranks),
+               "ranks": ranks, "threads_per_rank": 1, "steps": nsteps,
+               "seconds": summ["seconds"], "fft": fft,
+               "decomposition": "y slabs of the reference (src/parallel/others.c), halo "
+                                "exchanges, MPI_Alltoallv transposes"}
+    else:
+        o = po.Oracle(itot, jtot, ly=w["ly"], stretch=float(itot), Ra=w["Ra"], Pr=w["Pr"])
+        o.set_treatment(w["implicitx"], w["implicity"])
+        o.init_temperature(w["seed"], w["amplitude"], linear=False)
+        o.apply_boundaries()
+        t = time.perf_counter()
+        o.run(nsteps)
+        el = time.perf_counter() - t
+        nthr = po.num_threads()
+        res = {"value": itot * jtot * nsteps / el, "unit": "cell-steps/s", "cores": nthr,
+               "kind": "port",
+               "sample": "%d RK3 step(s) (time-step decision + 3 substages) of the same %dx%d "
+                         "workload from the same initial state, CPU oracle (C, gcc -O2, OpenMP "
+                         "%d threads)" % (nsteps, itot, jtot, nthr),
+               "threads": nthr, "steps": nsteps, "seconds": el, "fft": fft,
+               "decomposition": "one rank (MPI not available), OpenMP over rows and columns"}
+    res.update(host)
     return res
 
 

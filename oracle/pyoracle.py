@@ -309,3 +309,44 @@ def num_threads():
 
 def splitmix64(x):
     return lib().orc_splitmix64(x)
+
+
+# -- y-slab MPI restatement (oracle/sns_oracle_mpi) --------------------------
+
+MPI_EXE = os.path.join(_HERE, "sns_oracle_mpi")
+MPIEXEC_CANDIDATES = ["/opt/conda/bin/mpiexec"]
+
+
+def mpiexec():
+    """An MPI launcher that can run the MPI oracle, or None."""
+    for c in MPIEXEC_CANDIDATES:
+        if os.path.exists(c):
+            return c
+    return None
+
+
+def mpi_run(ranks, itot, jtot, Ra, Pr, implicitx, implicity, seed, amplitude, nsteps,
+            stretch=None, pin_gauge=False, out=None, timeout=None):
+    """Run the MPI oracle on `ranks` local ranks; returns its JSON summary
+    and, with `out` (a file path), the interior rows of ux, uy, temp, p."""
+    import json
+    exe, launcher = MPI_EXE, mpiexec()
+    if not os.path.exists(exe) or launcher is None:
+        raise FileNotFoundError("MPI oracle or mpiexec not available")
+    cmd = [launcher, "-launcher", "fork", "-n", str(int(ranks)), exe, str(itot), str(jtot),
+           repr(float(Ra)), repr(float(Pr)), str(int(implicitx)), str(int(implicity)),
+           str(int(seed)), repr(float(amplitude)), str(int(nsteps)),
+           repr(float(itot if stretch is None else stretch)), str(int(bool(pin_gauge)))]
+    if out:
+        cmd.append(out)
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    r = subprocess.run(cmd, capture_output=True, text=True, check=True, timeout=timeout, env=env)
+    summary = json.loads(r.stdout.strip().splitlines()[-1])
+    fields = None
+    if out:
+        d = np.fromfile(out)
+        fields, off = {}, 0
+        for n, w in (("ux", itot + 1), ("uy", itot + 2), ("temp", itot + 2), ("p", itot + 2)):
+            fields[n] = d[off:off + jtot * w].reshape(jtot, w)
+            off += jtot * w
+    return summary, fields

@@ -67,6 +67,14 @@ static void *xcalloc(size_t n, size_t sz){
   return ptr;
 }
 
+/* Hooks for the y-slab MPI restatement (sns_oracle_mpi.c, which includes
+ * this file): the halo exchange, the periodic y solve of an x-aligned
+ * increment and the global minimum of the advective bound.  NULL (this
+ * library): the single-rank versions. */
+static void (*g_halo_rows)(double *base, size_t pitch, int first_col, int cnt, int jtot) = NULL;
+static void (*g_solve_y)(int ni, int jtot, double prefactor, double dy, double *qx) = NULL;
+static void (*g_min_allreduce)(double *v) = NULL;
+
 /* ---------------------------------------------------------------- param */
 
 /* src/param/init.c:124-195 */
@@ -139,6 +147,7 @@ int orc_param_decide_dt(orc_param_t *param, const orc_state_t *s){
         dt_adv = fmin(dt_adv, dy/vel);
       }
     }
+    if(g_min_allreduce) g_min_allreduce(&dt_adv);  /* MPI_Allreduce MIN over the slabs */
     dt_adv *= safe_factors[0];
   }
   double dt_dif_x, dt_dif_y;
@@ -183,9 +192,15 @@ int orc_param_decide_dt(orc_param_t *param, const orc_state_t *s){
 
 /* single-rank halo exchange (src/parallel/halo.c:6-22): row 0 <- row jsize,
  * row jsize+1 <- row 1, over cnt entries starting at the given column */
-static void halo_rows(double *base, size_t pitch, int first_col, int cnt, int jtot){
+static void halo_rows_local(double *base, size_t pitch, int first_col, int cnt, int jtot){
   memcpy(base + 0*pitch + first_col, base + (size_t)jtot*pitch + first_col, sizeof(double)*cnt);
   memcpy(base + (size_t)(jtot+1)*pitch + first_col, base + 1*pitch + first_col, sizeof(double)*cnt);
+}
+
+
+static void halo_rows(double *base, size_t pitch, int first_col, int cnt, int jtot){
+  if(g_halo_rows) g_halo_rows(base, pitch, first_col, cnt, jtot);
+  else halo_rows_local(base, pitch, first_col, cnt, jtot);
 }
 
 /* src/fluid/boundary_conditions.c:6-19 */
@@ -624,7 +639,12 @@ static void temp_compute_src(const orc_param_t *param, orc_state_t *s){
 /* y-direction periodic Crank-Nicolson solve on an x-aligned increment
  * qx (ni x jtot, row pitch ni), shared by T, ux, uy
  * (src/temperature/update.c:168-220, src/fluid/update_velocity.c:281-333,425-477) */
+static void solve_y_periodic_local(int ni, int jtot, double prefactor, double dy, double *qx);
 static void solve_y_periodic(int ni, int jtot, double prefactor, double dy, double *qx){
+  if(g_solve_y) g_solve_y(ni, jtot, prefactor, dy, qx);
+  else solve_y_periodic_local(ni, jtot, prefactor, dy, qx);
+}
+static void solve_y_periodic_local(int ni, int jtot, double prefactor, double dy, double *qx){
   double *qy = xcalloc((size_t)ni*jtot, sizeof(double));
   transpose_d(ni, jtot, qx, qy);
 #pragma omp parallel
