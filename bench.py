@@ -14,8 +14,10 @@ Printed JSON (one line): value = cell-steps/s, plus
   roofline      -- the dominant kernel, timed with HIP events on the
                    instance's stream during the timed steps, against the
                    8 TB/s HBM peak, using that kernel's algorithmic bytes;
-  step_roofline -- the whole step against 600 B/cell-step of compulsory
-                   traffic (75 FP64 words per cell and step);
+  step_roofline -- the whole step against 520 B/cell-step of compulsory
+                   traffic (65 FP64 words per cell and step, with the
+                   corrections of substages 1 and 2 applied by the next
+                   predictor; 600 B with a correction kernel per substage);
   cpu_baseline  -- the reference algorithm restated in C (the oracle) with
                    the reference's y-slab MPI decomposition, one MPICH rank
                    per available CPU, timed on a bounded sample of the same
@@ -34,14 +36,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-B_ALG_STEP = 600.0  # bytes per cell-step, explicit-y treatments
 
 
-def bytes_per_cell_step(implicity):
-    """Compulsory HBM bytes per cell and RK3 step: 75 FP64 words, plus 27
-    for the implicit-y solve (write, read the increment, read the field, per
-    field and substage)."""
-    return 816.0 if implicity else 600.0
+def bytes_per_cell_step(implicitx, implicity, folded=True):
+    """Compulsory HBM bytes per cell and RK3 step: 75 FP64 words with a
+    correction kernel after every substage, plus 27 for the implicit-y solve
+    (write, read the increment, read the field, per field and substage).
+    With implicit x and explicit y (folded) the predictors of substages 2 and
+    3 apply the previous substage's correction as they load the fields: the
+    two corrections (7 words each) give way to a psi read and a pressure
+    write in each of those predictors, 65 words."""
+    if implicity:
+        return 816.0
+    return 520.0 if (implicitx and folded) else 600.0
 
 
 def mix_roofline(value, jobs_weights):
@@ -50,7 +57,8 @@ def mix_roofline(value, jobs_weights):
     tot = sum(w for _, w in jobs_weights)
     if tot <= 0:
         return None
-    b = sum(bytes_per_cell_step(j["implicity"]) * w for j, w in jobs_weights) / tot
+    b = sum(bytes_per_cell_step(j["implicitx"], j["implicity"]) * w
+            for j, w in jobs_weights) / tot
     ach = value * b / 1e9
     return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS, "bytes_per_cell_step": b}
@@ -60,7 +68,9 @@ def mix_roofline(value, jobs_weights):
 KERNEL_BYTES_PER_CELL = {
     # fused predictor: reads ux, uy, p, T and (substages 1, 2) the three RK
     # histories, writes T', ux*, uy* and (substages 0, 1) the histories:
-    # 10, 13, 10 words over the three substages, 11 on average
+    # 10, 13, 10 words over the three substages, 11 on average; with the
+    # folded corrections substages 1 and 2 also read psi and write p
+    # (10, 15, 12 words: pred_bytes_per_cell)
     "pred": 8 * 11,
     "poisson_fwd": 8 * 3,      # ux*, uy* -> spectrum (jtot/2+1 complex per column)
     "poisson_x": 8 * 2,        # spectrum in place
@@ -70,6 +80,14 @@ KERNEL_BYTES_PER_CELL = {
     "dt": 0,
 }
 KERNEL_NAMES = list(KERNEL_BYTES_PER_CELL.keys())
+
+
+def kernel_bytes_per_cell(kernel, folded):
+    """KERNEL_BYTES_PER_CELL, the predictor's average over its three
+    launches of a step with the folded corrections when folded."""
+    if kernel == "pred" and folded:
+        return 8 * 37 / 3
+    return KERNEL_BYTES_PER_CELL[kernel]
 
 WORKLOAD = {
     "workload": "rayleigh_benard_2d_large_single_instance",
@@ -408,6 +426,9 @@ def main():
     ntuned = sns.load_tuning() if os.path.exists(sns.TUNING_FILE) else 0
     inst = make_instance(sns, itot, jtot)
     launch = inst.launch()
+    w = WORKLOAD
+    folded = bool(w["implicitx"] and not w["implicity"] and not launch["split_corr"])
+    b_step = bytes_per_cell_step(w["implicitx"], w["implicity"], folded)
     step(sns, inst, a.warmup)
     inst.sync()
     # timed region, with the dominant-kernel events recorded live on the stream
@@ -434,7 +455,7 @@ def main():
     }
     if dominant and nlaunch:
         avg_s = kms / nlaunch * 1e-3
-        bytes_launch = KERNEL_BYTES_PER_CELL[dominant] * cells
+        bytes_launch = kernel_bytes_per_cell(dominant, folded) * cells
         ach = bytes_launch / avg_s / 1e9
         res["roofline"] = {"bound": "hbm", "kernel": dominant, "achieved": ach,
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
@@ -449,14 +470,14 @@ def main():
                                    "shapes of the tiled copy kernel and the plain one"}
     if "roofline" in res:
         res["roofline"]["frac_of_copy"] = res["roofline"]["achieved"] / ceiling["GB/s"]
-    step_gbs = value * B_ALG_STEP / 1e9
-    res["step_roofline"] = {"bytes_per_cell_step": B_ALG_STEP, "achieved": step_gbs,
+    step_gbs = value * b_step / 1e9
+    res["step_roofline"] = {"bytes_per_cell_step": b_step, "achieved": step_gbs,
                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS}
     if a.per_kernel:
         kt = per_kernel_times(sns, inst, 2)
         tot = sum(v[1] for v in kt.values())
         for k, (n, ms) in sorted(kt.items(), key=lambda kv: -kv[1][1]):
-            gbs = KERNEL_BYTES_PER_CELL[k] * cells / max(ms / n * 1e-3, 1e-12) / 1e9
+            gbs = kernel_bytes_per_cell(k, folded) * cells / max(ms / n * 1e-3, 1e-12) / 1e9
             print("%-12s launches %4d  avg %8.3f ms  share %5.1f%%  %7.0f GB/s" %
                   (k, n, ms / n, 100 * ms / tot, gbs), file=sys.stderr)
     inst.close()

@@ -67,6 +67,9 @@ typedef struct {
   double *temp, *temp_alt;
   double *srcuxa, *srcuya, *srctempa;   /* RK history of the explicit terms */
   double *ux_star, *uy_star;            /* predictor output, (jtot+2) rows */
+  double *p_alt;                        /* pressure between the substages whose
+                                           correction the next predictor applies
+                                           (layout of p) */
   long ld_ux, ld_c;                     /* row pitches in elements */
 } snsg_state;
 
@@ -224,6 +227,11 @@ typedef struct {
                            32 or 64 */
   int ysolve_recompute; /* its chunk interiors: 1 solved again in the apply
                            kernel, 2 stored and read back */
+  int split_corr;       /* 1: every substage ends with the correction kernel;
+                           0 (the rule): with implicit x and explicit y the
+                           predictors of substages 2 and 3 apply the previous
+                           substage's correction to the fields as they load
+                           them (the same operations, so the same bits) */
 } snsg_launch;
 /* SNSG_E_ARG for a value outside the ranges above, SNSG_E_SIZE for one the
  * grid does not admit (a factor that does not divide jtot or whose FFT pass

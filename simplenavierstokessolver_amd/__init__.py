@@ -37,7 +37,8 @@ class SnsgConfig(C.Structure):
 
 class SnsgState(C.Structure):
     _fields_ = [(n, _vp) for n in ["ux", "uy", "p", "psi", "temp", "temp_alt",
-                                   "srcuxa", "srcuya", "srctempa", "ux_star", "uy_star"]] + [
+                                   "srcuxa", "srcuya", "srctempa", "ux_star", "uy_star",
+                                   "p_alt"]] + [
         ("ld_ux", C.c_long), ("ld_c", C.c_long)]
 
 
@@ -56,7 +57,7 @@ class SnsgServerStats(C.Structure):
 
 class SnsgLaunch(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("fft_n1", "ysolve_chunk", "ysolve_cols",
-                                       "ysolve_recompute")]
+                                       "ysolve_recompute", "split_corr")]
 
 
 LAUNCH_FIELDS = tuple(n for n, _ in SnsgLaunch._fields_)

@@ -61,6 +61,8 @@ def candidates(sns, itot, jtot, field):
         vals = [16, 32, 64]
     elif field == "ysolve_recompute":
         vals = [1, 2]
+    elif field == "split_corr":
+        vals = [1]
     else:
         raise ValueError(field)
     out = []

@@ -53,12 +53,12 @@ struct Lane {
 };
 
 // shape class: size, treatment, temperature equation and launch shapes
-using LaneKey = std::tuple<int, int, int, int, int, int, int, int, int>;
+using LaneKey = std::tuple<int, int, int, int, int, int, int, int, int, int>;
 
 LaneKey key_of(const snsg_ctx *c) {
   const LaunchKey l = launch_key(c);
   return LaneKey(c->geo.itot, c->geo.jtot, c->treat.implicitx, c->treat.implicity,
-                 c->cfg.with_temperature != 0, l.n1, l.chunk, l.cols, l.recompute);
+                 c->cfg.with_temperature != 0, l.n1, l.chunk, l.cols, l.recompute, l.split);
 }
 
 // the class of a job's instance: its shapes come from the table as snsg_create
@@ -66,7 +66,7 @@ LaneKey key_of(const snsg_ctx *c) {
 LaneKey key_of(const snsg_config &cfg) {
   const LaunchKey l = launch_key(cfg.itot, cfg.jtot, tuned_launch(cfg.itot, cfg.jtot));
   return LaneKey(cfg.itot, cfg.jtot, cfg.implicitx != 0, cfg.implicity != 0,
-                 cfg.with_temperature != 0, l.n1, l.chunk, l.cols, l.recompute);
+                 cfg.with_temperature != 0, l.n1, l.chunk, l.cols, l.recompute, l.split);
 }
 
 int lane_init(Lane &k, const snsg_ctx *c) {
@@ -199,7 +199,7 @@ void lane_launch(const Lane &k, int p) {
       }
     for (int stage = 0; stage < 3; stage++)
       launch_poisson_batch(s, sh.pois[rk], k.pois[rk], nz, stage);
-    launch_corr_batch(s, sh.corr[rk], k.corr[rk], nz);
+    if (sh.corr_on[rk]) launch_corr_batch(s, sh.corr[rk], k.corr[rk], nz);
   }
 }
 

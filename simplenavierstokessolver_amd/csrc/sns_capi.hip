@@ -410,6 +410,7 @@ LaunchKey launch_key(int itot, int jtot, const snsg_launch &req) {
   const int SR = part ? jtot / k.chunk - 1 : 0;
   k.cols = part && SR <= 256 ? (req.ysolve_cols ? req.ysolve_cols : ysolve_cols_rule(SR)) : 0;
   k.recompute = part ? req.ysolve_recompute : 0;
+  k.split = req.split_corr;
   return k;
 }
 
@@ -424,7 +425,8 @@ extern "C" int snsg_launch_check(int itot, int jtot, const snsg_launch *l) {
   if (itot < 4 || jtot < 4) return set_err(SNSG_E_SIZE, "snsg_launch_check: itot, jtot < 4");
   const int w = l->ysolve_cols;
   if (l->fft_n1 < 0 || l->ysolve_chunk < 0 || (w && w != 16 && w != 32 && w != 64) ||
-      l->ysolve_recompute < 0 || l->ysolve_recompute > 2)
+      l->ysolve_recompute < 0 || l->ysolve_recompute > 2 || l->split_corr < 0 ||
+      l->split_corr > 1)
     return set_err(SNSG_E_ARG, "snsg_launch_check: value out of range");
   Fft2Plan pl;
   if (l->fft_n1 && !fft2_plan(jtot, itot, pl, l->fft_n1))
@@ -459,6 +461,7 @@ extern "C" int snsg_get_launch(const snsg_ctx *c, snsg_launch *l) {
   l->ysolve_chunk = k.chunk;
   l->ysolve_cols = k.cols;
   l->ysolve_recompute = k.recompute;
+  l->split_corr = k.split;
   return 0;
 }
 
@@ -761,6 +764,7 @@ extern "C" int snsg_state_alloc(snsg_ctx *c, snsg_state *st) {
   if ((rc = alloc_c(c, &st->temp_alt))) return rc;
   if ((rc = ctx_alloc((void **)&st->ux_star, bytes_ux(c), &c->mem))) return rc;
   if ((rc = alloc_c(c, &st->uy_star))) return rc;
+  if ((rc = alloc_c(c, &st->p_alt))) return rc;
   if ((rc = ctx_alloc((void **)&st->srcuxa, sizeof(double) * (size_t)(itot - 1) * jtot, &c->mem))) return rc;
   if ((rc = ctx_alloc((void **)&st->srcuya, sizeof(double) * (size_t)itot * jtot, &c->mem))) return rc;
   if ((rc = ctx_alloc((void **)&st->srctempa, sizeof(double) * (size_t)itot * jtot, &c->mem))) return rc;
@@ -774,7 +778,7 @@ extern "C" int snsg_state_free(snsg_ctx *c, snsg_state *st) {
                                "first: its graphs hold the state's fields)");
   HIPCHECK(hipStreamSynchronize(c->stream));
   double *plain[] = {st->ux, st->ux_star, st->srcuxa, st->srcuya, st->srctempa};
-  double *cell[] = {st->uy, st->p, st->psi, st->temp, st->temp_alt, st->uy_star};
+  double *cell[] = {st->uy, st->p, st->psi, st->temp, st->temp_alt, st->uy_star, st->p_alt};
   for (double *p : plain)
     if (p) hipFree(p);
   for (double *p : cell)
@@ -782,7 +786,7 @@ extern "C" int snsg_state_free(snsg_ctx *c, snsg_state *st) {
   if (st->ux) {
     const size_t sx = sizeof(double) * (size_t)(c->geo.itot - 1) * c->geo.jtot;
     const size_t sy = sizeof(double) * (size_t)c->geo.itot * c->geo.jtot;
-    c->mem -= 2 * bytes_ux(c) + 6 * bytes_c(c) + sx + 2 * sy;
+    c->mem -= 2 * bytes_ux(c) + 7 * bytes_c(c) + sx + 2 * sy;
   }
   memset(st, 0, sizeof *st);
   return 0;
@@ -799,7 +803,8 @@ int state_reset(snsg_ctx *c, snsg_state *st) {
   } a[] = {{st->ux, bu},           {st->ux_star, bu},       {st->srcuxa, sx},
            {st->srcuya, sy},       {st->srctempa, sy},      {st->uy - kStateCOff, bc},
            {st->p - kStateCOff, bc}, {st->psi - kStateCOff, bc}, {st->temp - kStateCOff, bc},
-           {st->temp_alt - kStateCOff, bc}, {st->uy_star - kStateCOff, bc}};
+           {st->temp_alt - kStateCOff, bc}, {st->uy_star - kStateCOff, bc},
+           {st->p_alt - kStateCOff, bc}};
   for (auto &x : a) HIPCHECK(hipMemsetAsync(x.p, 0, x.bytes, c->stream));
   return 0;
 }
@@ -843,7 +848,7 @@ extern "C" int snsg_instance_bytes(const snsg_config *cfg, size_t *bytes) {
   n += d * kTables * stride + z * kTables2 * stride;
   n += d * poisson_factor_doubles((int)itot, (int)jtot) + d * 2 * (size_t)poisson_factor_entries((int)itot);
   // snsg_state_alloc
-  n += 2 * d * pu * (jtot + 2) + 6 * d * (pc * (jtot + 2) + 16);
+  n += 2 * d * pu * (jtot + 2) + 7 * d * (pc * (jtot + 2) + 16);
   n += d * (itot - 1) * jtot + 2 * d * itot * jtot;
   // first fused step
   if (cfg->implicity) {
@@ -1092,14 +1097,27 @@ void step_args(const snsg_ctx *c, const snsg_state *st, bool reduce, StepArgs &s
   sa.yf.S = yL ? g.jtot / yL : 0;
   sa.yf.stride = (long)ystride;
   double *temp = st->temp, *temp_alt = st->temp_alt;
+  // Folded corrections (implicit x, explicit y): substage 1 predicts from
+  // (ux, uy, p) into (ux*, uy*) and stops at psi; substage 2's predictor
+  // corrects ux*, uy* and p with that psi as it loads them, predicts into
+  // (ux, uy) and stores the corrected p in p_alt; substage 3 does the same
+  // from (ux, uy, p_alt) into (ux*, uy*) and p, and its correction kernel
+  // writes ux, uy, p as in the split sequence.
+  const bool fold = impx && !impy && !c->launch.split_corr;
+  double *in_u = st->ux, *in_v = st->uy, *out_u = st->ux_star, *out_v = st->uy_star;
+  double *in_p = st->p;
   for (int rk = 0; rk < 3; rk++) {
+    const bool folded = fold && rk > 0;  // applies the correction of substage rk-1
     PredArgs &pa = sa.pred[rk];
     pa.g = g;
     pa.tb = c->tab;
     pa.tr = c->treat;
-    pa.ux = st->ux; pa.uy = st->uy; pa.p = st->p; pa.temp = temp;
+    pa.ux = in_u; pa.uy = in_v; pa.p = in_p; pa.temp = temp;
     pa.hist_ux = st->srcuxa; pa.hist_uy = st->srcuya; pa.hist_t = st->srctempa;
-    pa.out_u = st->ux_star; pa.out_v = st->uy_star; pa.out_t = temp_alt;
+    pa.out_u = out_u; pa.out_v = out_v; pa.out_t = temp_alt;
+    pa.psi = folded ? st->psi : nullptr;
+    pa.p_out = folded ? (rk == 1 ? st->p_alt : st->p) : nullptr;
+    pa.corr_gamma = folded ? c->cfg.gamma[rk - 1] : 0.;
     pa.q_u = impy ? c->d_qf[1] : nullptr;
     pa.q_v = impy ? c->d_qf[2] : nullptr;
     pa.q_t = impy ? c->d_qf[0] : nullptr;
@@ -1112,8 +1130,8 @@ void step_args(const snsg_ctx *c, const snsg_state *st, bool reduce, StepArgs &s
     pa.lu = c->d_lu ? c->d_lu + (size_t)(rk * 3 * 3) * c->lu_ntm : nullptr;
     pa.lu_hdr = c->d_lu ? c->d_lu + 27 * (size_t)c->lu_ntm + rk * 3 * kLuHdr : nullptr;
     pa.ntm = c->lu_ntm;
-    const double *ins[3] = {temp, st->ux, st->uy};
-    double *outs[3] = {temp_alt, st->ux_star, st->uy_star};
+    const double *ins[3] = {temp, in_u, in_v};
+    double *outs[3] = {temp_alt, out_u, out_v};
     for (int f = 0; f < 3; f++) {
       YArgs &y = sa.y[rk][f];
       y.g = g;
@@ -1131,17 +1149,23 @@ void step_args(const snsg_ctx *c, const snsg_state *st, bool reduce, StepArgs &s
       y.cols = c->launch.ysolve_cols;
       y.recompute = c->launch.ysolve_recompute;
     }
-    sa.pois[rk] = pois_args(c, g, c->cfg.gamma[rk], st->ux_star, st->uy_star, st->psi);
+    sa.pois[rk] = pois_args(c, g, c->cfg.gamma[rk], out_u, out_v, st->psi);
+    sa.corr_on[rk] = !fold || rk == 2;
     CorrArgs &ca = sa.corr[rk];
     ca.g = g;
     ca.tb = c->tab;
     ca.tr = c->treat;
-    ca.ux_star = st->ux_star; ca.uy_star = st->uy_star; ca.psi = st->psi;
+    ca.ux_star = out_u; ca.uy_star = out_v; ca.psi = st->psi;
     ca.ux = st->ux; ca.uy = st->uy; ca.p = st->p;
     ca.dtp = c->d_dt;
     ca.gamma = c->cfg.gamma[rk];
     ca.red = (reduce && rk == 2) ? c->d_part : nullptr;
     if (with_temp) std::swap(temp, temp_alt);
+    if (fold) {
+      std::swap(in_u, out_u);
+      std::swap(in_v, out_v);
+      if (pa.p_out) in_p = pa.p_out;
+    }
   }
 }
 
@@ -1174,7 +1198,8 @@ static int integrate_fused(snsg_ctx *c, snsg_state *st, bool reduce) {
     if ((rc = run(c, "poisson_fwd", [&] { launch_poisson(c->stream, pa, 0); }))) return rc;
     if ((rc = run(c, "poisson_x", [&] { launch_poisson(c->stream, pa, 1); }))) return rc;
     if ((rc = run(c, "poisson_inv", [&] { launch_poisson(c->stream, pa, 2); }))) return rc;
-    if ((rc = run(c, "corr", [&] { launch_corr(c->stream, sa.corr[rk]); }))) return rc;
+    if (sa.corr_on[rk] && (rc = run(c, "corr", [&] { launch_corr(c->stream, sa.corr[rk]); })))
+      return rc;
     if (with_temp) std::swap(st->temp, st->temp_alt);
   }
   return 0;

@@ -94,6 +94,9 @@ struct StepArgs {
   YArgs y[3][3];   // [substage][T, ux, uy]
   PoisArgs pois[3];
   CorrArgs corr[3];
+  // substage ends with the correction kernel (otherwise the next substage's
+  // predictor applies it as it loads the fields)
+  bool corr_on[3];
 };
 
 // field kinds of the y solves, in the order T, ux, uy
@@ -111,7 +114,7 @@ int step_prepare(snsg_ctx *c);
 // recompute choice (0: decided at launch).
 snsg_launch tuned_launch(int itot, int jtot);
 struct LaunchKey {
-  int n1, chunk, cols, recompute;
+  int n1, chunk, cols, recompute, split;
 };
 LaunchKey launch_key(int itot, int jtot, const snsg_launch &req);
 LaunchKey launch_key(const snsg_ctx *c);

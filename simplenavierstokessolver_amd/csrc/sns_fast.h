@@ -371,6 +371,54 @@ __device__ __forceinline__ void put_c(const Geo &g, double *f, int i, int j, dou
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// ------------------------------------------------- projection of one point
+
+// src/fluid/correct_velocity.c:6-42 and src/fluid/update_pressure.c:7-60
+// at one point.  k_corr and the predictors that apply the previous
+// substage's correction to the fields they load both use these, so both
+// produce the same bits.  gdt is gamma*dt of the substage that solved for psi.
+
+// ux(i,j) = ux*(i,j) - gdt (psi(i,j) - psi(i-1,j)) / DXC(i), the quotient
+// as a product with the tabulated 1./DXC(i)
+__device__ __forceinline__ double proj_ux(double us, double psi_c, double psi_w, double gdt,
+                                          double rdxc) {
+  return us - gdt * (psi_c - psi_w) * rdxc;
+}
+
+// uy(i,j) = uy*(i,j) - gdt (psi(i,j) - psi(i,j-1)) / dy.  P2 (dy a power
+// of two): the product with 1/dy, the same bits as div_cr (see dv).
+template <bool P2 = false>
+__device__ __forceinline__ double proj_uy(double vs, double psi_c, double psi_s, double gdt,
+                                          double dy, double rdy) {
+  return vs - dv<P2>(gdt * (psi_c - psi_s), dy, rdy);
+}
+
+// Metrics of cell i: DXC(i), DXC(i+1), DXF(i) and their reciprocals
+struct ProjX {
+  double dxc_i, rdxc_i, dxc_p, rdxc_p, dxf_i, rdxf_i;
+};
+
+// p(i,j) + psi(i,j) + fac * (the implicitly treated parts of the Laplacian
+// of psi), fac = -gdt / (2 Re); psi_w, psi_e, psi_s, psi_n are the values at
+// (i-1,j), (i+1,j), (i,j-1), (i,j+1).  P2 as for proj_uy.
+template <bool P2 = false>
+__device__ __forceinline__ double proj_p(double pj, double psi_c, double psi_w, double psi_e,
+                                         double psi_s, double psi_n, const ProjX &x, double dy,
+                                         double rdy, double fac, int implicitx, int implicity) {
+  double pv = pj + psi_c;
+  if (implicitx) {
+    const double dm = dv<P2>(-psi_w + psi_c, x.dxc_i, x.rdxc_i);
+    const double dp = dv<P2>(-psi_c + psi_e, x.dxc_p, x.rdxc_p);
+    pv += fac * (+dv<P2>(dp - dm, x.dxf_i, x.rdxf_i));
+  }
+  if (implicity) {
+    const double dm = dv<P2>(-psi_s + psi_c, dy, rdy);
+    const double dp = dv<P2>(-psi_c + psi_n, dy, rdy);
+    pv += fac * (+dv<P2>(dp - dm, dy, rdy));
+  }
+  return pv;
+}
+
 // Block index -> work index so that consecutive work items run on the same
 // XCD (blocks are dispatched round-robin over the 8 XCDs, each with its own
 // L2); neighbouring rows / strips then share that L2.

@@ -21,6 +21,13 @@ struct PredArgs {
   const double *lu;                       // implicit x: LU tables of the substage (3 fields)
   const double *lu_hdr;                   // their headers (kLuHdr doubles per field)
   int ntm;                                // entries per LU table
+  // Folded correction (implicit x, explicit y): when psi is set, ux, uy and
+  // p are the previous substage's ux*, uy* and pressure, corrected with psi
+  // and corr_gamma*dt as they are loaded (k_corr's operations); the
+  // uy workgroups store the corrected pressure of their rows in p_out.
+  const double *psi;
+  double *p_out;
+  double corr_gamma;
 };
 
 struct CorrArgs {

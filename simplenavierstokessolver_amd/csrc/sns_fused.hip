@@ -112,6 +112,8 @@ static void pred_exp_launch(hipStream_t s, const PredArgs &shape, S src, int nz)
     hipLaunchKernelGGL((k_pred_exp<false, S>), grid, dim3(256), 0, s, src);
 }
 
+// A folded correction (psi set) is only requested with implicit x: the
+// explicit-x kernel takes its fields as they are.
 void launch_pred(hipStream_t s, const PredArgs &a, int implicitx) {
   if (implicitx) launch_pred_rows(s, a);
   else pred_exp_launch(s, a, ArgOne<PredArgs>{a}, 1);
@@ -190,26 +192,18 @@ __global__ void __launch_bounds__(kCorrThreads) k_corr(S src) {
     const double pj = ldr(a.p + (size_t)j * wc, ci);
     if (out) {
       if (i >= 2) {
-        const double v = us - gdt * (psi_c - psi_w) * rdxc_i;  // (1./DXC(i)) as a product
+        const double v = proj_ux(us, psi_c, psi_w, gdt, rdxc_i);
         put_ux(g, a.ux, i, j, v);
         if (a.red) m = fmin(m, dxc_i / fmax(fabs(v), 1.e-8));
       }
       {
-        const double v = vs - div_cr(gdt * (psi_c - psi_s), tb.dy, tb.rdy);
+        const double v = proj_uy(vs, psi_c, psi_s, gdt, tb.dy, tb.rdy);
         put_c<0>(g, a.uy, i, j, v);
         if (a.red) m = fmin(m, tb.dy / fmax(fabs(v), 1.e-8));
       }
-      double pv = pj + psi_c;
-      if (a.tr.implicitx) {
-        const double dm = div_cr(-psi_w + psi_c, dxc_i, rdxc_i);
-        const double dp = div_cr(-psi_c + psi_e, dxc_p, rdxc_p);
-        pv += fac * (+div_cr(dp - dm, dxf_i, rdxf_i));
-      }
-      if (a.tr.implicity) {
-        const double dm = div_cr(-psi_s + psi_c, tb.dy, tb.rdy);
-        const double dp = div_cr(-psi_c + psi_n, tb.dy, tb.rdy);
-        pv += fac * (+div_cr(dp - dm, tb.dy, tb.rdy));
-      }
+      const ProjX x = {dxc_i, rdxc_i, dxc_p, rdxc_p, dxf_i, rdxf_i};
+      const double pv = proj_p(pj, psi_c, psi_w, psi_e, psi_s, psi_n, x, tb.dy, tb.rdy, fac,
+                               a.tr.implicitx, a.tr.implicity);
       put_c<1>(g, a.p, i, j, pv);
     }
     psi_s = psi_c;

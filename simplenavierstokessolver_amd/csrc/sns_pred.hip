@@ -311,6 +311,104 @@ __device__ __forceinline__ Raw<F> raw_load(const PredArgs &a, const LanePos &p, 
   return r;
 }
 
+// The previous substage's correction applied to the values raw_load took
+// from its ux*, uy* and pressure, with the operations of k_corr (proj_* of
+// sns_fast.h; P2 gives the same bits as its divisions): psi of the rows the
+// corrected values need is loaded here, its x neighbours come from the
+// adjacent lanes.  Lane 0 of a ux workgroup loads psi(i-1) itself, since
+// the face and the pressure it corrects feed lane 1; in the other
+// workgroups lane 0 only feeds uy and T, which need no x neighbour of psi.
+//
+// Faces 2..itot and cells 1..itot are corrected, the rest keep what the
+// predictor stored, which is what k_corr leaves there:
+//  * faces: psi's wall columns are exact copies of its columns 1 and itot,
+//    so psi(i) - psi(i-1) is +0 at the wall faces 1 and itot+1 (and at the
+//    clamped lanes past them) and the correction leaves those faces as they
+//    are, with no select;
+//  * uy: the wall columns 0 and itot+1 are kept by a select in the uy
+//    workgroup, the only one whose outputs read them;
+//  * p: values outside cells 1..itot feed no output.
+// Row ja-2 of a uy workgroup wraps to jtot-1 for ja = 1 (periodic in y), and
+// its pressure of row 0 is taken from row jtot: ux*, uy* and psi carry
+// periodic copies in their halo rows, the pressure only once k_corr has
+// written it.  The uy workgroup stores the corrected pressure of its rows in
+// p_out, the stores of k_corr for those cells.
+template <int F, bool P2>
+__device__ __forceinline__ void raw_fold(const PredArgs &a, const LanePos &p, int ja, int jb,
+                                         bool write_a, double dt, Raw<F> &r) {
+  const Geo &g = a.g;
+  const Tables &tb = a.tb;
+  const int itot = g.itot;
+  const size_t wc = (size_t)g.pc;
+  const double *S = a.psi;
+  const double gdt = a.corr_gamma * dt;
+  const double fac = -gdt / g.two_Re;
+  const double rdxc = r.c2.y;  // 1./DXC(i)
+  const double dy = tb.dy, rdy = tb.rdy;
+  const int imx = a.tr.implicitx, imy = a.tr.implicity;
+  const double s_am = ldr(S + (size_t)(ja - 1) * wc, p.ci);
+  const double s_a = ldr(S + (size_t)ja * wc, p.ci);
+  const double s_b = ldr(S + (size_t)jb * wc, p.ci);
+  const double s_bp = ldr(S + (size_t)(jb + 1) * wc, p.ci);
+  // Lane shifts run on every lane, never under a divergent condition (a
+  // shift there would read the inactive neighbour lanes as 0).
+  if constexpr (F == 0) {
+    r.u_a = proj_ux(r.u_a, s_a, from_left(s_a), gdt, rdxc);
+    r.u_b = proj_ux(r.u_b, s_b, from_left(s_b), gdt, rdxc);
+    r.v_a = proj_uy<P2>(r.v_a, s_a, s_am, gdt, dy, rdy);
+    r.v_b = proj_uy<P2>(r.v_b, s_b, s_a, gdt, dy, rdy);
+    r.v_bp = proj_uy<P2>(r.v_bp, s_bp, s_b, gdt, dy, rdy);
+  } else {
+    const ProjX x = {r.c2.x, r.c2.y, from_right(r.c2.x), from_right(r.c2.y), r.f2.x, r.f2.y};
+    if constexpr (F == 1) {
+      const int cw = clampi(p.i - 1, 0, itot + 1);
+      double w_am = from_left(s_am), w_a = from_left(s_a), w_b = from_left(s_b),
+             w_bp = from_left(s_bp);
+      if ((threadIdx.x & 63) == 0) {
+        w_am = ldr(S + (size_t)(ja - 1) * wc, cw);
+        w_a = ldr(S + (size_t)ja * wc, cw);
+        w_b = ldr(S + (size_t)jb * wc, cw);
+        w_bp = ldr(S + (size_t)(jb + 1) * wc, cw);
+      }
+      const double e_a = from_right(s_a), e_b = from_right(s_b);
+      r.u_am = proj_ux(r.u_am, s_am, w_am, gdt, rdxc);
+      r.u_a = proj_ux(r.u_a, s_a, w_a, gdt, rdxc);
+      r.u_b = proj_ux(r.u_b, s_b, w_b, gdt, rdxc);
+      r.u_bp = proj_ux(r.u_bp, s_bp, w_bp, gdt, rdxc);
+      r.v_a = proj_uy<P2>(r.v_a, s_a, s_am, gdt, dy, rdy);
+      r.v_b = proj_uy<P2>(r.v_b, s_b, s_a, gdt, dy, rdy);
+      r.v_bp = proj_uy<P2>(r.v_bp, s_bp, s_b, gdt, dy, rdy);
+      r.p_a = proj_p<P2>(r.p_a, s_a, w_a, e_a, s_am, s_b, x, dy, rdy, fac, imx, imy);
+      r.p_b = proj_p<P2>(r.p_b, s_b, w_b, e_b, s_a, s_bp, x, dy, rdy, fac, imx, imy);
+    } else {
+      const bool cc = p.i >= 1 && p.i <= itot;  // uy cells the correction moves
+      const int jmm = ja >= 2 ? ja - 2 : g.jtot - 1;
+      const double s_amm = ldr(S + (size_t)jmm * wc, p.ci);
+      if (ja == 1) r.p_am = ldr(a.p + (size_t)g.jtot * wc, p.ci);
+      const double w_am = from_left(s_am), w_a = from_left(s_a), w_b = from_left(s_b);
+      const double e_am = from_right(s_am), e_a = from_right(s_a), e_b = from_right(s_b);
+      const double v_am = proj_uy<P2>(r.v_am, s_am, s_amm, gdt, dy, rdy);
+      const double v_a = proj_uy<P2>(r.v_a, s_a, s_am, gdt, dy, rdy);
+      const double v_b = proj_uy<P2>(r.v_b, s_b, s_a, gdt, dy, rdy);
+      const double v_bp = proj_uy<P2>(r.v_bp, s_bp, s_b, gdt, dy, rdy);
+      r.v_am = cc ? v_am : r.v_am;
+      r.v_a = cc ? v_a : r.v_a;
+      r.v_b = cc ? v_b : r.v_b;
+      r.v_bp = cc ? v_bp : r.v_bp;
+      r.u_am = proj_ux(r.u_am, s_am, w_am, gdt, rdxc);
+      r.u_a = proj_ux(r.u_a, s_a, w_a, gdt, rdxc);
+      r.u_b = proj_ux(r.u_b, s_b, w_b, gdt, rdxc);
+      r.p_am = proj_p<P2>(r.p_am, s_am, w_am, e_am, s_amm, s_a, x, dy, rdy, fac, imx, imy);
+      r.p_a = proj_p<P2>(r.p_a, s_a, w_a, e_a, s_am, s_b, x, dy, rdy, fac, imx, imy);
+      r.p_b = proj_p<P2>(r.p_b, s_b, w_b, e_b, s_a, s_bp, x, dy, rdy, fac, imx, imy);
+      if (p.out) {
+        if (write_a) put_c<1>(g, a.p_out, p.i, ja, r.p_a);
+        put_c<1>(g, a.p_out, p.i, jb, r.p_b);
+      }
+    }
+  }
+}
+
 // Source terms and RK increments of both rows from the loaded values (x
 // neighbours from the adjacent lanes); the new RK history is stored.
 template <int F, bool P2>
@@ -413,14 +511,15 @@ __device__ __forceinline__ double2 raw_own(const Raw<F> &r) {
 // (compile-time block index).  The workgroup's own field at rows ja, jb is
 // kept per block in own[] for the final values, so those rows are not read
 // a second time after the line solve.
-template <int K, int F, int NT, int M, bool P2>
+template <int K, int F, int NT, int M, bool P2, bool FOLD>
 __device__ __forceinline__ void pred_stencils(const PredArgs &a, PredRowsLds<NT, M> &L, int ja,
                                               int jb, bool write_a, int n, int i0, double dt,
                                               int nk, double2 (&own)[stencil_blocks<NT, M>()]) {
   if constexpr (K < stencil_blocks<NT, M>()) {
     if (K >= nk) return;
     const LanePos p = lane_pos(K, NT / 64, i0, n, a.g.itot);
-    const Raw<F> r = raw_load<F>(a, p, ja, jb, n);
+    Raw<F> r = raw_load<F>(a, p, ja, jb, n);
+    if constexpr (FOLD) raw_fold<F, P2>(a, p, ja, jb, write_a, dt, r);
     double da, db;
     raw_eval<F, P2>(a, r, p, ja, jb, n, write_a, dt, da, db);
     own[K] = raw_own<F>(r);
@@ -428,7 +527,7 @@ __device__ __forceinline__ void pred_stencils(const PredArgs &a, PredRowsLds<NT,
       L.d[0][pad_idx<M>(p.ii)] = da;
       L.d[1][pad_idx<M>(p.ii)] = db;
     }
-    pred_stencils<K + 1, F, NT, M, P2>(a, L, ja, jb, write_a, n, i0, dt, nk, own);
+    pred_stencils<K + 1, F, NT, M, P2, FOLD>(a, L, ja, jb, write_a, n, i0, dt, nk, own);
   }
 }
 
@@ -462,7 +561,7 @@ __device__ __forceinline__ void pred_emit(const PredArgs &a, const PredRowsLds<N
 
 // One workgroup's field F for rows ja and jb (write_a: row ja is written):
 // stencils into LDS, the CN-x solve of both rows, final values.
-template <int F, int NT, int M, bool P2>
+template <int F, int NT, int M, bool P2, bool FOLD>
 __device__ __forceinline__ void pred_rows_body(const PredArgs &a, PredRowsLds<NT, M> &L, int ja,
                                                int jb, bool write_a) {
   constexpr int NW = NT / 64;
@@ -476,7 +575,7 @@ __device__ __forceinline__ void pred_rows_body(const PredArgs &a, PredRowsLds<NT
 
   // ---- stencils and RK increments of both rows into LDS (lane order)
   double2 own[KB];
-  pred_stencils<0, F, NT, M, P2>(a, L, ja, jb, write_a, n, i0, dt, nk, own);
+  pred_stencils<0, F, NT, M, P2, FOLD>(a, L, ja, jb, write_a, n, i0, dt, nk, own);
   __syncthreads();
 
   // ---- Crank-Nicolson in x of both rows (chunk order)
@@ -509,7 +608,8 @@ __device__ __forceinline__ void pred_rows_body(const PredArgs &a, PredRowsLds<NT
 // Workgroup w: field w % nfld (+1 without the temperature equation) of rows
 // a = 2q+1 and b = a+1, q = w / nfld.  For odd jtot the last pair is shifted
 // down to (jtot-1, jtot) and writes row b only.  Consecutive w share an XCD.
-template <int NT, int M, bool P2, class S>
+// FOLD: the fields are corrected as they are loaded (raw_fold).
+template <int NT, int M, bool P2, bool FOLD, class S>
 __global__ void __launch_bounds__(NT) k_pred_rows(S src, int nfld) {
   const PredArgs a = src.get();
   extern __shared__ double lds_raw[];
@@ -519,26 +619,26 @@ __global__ void __launch_bounds__(NT) k_pred_rows(S src, int nfld) {
   const int f = w % nfld + (nfld == 3 ? 0 : 1);
   const int ja = min(2 * q + 1, a.g.jtot - 1), jb = ja + 1;
   const bool write_a = ja == 2 * q + 1;
-  if (f == 0) pred_rows_body<0, NT, M, P2>(a, L, ja, jb, write_a);
-  else if (f == 1) pred_rows_body<1, NT, M, P2>(a, L, ja, jb, write_a);
-  else pred_rows_body<2, NT, M, P2>(a, L, ja, jb, write_a);
+  if (f == 0) pred_rows_body<0, NT, M, P2, FOLD>(a, L, ja, jb, write_a);
+  else if (f == 1) pred_rows_body<1, NT, M, P2, FOLD>(a, L, ja, jb, write_a);
+  else pred_rows_body<2, NT, M, P2, FOLD>(a, L, ja, jb, write_a);
 }
 
 // ---------------------------------------------------------------- launch
 
-template <int NT, int M, bool P2, class S>
+template <int NT, int M, bool P2, bool FOLD, class S>
 static void pred_rows_launch(hipStream_t s, const PredArgs &shape, S src, int nz) {
   const size_t lds = sizeof(PredRowsLds<NT, M>);
   static bool attr = false;
   if (!attr) {
-    hipFuncSetAttribute((const void *)k_pred_rows<NT, M, P2, S>,
+    hipFuncSetAttribute((const void *)k_pred_rows<NT, M, P2, FOLD, S>,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr = true;
   }
   const int nfld = shape.with_temp ? 3 : 2;
   const int npair = (shape.g.jtot + 1) / 2;
-  hipLaunchKernelGGL((k_pred_rows<NT, M, P2, S>), dim3(npair * nfld, 1, nz), dim3(NT), lds, s, src,
-                     nfld);
+  hipLaunchKernelGGL((k_pred_rows<NT, M, P2, FOLD, S>), dim3(npair * nfld, 1, nz), dim3(NT), lds,
+                     s, src, nfld);
 }
 
 template <int NT, int M, class S>
@@ -591,24 +691,28 @@ void launch_lu_factor_batch(hipStream_t s, const LuFactorArgs &shape, const LuFa
   lu_factor_dispatch(s, shape.g.itot, arg_batch<LuFactorArgs>(dev), nz);
 }
 
-template <bool P2, class S>
+template <bool P2, bool FOLD, class S>
 static void pred_rows_select(hipStream_t s, const PredArgs &shape, S src, int nz, int NT) {
   switch (NT) {
-    case 64: pred_rows_launch<64, kRowM, P2>(s, shape, src, nz); break;
-    case 128: pred_rows_launch<128, kRowM, P2>(s, shape, src, nz); break;
-    case 256: pred_rows_launch<256, kRowM, P2>(s, shape, src, nz); break;
-    case 512: pred_rows_launch<512, kRowM, P2>(s, shape, src, nz); break;
-    default: pred_rows_launch<1024, kRowM, P2>(s, shape, src, nz); break;
+    case 64: pred_rows_launch<64, kRowM, P2, FOLD>(s, shape, src, nz); break;
+    case 128: pred_rows_launch<128, kRowM, P2, FOLD>(s, shape, src, nz); break;
+    case 256: pred_rows_launch<256, kRowM, P2, FOLD>(s, shape, src, nz); break;
+    case 512: pred_rows_launch<512, kRowM, P2, FOLD>(s, shape, src, nz); break;
+    default: pred_rows_launch<1024, kRowM, P2, FOLD>(s, shape, src, nz); break;
   }
 }
 
-// Power-of-two divisors select the kernel that divides by multiplying.
+// Power-of-two divisors select the kernel that divides by multiplying; a
+// psi argument the one that applies the previous substage's correction.
 template <class S>
 static void pred_rows_dispatch(hipStream_t s, const PredArgs &shape, S src, int nz) {
   int NT, M;
   row_config(shape.g.itot, NT, M);
-  if (pow2_division(shape.tb)) pred_rows_select<true>(s, shape, src, nz, NT);
-  else pred_rows_select<false>(s, shape, src, nz, NT);
+  const bool p2 = pow2_division(shape.tb), fold = shape.psi != nullptr;
+  if (p2 && fold) pred_rows_select<true, true>(s, shape, src, nz, NT);
+  else if (p2) pred_rows_select<true, false>(s, shape, src, nz, NT);
+  else if (fold) pred_rows_select<false, true>(s, shape, src, nz, NT);
+  else pred_rows_select<false, false>(s, shape, src, nz, NT);
 }
 
 void launch_pred_rows(hipStream_t s, const PredArgs &a) {

@@ -90,7 +90,7 @@ def counter_avgs(paths, counter):
     return {k: (len(v), sum(v) / len(v)) for k, v in acc.items()}
 
 
-def traffic(fetch_paths, write_paths, copy_bytes, cells):
+def traffic(fetch_paths, write_paths, copy_bytes, cells, folded=True):
     fe = counter_avgs(fetch_paths, "FETCH_SIZE")
     wr = counter_avgs(write_paths, "WRITE_SIZE")
     if "k_copy" not in fe or "k_copy" not in wr:
@@ -98,7 +98,9 @@ def traffic(fetch_paths, write_paths, copy_bytes, cells):
     f_fac = copy_bytes / (fe["k_copy"][1] * 1024.0)
     w_fac = copy_bytes / (wr["k_copy"][1] * 1024.0)
     sys.path.insert(0, __file__.rsplit("/", 2)[0])
-    from bench import KERNEL_BYTES_PER_CELL  # algorithmic bytes per cell and launch
+    # algorithmic bytes per cell and launch (folded: the corrections of
+    # substages 1 and 2 applied by the next predictor)
+    from bench import KERNEL_BYTES_PER_CELL, kernel_bytes_per_cell
     rows = []
     for k in sorted(set(fe) | set(wr)):
         if k == "k_copy":
@@ -106,7 +108,7 @@ def traffic(fetch_paths, write_paths, copy_bytes, cells):
         g = group_of("sns::" + k + "(")
         fb = fe.get(k, (0, 0.0))[1] * 1024.0 * f_fac
         wb = wr.get(k, (0, 0.0))[1] * 1024.0 * w_fac
-        alg = KERNEL_BYTES_PER_CELL.get(g)
+        alg = kernel_bytes_per_cell(g, folded) if g in KERNEL_BYTES_PER_CELL else None
         rows.append({"kernel": k, "group": g, "dispatches": fe.get(k, (0, 0))[0],
                      "fetch_bytes": fb, "write_bytes": wb, "traffic_bytes": fb + wb,
                      "group_algorithmic_bytes": None if alg is None else alg * cells})
@@ -194,6 +196,8 @@ def main(argv=None):
     s2.add_argument("--write", nargs="+", required=True)
     s2.add_argument("--copy-bytes", type=int, required=True)
     s2.add_argument("--cells", type=int, required=True)
+    s2.add_argument("--split-corr", action="store_true",
+                    help="the step ran a correction kernel after every substage")
     s2.add_argument("--out")
     s2.add_argument("--json")
     s2.add_argument("--command", action="append", default=[])
@@ -258,7 +262,7 @@ def main(argv=None):
             with open(a.json, "w") as f:
                 json.dump(t, f, indent=1)
     else:
-        t = traffic(a.fetch, a.write, a.copy_bytes, a.cells)
+        t = traffic(a.fetch, a.write, a.copy_bytes, a.cells, folded=not a.split_corr)
         lines += ["Calibration: k_copy of %d bytes read %.0f KiB (FETCH_SIZE) and wrote %.0f KiB "
                   "(WRITE_SIZE) per dispatch; factors %.4f (fetch), %.4f (write)."
                   % (t["copy_bytes"], t["copy_fetch_kib"], t["copy_write_kib"],

@@ -144,7 +144,8 @@ def test_chunk_change_between_steps(sns, pinned_gauge):
 def test_set_launch_rejects_bad_shapes(sns):
     inst = sns.Instance(64, 256)
     for bad in (dict(fft_n1=3), dict(fft_n1=256), dict(ysolve_chunk=33), dict(ysolve_chunk=6),
-                dict(ysolve_cols=8), dict(ysolve_recompute=3)):
+                dict(ysolve_cols=8), dict(ysolve_recompute=3), dict(split_corr=2),
+                dict(split_corr=-1)):
         with pytest.raises(sns.GpuError):
             inst.set_launch(**bad)
     inst.close()
@@ -231,3 +232,43 @@ def test_table_served_matches_alone(table):
         inst.close()
         srv.release(jid)
     srv.close()
+
+
+# (itot, jtot, stretch, with_temperature, steps): implicit x, explicit y
+FOLD_CASES = [
+    (64, 256, 3.0, True, 6),       # stretched grid, 64-thread rows
+    (128, 64, 1e6, True, 6),       # uniform grid, power-of-two divisors
+    (96, 90, 1e6, True, 6),        # uniform, divisors not powers of two
+    (40, 45, 2.0, True, 6),        # odd jtot: the last row pair is shifted
+    (48, 4, 3.0, True, 6),         # jtot = 4: psi row ja-2 wraps to jtot-1
+    (64, 96, 3.0, False, 6),       # no temperature equation (two fields)
+    (1500, 16, 3.0, True, 3),      # 256-thread rows
+    (4500, 8, 1e6, True, 2),       # 1024-thread rows
+]
+
+
+@pytest.mark.parametrize("case", FOLD_CASES, ids=lambda c: "x".join(map(str, c[:2])))
+def test_folded_correction_bitwise(sns, case):
+    """With implicit x and explicit y the predictors of substages 2 and 3
+    apply the previous substage's correction as they load the fields; every
+    field after whole steps equals, bit for bit, the sequence with the
+    correction kernel after each substage (split_corr)."""
+    itot, jtot, stretch, with_t, steps = case
+    out = []
+    for split in (0, 1):
+        inst = sns.Instance(itot, jtot, stretch=stretch, Ra=1e8, Pr=0.7, implicitx=1,
+                            implicity=0, with_temperature=with_t)
+        inst.set_launch(split_corr=split)
+        assert inst.launch()["split_corr"] == split
+        inst.state_alloc()
+        r = np.random.default_rng(itot * 7919 + jtot)
+        for n in ("ux", "uy", "p", "temp"):
+            shape = inst.shape(sns._state_shape(n))
+            inst.state_upload(n, 1e-3 * r.standard_normal(shape))
+        inst.advance(steps)
+        out.append({n: inst.state_download(n) for n in
+                    ("ux", "uy", "p", "psi", "temp", "ux_star", "uy_star")})
+        out[-1]["time"] = np.array(inst.get_time())
+        inst.close()
+    for n in out[0]:
+        assert same(out[0][n], out[1][n]), (case, n)

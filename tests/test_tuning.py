@@ -49,6 +49,14 @@ def test_unknown_shape_name():
         sns.launch_check(64, 128, fft_radix=4)
 
 
+def test_split_corr_values():
+    sns.launch_check(64, 128, split_corr=0)
+    sns.launch_check(64, 128, split_corr=1)
+    for bad in (-1, 2):
+        with pytest.raises(sns.GpuError):
+            sns.launch_check(64, 128, split_corr=bad)
+
+
 def test_committed_table():
     """Every entry of the committed table is a valid shape set for its size,
     the sizes are the autotuner's, and no entry is slower than the rules it
