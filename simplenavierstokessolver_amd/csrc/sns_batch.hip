@@ -128,7 +128,9 @@ int lane_build(Lane &k) {
   for (int p = 0; p < 2; p++)
     for (int rk = 0; rk < 3; rk++) {
       o_pred[p][rk] = reserve(nz * sizeof(PredArgs));
-      for (int f = 0; f < 3; f++) o_y[p][rk][f] = reserve(nz * sizeof(YArgs));
+      // the three fields' blocks contiguous: one launch covers them
+      const size_t oy = reserve(3 * nz * sizeof(YArgs));
+      for (int f = 0; f < 3; f++) o_y[p][rk][f] = oy + f * nz * sizeof(YArgs);
     }
   for (int rk = 0; rk < 3; rk++) {
     o_pois[rk] = reserve(nz * sizeof(PoisArgs));
@@ -191,12 +193,15 @@ void lane_launch(const Lane &k, int p) {
   if (k.impy && sh.yf.L) launch_yfactor_batch(s, k.yf, nz);
   for (int rk = 0; rk < 3; rk++) {
     launch_pred_batch(s, sh.pred[rk], k.impx, k.pred[p][rk], nz);
-    if (k.impy)
-      for (int f = 0; f < 3; f++) {
-        if (f == 0 && !k.with_temp) continue;
-        if (sh.y[rk][f].L) launch_ypart_batch(s, kYKinds[f], sh.y[rk][f], k.y[p][rk][f], nz);
-        else launch_ysolve_batch(s, kYKinds[f], sh.y[rk][f], k.y[p][rk][f], nz);
+    if (k.impy) {
+      const int f0 = k.with_temp ? 0 : 1;
+      if (sh.y[rk][f0].L) {
+        launch_ypart_fields_batch(s, sh.y[rk][f0], k.y[p][rk][f0], nz, 3 - f0);
+      } else {
+        for (int f = f0; f < 3; f++)
+          launch_ysolve_batch(s, kYKinds[f], sh.y[rk][f], k.y[p][rk][f], nz);
       }
+    }
     for (int stage = 0; stage < 3; stage++)
       launch_poisson_batch(s, sh.pois[rk], k.pois[rk], nz, stage);
     if (sh.corr_on[rk]) launch_corr_batch(s, sh.corr[rk], k.corr[rk], nz);

@@ -854,7 +854,7 @@ extern "C" int snsg_instance_bytes(const snsg_config *cfg, size_t *bytes) {
   if (cfg->implicity) {
     n += d * (2 * itot + itot - 1) * jtot;
     const int L = launch_key((int)itot, (int)jtot, tuned_launch((int)itot, (int)jtot)).chunk;
-    if (L > 1) n += d * 9 * ysolve_table_doubles(L) + 3 * d * (jtot / L) * itot;
+    if (L > 1) n += d * 9 * ysolve_table_doubles(L) + 9 * d * (jtot / L) * itot;
   }
   if (cfg->implicitx) n += d * lu_table_doubles(lu_table_entries((int)itot));
   Geo g{};
@@ -1027,8 +1027,9 @@ int step_prepare(snsg_ctx *c) {
   // the chunk length (the launch shapes may have changed it)
   auto ybytes = [&](int L, size_t &nt, size_t &nz) {
     nt = sizeof(double) * 9 * ysolve_table_doubles(L);
-    // interface values and the first and last interior values of each chunk
-    nz = sizeof(double) * 3 * (size_t)(jtot / L) * itot;
+    // interface values and the first and last interior values of each
+    // chunk, for each of the three fields (their solves run concurrently)
+    nz = sizeof(double) * 9 * (size_t)(jtot / L) * itot;
   };
   int L = c->treat.implicity ? launch_key(c).chunk : c->ychunk;
   if (L == 1) L = 0;  // transposed solves
@@ -1137,6 +1138,7 @@ void step_args(const snsg_ctx *c, const snsg_state *st, bool reduce, StepArgs &s
       y.g = g;
       y.rk = rk_of(c, rk);
       y.dtp = c->d_dt;
+      y.kind = kYKinds[f];
       y.n = kYKinds[f] == KIND_UX ? g.itot - 1 : g.itot;
       y.in = ins[f];
       y.qf = c->d_qf[f];
@@ -1145,7 +1147,7 @@ void step_args(const snsg_ctx *c, const snsg_state *st, bool reduce, StepArgs &s
       y.L = yL;
       y.S = yL ? g.jtot / yL : 0;
       y.yfac = yL ? c->d_yfac + (size_t)(rk * 3 + f) * ystride : nullptr;
-      y.qz = c->d_qz;
+      y.qz = yL ? c->d_qz + (size_t)f * 3 * (g.jtot / yL) * g.itot : nullptr;
       y.cols = c->launch.ysolve_cols;
       y.recompute = c->launch.ysolve_recompute;
     }
@@ -1185,13 +1187,16 @@ static int integrate_fused(snsg_ctx *c, snsg_state *st, bool reduce) {
   for (int rk = 0; rk < 3; rk++) {
     if ((rc = run(c, "pred", [&] { launch_pred(c->stream, sa.pred[rk], impx); }))) return rc;
     if (impy) {
-      for (int f = 0; f < 3; f++) {
-        if (f == 0 && !with_temp) continue;
-        rc = run(c, "ysolve", [&] {
-          if (sa.y[rk][f].L) launch_ypart(c->stream, kYKinds[f], sa.y[rk][f]);
-          else launch_ysolve(c->stream, kYKinds[f], sa.y[rk][f]);
-        });
+      // the partitioned solves of all fields in one launch per kernel
+      const int f0 = with_temp ? 0 : 1;
+      if (sa.y[rk][f0].L) {
+        rc = run(c, "ysolve", [&] { launch_ypart_fields(c->stream, &sa.y[rk][f0], 3 - f0); });
         if (rc) return rc;
+      } else {
+        for (int f = f0; f < 3; f++) {
+          rc = run(c, "ysolve", [&] { launch_ysolve(c->stream, kYKinds[f], sa.y[rk][f]); });
+          if (rc) return rc;
+        }
       }
     }
     const PoisArgs &pa = sa.pois[rk];

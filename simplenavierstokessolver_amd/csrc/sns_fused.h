@@ -84,10 +84,13 @@ int ysolve_cols_rule(int SR);
 size_t ysolve_table_doubles(int L);
 void launch_yfactor(hipStream_t s, const YFactorArgs &a);
 void launch_yfactor_batch(hipStream_t s, const YFactorArgs *dev, int nz);
-// y solve of one field (kind) on the row-major increments a.qf, then
-// out = in + increment with boundary values (a.L > 0)
-void launch_ypart(hipStream_t s, int kind, const YArgs &a);
-void launch_ypart_batch(hipStream_t s, int kind, const YArgs &shape, const YArgs *dev, int nz);
+// The partitioned CN-y solve (a.L > 0) on the row-major increments a.qf,
+// then out = in + increment with boundary values, for nf consecutive fields
+// of the order T, ux, uy (a[0..nf-1]), each kernel launched once for all
+void launch_ypart_fields(hipStream_t s, const YArgs *a, int nf);
+// and for nz instances: dev holds nf blocks of nz arguments, field-major
+void launch_ypart_fields_batch(hipStream_t s, const YArgs &shape, const YArgs *dev, int nz,
+                               int nf);
 
 bool pred_impx_supported(int itot);
 int lu_table_entries(int itot);

@@ -65,6 +65,7 @@ struct YArgs {
   Geo g;
   RK rk;
   const double *dtp;
+  int kind;          // field kind (KIND_TEMP, KIND_UX, KIND_UY)
   int n;             // unknowns per x row: itot-1 (ux) or itot
   const double *in;  // substage-start field
   double *qf, *qy;

@@ -242,6 +242,7 @@ __global__ void __launch_bounds__(kYRedNT) k_yreduced(S src) {
   __shared__ double scr[2 * 2];
   __shared__ double ends[3];
   const int col = blockIdx.x;
+  if (col >= a.n) return;  // the launch covers the widest field of a group
   const int n = a.n, L = a.L, SR = a.S - 1;
   const YTab tb{a.yfac, L - 1};
   const double lam = tb.lambda(), O = tb.O();
@@ -400,19 +401,22 @@ static void yreduced_c_launch(hipStream_t s, int n, int SR, S src, int nz) {
 // Chunk s = blockIdx.y*4 + wave, column ii = lane: x = p_s on the interface
 // row, the interior solution (solved again) with the spikes of both
 // interfaces on the others; out = in + x with boundary values, row j = row+1.
-template <int KIND>
 __device__ __forceinline__ void yput(const YArgs &a, int ii, int row, double x) {
   const Geo &g = a.g;
-  const int i = ii + (KIND == KIND_UX ? 2 : 1), j = row + 1;
-  if (KIND == KIND_UX) put_ux(g, a.out, i, j, a.in[ux_idx(g, i, j)] + x);
-  else if (KIND == KIND_UY) put_c<0>(g, a.out, i, j, a.in[c_idx(g, i, j)] + x);
-  else put_c<2>(g, a.out, i, j, a.in[c_idx(g, i, j)] + x);
+  const int j = row + 1;
+  if (a.kind == KIND_UX) {
+    put_ux(g, a.out, ii + 2, j, a.in[ux_idx(g, ii + 2, j)] + x);
+  } else if (a.kind == KIND_UY) {
+    put_c<0>(g, a.out, ii + 1, j, a.in[c_idx(g, ii + 1, j)] + x);
+  } else {
+    put_c<2>(g, a.out, ii + 1, j, a.in[c_idx(g, ii + 1, j)] + x);
+  }
 }
 
 // Row blockIdx.y (j = row+1) after k_ylocal<true>: x = p_s on an interface
 // row, else the stored interior solution with the spikes of both
 // interfaces.
-template <int KIND, class S>
+template <class S>
 __global__ void __launch_bounds__(256) k_yapply_rows(S src) {
   const YArgs a = src.get();
   const int n = a.n;
@@ -432,10 +436,10 @@ __global__ void __launch_bounds__(256) k_yapply_rows(S src) {
     const double y = a.qf[(size_t)row * n + ii];
     x = y - lam * zs * tb.v()[m - 1] - lam * zn * tb.v()[a.L - 1 - m];
   }
-  yput<KIND>(a, ii, row, x);
+  yput(a, ii, row, x);
 }
 
-template <int KIND, class S>
+template <class S>
 __global__ void __launch_bounds__(256) k_yapply(S src) {
   const YArgs a = src.get();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -448,11 +452,11 @@ __global__ void __launch_bounds__(256) k_yapply(S src) {
   const double zs = a.qz[(size_t)s * n + ii], zn = a.qz[(size_t)s1 * n + ii];
   double x[kYMaxL - 1];
   ychunk_solve(a, tb, s, ii, x);
-  yput<KIND>(a, ii, s * a.L, zs);
+  yput(a, ii, s * a.L, zs);
 #pragma unroll
   for (int m = 1; m < kYMaxL; m++)
     if (m <= L1)
-      yput<KIND>(a, ii, s * a.L + m,
+      yput(a, ii, s * a.L + m,
                  x[m - 1] - lam * zs * tb.v()[m - 1] - lam * zn * tb.v()[a.L - 1 - m]);
 }
 
@@ -492,10 +496,12 @@ static int y_cols(int SR, int chosen) {
   return ysolve_cols_rule(SR);
 }
 
+// The partitioned solve of nz argument blocks (instances, or fields of one
+// or more instances; each kernel checks its block's n): n the widest n
+// among them, nrule the instances the recompute rule counts, per field.
 template <class S>
-static void ypart(hipStream_t s, int kind, const YArgs &shape, S src, int nz) {
-  const int n = shape.n;
-  const bool recompute = y_recompute((size_t)n * shape.S * nz, shape.recompute);
+static void ypart(hipStream_t s, const YArgs &shape, int n, S src, int nz, int nrule) {
+  const bool recompute = y_recompute((size_t)n * shape.S * nrule, shape.recompute);
   const dim3 gc((n + 63) / 64, (shape.S + 3) / 4, nz);
   if (recompute) hipLaunchKernelGGL((k_ylocal<false, S>), gc, dim3(256), 0, s, src);
   else hipLaunchKernelGGL((k_ylocal<true, S>), gc, dim3(256), 0, s, src);
@@ -507,23 +513,37 @@ static void ypart(hipStream_t s, int kind, const YArgs &shape, S src, int nz) {
   else if (cols == 32) yreduced_c_launch<32>(s, n, SR, src, nz);
   else yreduced_c_launch<16>(s, n, SR, src, nz);
   if (recompute) {
-    if (kind == KIND_UX) hipLaunchKernelGGL((k_yapply<KIND_UX, S>), gc, dim3(256), 0, s, src);
-    else if (kind == KIND_UY) hipLaunchKernelGGL((k_yapply<KIND_UY, S>), gc, dim3(256), 0, s, src);
-    else hipLaunchKernelGGL((k_yapply<KIND_TEMP, S>), gc, dim3(256), 0, s, src);
+    hipLaunchKernelGGL((k_yapply<S>), gc, dim3(256), 0, s, src);
   } else {
     const dim3 gr((n + 255) / 256, shape.g.jtot, nz);
-    if (kind == KIND_UX) hipLaunchKernelGGL((k_yapply_rows<KIND_UX, S>), gr, dim3(256), 0, s, src);
-    else if (kind == KIND_UY) hipLaunchKernelGGL((k_yapply_rows<KIND_UY, S>), gr, dim3(256), 0, s, src);
-    else hipLaunchKernelGGL((k_yapply_rows<KIND_TEMP, S>), gr, dim3(256), 0, s, src);
+    hipLaunchKernelGGL((k_yapply_rows<S>), gr, dim3(256), 0, s, src);
   }
 }
 
-void launch_ypart(hipStream_t s, int kind, const YArgs &a) {
-  ypart(s, kind, a, ArgOne<YArgs>{a}, 1);
+// Up to three fields' arguments by value, field blockIdx.z
+struct YFields {
+  YArgs f[3];
+  __device__ __forceinline__ YArgs get() const {
+    return blockIdx.z == 0 ? f[0] : (blockIdx.z == 1 ? f[1] : f[2]);
+  }
+};
+
+static int widest(const YArgs *a, int nf) {
+  int n = 0;
+  for (int f = 0; f < nf; f++) n = a[f].n > n ? a[f].n : n;
+  return n;
 }
 
-void launch_ypart_batch(hipStream_t s, int kind, const YArgs &shape, const YArgs *dev, int nz) {
-  ypart(s, kind, shape, arg_batch<YArgs>(dev), nz);
+void launch_ypart_fields(hipStream_t s, const YArgs *a, int nf) {
+  YFields src{};
+  for (int f = 0; f < nf; f++) src.f[f] = a[f];
+  ypart(s, a[0], widest(a, nf), src, nf, 1);
+}
+
+void launch_ypart_fields_batch(hipStream_t s, const YArgs &shape, const YArgs *dev, int nz,
+                               int nf) {
+  // every field's n is itot or itot-1, so itot covers them
+  ypart(s, shape, shape.g.itot, arg_batch<YArgs>(dev), nf * nz, nz);
 }
 
 void launch_yfactor(hipStream_t s, const YFactorArgs &a) {
