@@ -411,8 +411,8 @@ def measured_traffic(group, cells):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
-              "(multi-GPU sharding is described in DESIGN.md, not implemented)" % a.gpus,
+        print("bench.py: the solver runs on one GPU; --gpus %d is not supported "
+              "(a y-slab sharding over GPUs is designed, not implemented)" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
     import simplenavierstokessolver_amd as sns
