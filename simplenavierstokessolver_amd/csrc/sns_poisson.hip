@@ -462,27 +462,33 @@ __global__ void __launch_bounds__(NTF, 8) k_pinv_a(S src) {
   pinv_a<L, false>(a.g, a.pl, a.tw, a.Y, a.psi);
 }
 
-static constexpr int kPxM = 8;
-// threads of the x solve for n = itot unknowns (NT*kPxM >= n)
-static int px_threads(int n) {
-  if (n <= 64 * kPxM) return 64;
-  if (n <= 128 * kPxM) return 128;
-  if (n <= 256 * kPxM) return 256;
-  if (n <= 512 * kPxM) return 512;
-  return 1024;
+// Threads NT and unknowns per thread M of the x solve for n = itot unknowns
+// (NT*M >= n).  One workgroup solves one wavenumber row, so with only
+// jtot/2+1 rows the narrower grids take 4 unknowns per thread (twice the
+// waves per row) and the wider ones 8.
+static void px_shape(int n, int &NT, int &M) {
+  M = n <= 1024 ? 4 : 8;
+  NT = 64;
+  while (NT < 1024 && NT * M < n) NT *= 2;
 }
 
-int poisson_factor_entries(int itot) { return px_threads(itot) * kPxM; }
+int poisson_factor_entries(int itot) {
+  int NT, M;
+  px_shape(itot, NT, M);
+  return NT * M;
+}
 
 size_t poisson_factor_doubles(int itot, int jtot) {
   return (size_t)(jtot / 2 + 1) * (poisson_factor_entries(itot) + 3);
 }
 
 void poisson_lu_coefs(int itot, const double *pl, const double *pu, double *plu) {
-  const int NT = px_threads(itot), ntm = NT * kPxM;
+  int NT, M;
+  px_shape(itot, NT, M);
+  const int ntm = NT * M;
   for (int k = 0; k < 2 * ntm; k++) plu[k] = 0.;
   for (int i = 0; i < itot; i++) {
-    const int slot = (i % kPxM) * NT + i / kPxM;
+    const int slot = (i % M) * NT + i / M;
     plu[slot] = pl[i];
     plu[ntm + slot] = pu[i];
   }
@@ -490,10 +496,12 @@ void poisson_lu_coefs(int itot, const double *pl, const double *pu, double *plu)
 
 void launch_poisson_factor(hipStream_t s, const Geo &g, const Tables &tb, const double *eigen,
                            double *fac) {
-  const int NT = px_threads(g.itot), ntm = NT * kPxM;
+  int NT, M;
+  px_shape(g.itot, NT, M);
+  const int ntm = NT * M;
   const int nJ = g.jtot / 2 + 1;
   hipLaunchKernelGGL(k_poisson_factor, dim3((nJ + 63) / 64), dim3(64), 0, s, g, tb, eigen, NT,
-                     kPxM, ntm, fac);
+                     M, ntm, fac);
 }
 
 template <int NT, int M, class S>
@@ -602,12 +610,20 @@ static void poisson_stage(hipStream_t s, const Geo &g, const Fft2Plan &pl, bool 
     SNS_P2_LAUNCH(k_pfwd_b, Lb, with_z(grid_b(pl), nz));
   }
   if (stage == 1 || stage < 0) {
-    switch (px_threads(g.itot)) {
-      case 64: px_launch<64, kPxM>(s, g, src, nz); break;
-      case 128: px_launch<128, kPxM>(s, g, src, nz); break;
-      case 256: px_launch<256, kPxM>(s, g, src, nz); break;
-      case 512: px_launch<512, kPxM>(s, g, src, nz); break;
-      default: px_launch<1024, kPxM>(s, g, src, nz); break;
+    int NT, M;
+    px_shape(g.itot, NT, M);
+    if (M == 4) {
+      switch (NT) {
+        case 64: px_launch<64, 4>(s, g, src, nz); break;
+        case 128: px_launch<128, 4>(s, g, src, nz); break;
+        default: px_launch<256, 4>(s, g, src, nz); break;
+      }
+    } else {
+      switch (NT) {
+        case 256: px_launch<256, 8>(s, g, src, nz); break;
+        case 512: px_launch<512, 8>(s, g, src, nz); break;
+        default: px_launch<1024, 8>(s, g, src, nz); break;
+      }
     }
   }
   if (stage == 2 || stage < 0) {
