@@ -126,7 +126,9 @@ __device__ __forceinline__ void aff_after(double &A, double &B, double A1, doubl
 // Exclusive affine scan over the threads of a workgroup (forward: thread t
 // receives the composition of maps 0..t-1 applied to 0; reverse: of maps
 // NT-1..t+1).  NF independent scans share the barriers.  scr holds
-// 2*NF*(NT/64) doubles.
+// 2*NF*(NT/64) doubles; the caller orders any later write of scr after the
+// reads here (another scan uses other scratch, and the callers' next LDS
+// phase starts behind a barrier of their own).
 template <int NT, int NF, bool REVERSE>
 __device__ __forceinline__ void affine_scan_excl(double (&A)[NF], double (&B)[NF], double (&C)[NF],
                                                  double *scr) {
@@ -166,11 +168,12 @@ __device__ __forceinline__ void affine_scan_excl(double (&A)[NF], double (&B)[NF
     const bool edge = REVERSE ? (lane == 63) : (lane == 0);
     C[f] = edge ? cin : prev;
   }
-  __syncthreads();
 }
 
 // Solve NS systems that share one matrix (tab) for this thread's chunk
-// d[s][0..M-1] (unknowns t*M .. t*M+M-1).
+// d[s][0..M-1] (unknowns t*M .. t*M+M-1).  scr holds 4*NS*(NT/64) doubles
+// (the two scans' scratch apart); the caller places a barrier between this
+// call and its next write of scr.
 template <int NT, int M, int NS, class TAB = LuTab>
 __device__ __forceinline__ void lu_solve_shared(double (&d)[NS][M], const TAB &tab,
                                                 double *scr) {
@@ -223,7 +226,7 @@ __device__ __forceinline__ void lu_solve_shared(double (&d)[NS][M], const TAB &t
       B[s] = x[s];
     }
   }
-  affine_scan_excl<NT, NS, true>(A, B, C, scr);
+  affine_scan_excl<NT, NS, true>(A, B, C, scr + 2 * NS * (NT / 64));
 #pragma unroll
   for (int s = 0; s < NS; s++)
 #pragma unroll

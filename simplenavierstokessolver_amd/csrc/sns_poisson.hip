@@ -369,7 +369,7 @@ __global__ void __launch_bounds__(64)
 template <int NT, int M>
 struct PxLds {
   double2 row[NT * (M + 1)];
-  double scr[2 * 2 * (NT / 64)];
+  double scr[4 * 2 * (NT / 64)];
 };
 
 // One workgroup per wavenumber row; the row is staged through LDS (coalesced

@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(NT) k_lu_factor(S src) {
 template <int NT, int M>
 struct PredRowsLds {
   double d[2][NT * (M + 1)];  // padded: chunk t starts at t*(M+1)
-  double scr[2 * 2 * (NT / 64)];
+  double scr[4 * 2 * (NT / 64)];
 };
 
 template <int M>

@@ -239,7 +239,7 @@ __global__ void __launch_bounds__(256) k_ylocal(S src) {
 template <class S>
 __global__ void __launch_bounds__(kYRedNT) k_yreduced(S src) {
   const YArgs a = src.get();
-  __shared__ double scr[2 * 2];
+  __shared__ double scr[4 * 1 * (kYRedNT / 64)];
   __shared__ double ends[3];
   const int col = blockIdx.x;
   if (col >= a.n) return;  // the launch covers the widest field of a group
