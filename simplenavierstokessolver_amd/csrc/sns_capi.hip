@@ -610,6 +610,45 @@ static int potential(snsg_ctx *c, const Geo &g, double gamma, const double *ux, 
   return run(c, "poisson_inv", [&] { launch_poisson(c->stream, a, 2); });
 }
 
+// Crank-Nicolson in x of one field on the reference layouts: the LU tables
+// of all substages for the current dt (k_lu_factor), then the row solves of
+// substage rkstep, into the field (qout == nullptr) or into qout.
+static int cnx_solve(snsg_ctx *c, int kind, int rkstep, const double *srca, const double *srcb,
+                     const double *srcg, double *field, double *qout) {
+  const Geo &g = c->geo;
+  int rc;
+  if (!lu_row_threads(g.itot))
+    return set_err(SNSG_E_SIZE, "implicit x needs itot <= 8192");
+  if (!c->d_lu) {
+    c->lu_ntm = lu_table_entries(g.itot);
+    if ((rc = ctx_alloc((void **)&c->d_lu, sizeof(double) * lu_table_doubles(c->lu_ntm),
+                        &c->mem)))
+      return rc;
+  }
+  LuFactorArgs lf{};
+  lf.g = g;
+  lf.tb = c->tab;
+  lf.dtp = c->d_dt;
+  for (int k = 0; k < 3; k++) lf.gamma[k] = c->cfg.gamma[k];
+  lf.out = c->d_lu;
+  lf.ntm = c->lu_ntm;
+  if ((rc = run(c, "lu_factor", [&] { launch_lu_factor(c->stream, lf); }))) return rc;
+  const int f = kind == KIND_TEMP ? 0 : (kind == KIND_UX ? 1 : 2);
+  CnxLuArgs a{};
+  a.g = g;
+  a.rk = rk_of(c, rkstep);
+  a.dtp = c->d_dt;
+  a.srca = srca;
+  a.srcb = srcb;
+  a.srcg = srcg;
+  a.field = field;
+  a.qout = qout;
+  a.ntm = c->lu_ntm;
+  a.lu = c->d_lu + (size_t)(rkstep * 3 * 3 + f * 3) * c->lu_ntm;
+  a.lu_hdr = c->d_lu + 27 * (size_t)c->lu_ntm + (rkstep * 3 + f) * kLuHdr;
+  return run(c, "cnx", [&] { launch_cnx_lu(c->stream, kind, a); });
+}
+
 static int update_field(snsg_ctx *c, int kind, int rkstep, const double *srca, const double *srcb,
                         const double *srcg, double *field) {
   const Geo &g = c->geo;
@@ -621,14 +660,10 @@ static int update_field(snsg_ctx *c, int kind, int rkstep, const double *srca, c
       return run(c, "incr", [&] {
         launch_incr(c->stream, g, kind, rk, c->d_dt, srca, srcb, srcg, field, nullptr);
       });
-    return run(c, "cnx", [&] {
-      launch_cnx(c->stream, g, kind, rk, c->d_dt, srca, srcb, srcg, nullptr, field, nullptr);
-    });
+    return cnx_solve(c, kind, rkstep, srca, srcb, srcg, field, nullptr);
   }
   if (c->treat.implicitx)
-    rc = run(c, "cnx", [&] {
-      launch_cnx(c->stream, g, kind, rk, c->d_dt, srca, srcb, srcg, nullptr, nullptr, c->d_q);
-    });
+    rc = cnx_solve(c, kind, rkstep, srca, srcb, srcg, nullptr, c->d_q);
   else
     rc = run(c, "incr", [&] {
       launch_incr(c->stream, g, kind, rk, c->d_dt, srca, srcb, srcg, nullptr, c->d_q);

@@ -58,6 +58,24 @@ struct DtArgs {
 // bit for bit, and that triple (lu_table_doubles in all)
 constexpr int kLuHdr = 4;
 inline size_t lu_table_doubles(int ntm) { return 27 * (size_t)ntm + 9 * kLuHdr; }
+// threads of the row kernels for itot (the tables interleave lu_table_entries
+// = threads x 8 unknowns per thread); 0 where itot is not supported
+int lu_row_threads(int itot);
+
+// Crank-Nicolson in x on the reference layouts (the per-function entry
+// points): the increment of each row from the RK sources (or qin), solved
+// with the LU tables of one substage and field (lu, lu_hdr: as PredArgs),
+// then added to `field` with boundary values, or stored to qout.
+struct CnxLuArgs {
+  Geo g;
+  RK rk;
+  const double *dtp;
+  const double *srca, *srcb, *srcg, *qin;
+  double *field, *qout;
+  const double *lu, *lu_hdr;
+  int ntm;
+};
+void launch_cnx_lu(hipStream_t s, int kind, const CnxLuArgs &a);
 struct LuFactorArgs {
   Geo g;
   Tables tb;

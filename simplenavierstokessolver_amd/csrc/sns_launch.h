@@ -106,12 +106,6 @@ void launch_incr(hipStream_t s, const Geo &g, int kind, RK rk, const double *dtp
 // field += q (x-aligned), then boundary values
 void launch_addq(hipStream_t s, const Geo &g, int kind, const double *q, double *field);
 
-// ---- Crank-Nicolson in x: one workgroup per row.  Input increment from the
-// sources (qin == nullptr) or from qin; output to the field (qout == nullptr)
-// or to qout (x-aligned).
-void launch_cnx(hipStream_t s, const Geo &g, int kind, RK rk, const double *dtp,
-                const double *srca, const double *srcb, const double *srcg, const double *qin,
-                double *field, double *qout);
 // ---- Crank-Nicolson in y on y-aligned rows (after a transpose): periodic
 // solve of each of nrows rows of length jtot, in place.
 void launch_cny_rows(hipStream_t s, const Geo &g, int kind, RK rk, const double *dtp, int nrows,

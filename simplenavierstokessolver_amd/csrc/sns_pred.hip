@@ -669,6 +669,11 @@ int lu_table_entries(int itot) {
   return row_config(itot, NT, M);
 }
 
+int lu_row_threads(int itot) {
+  int NT, M;
+  return row_config(itot, NT, M) ? NT : 0;
+}
+
 template <class S>
 static void lu_factor_dispatch(hipStream_t s, int itot, S src, int nz) {
   int NT, M;

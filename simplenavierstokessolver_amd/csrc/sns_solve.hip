@@ -3,6 +3,7 @@
 #include "sns_tridiag.h"
 #include "sns_fast.h"
 #include "sns_fused.h"
+#include "sns_lu.h"
 
 namespace sns {
 
@@ -14,14 +15,6 @@ __device__ __forceinline__ double cn_prefactor(const Geo &g, double gamma, doubl
   // ux,uy: (gamma*dt*sqrt(Pr))/(2.*sqrt(Ra))   src/fluid/update_velocity.c:254,399
   return KIND == KIND_TEMP ? (gamma * dt) / g.two_sqrtPr_sqrtRa
                            : (gamma * dt * g.sqrtPr) / g.two_sqrtRa;
-}
-
-template <int KIND>
-__device__ __forceinline__ void cnx_coef(const Geo &g, double pf, int i, double &l, double &c,
-                                         double &u) {
-  if (KIND == KIND_UX) cnx_coef_ux(g, pf, i, l, c, u);
-  else if (KIND == KIND_UY) cnx_coef_uy(g, pf, i, l, c, u);
-  else cnx_coef_temp(g, pf, i, l, c, u);
 }
 
 template <int KIND>
@@ -80,79 +73,85 @@ __device__ __forceinline__ double field_at(const Geo &g, const double *f, int i,
 
 // ------------------------------------------------- Crank-Nicolson in x
 
-// One workgroup per row j: assemble the increment (from the RK sources or
-// from qin), solve (1 - pf*Dxx) q = increment along x, then either add to the
-// field with boundary values or store to qout.
-// src/temperature/update.c:123-167; src/fluid/update_velocity.c:230-280,380-424
+// One workgroup per row j (src/temperature/update.c:123-167,
+// src/fluid/update_velocity.c:230-280,380-424): the increment, from the RK
+// sources or from qin, is loaded coalesced and staged in LDS in the chunk
+// order of the substitutions, solved with the LU tables of k_lu_factor (the
+// fused predictor's solve, so the two paths return the same bits), and added
+// to the field with boundary values, or stored to qout.
 template <int KIND, int NT, int M>
-__global__ void __launch_bounds__(NT)
-    k_cnx(Geo g, RK rk, const double *__restrict__ dtp, const double *__restrict__ srca,
-          const double *__restrict__ srcb, const double *__restrict__ srcg,
-          const double *__restrict__ qin, double *__restrict__ field, double *__restrict__ qout) {
-  __shared__ TridiagLds<NT, 1> lds;
+__global__ void __launch_bounds__(NT) k_cnx_lu(CnxLuArgs a) {
+  extern __shared__ double cn_lds[];
+  double *row = cn_lds;                 // NT*(M+1): chunk t at t*(M+1)
+  double *scr = cn_lds + NT * (M + 1);  // scan scratch of one system
+  const Geo &g = a.g;
   const int j = blockIdx.x + 1;
   const int n = KIND == KIND_UX ? g.itot - 1 : g.itot;
   const int i0 = KIND == KIND_UX ? 2 : 1;
-  const double dt = *dtp;
-  const double pf = cn_prefactor<KIND>(g, rk.gamma, dt);
-  double a[M], b[M], c[M], d[1][M];
+  const double dt = *a.dtp;
+  const RK rk = a.rk;
   const size_t rowoff = (size_t)(j - 1) * n;
-#pragma unroll
-  for (int k = 0; k < M; k++) {
-    const int r = threadIdx.x * M + k;
-    if (r < n) {
-      double l, cc, u;
-      cnx_coef<KIND>(g, pf, r + i0, l, cc, u);
-      a[k] = (r == 0) ? 0. : l;
-      b[k] = cc;
-      c[k] = (r == n - 1) ? 0. : u;
-      const size_t s = rowoff + r;
-      d[0][k] = qin ? qin[s] : rk_incr(rk.alpha, rk.beta, rk.gamma, dt, srca[s], srcb[s], srcg[s]);
-    } else {
-      a[k] = 0.; b[k] = 1.; c[k] = 0.; d[0][k] = 0.;
-    }
+  for (int r = threadIdx.x; r < n; r += NT) {
+    const size_t s = rowoff + r;
+    row[r + r / M] = a.qin ? a.qin[s]
+                           : rk_incr(rk.alpha, rk.beta, rk.gamma, dt, a.srca[s], a.srcb[s],
+                                     a.srcg[s]);
   }
-  block_tridiag<NT, M, 1>(a, b, c, d, lds);
+  __syncthreads();
+  double d[1][M];
 #pragma unroll
   for (int k = 0; k < M; k++) {
     const int r = threadIdx.x * M + k;
-    if (r < n) {
-      if (qout) {
-        qout[rowoff + r] = d[0][k];
-      } else {
-        const int i = r + i0;
-        store_field_bc<KIND>(g, field, i, j, field_at<KIND>(g, field, i, j) + d[0][k]);
-      }
+    d[0][k] = r < n ? row[r + r / M] : 0.;
+  }
+  const LuTabC<M> tab = {a.lu, a.lu + a.ntm, a.lu + 2 * a.ntm, (int)a.lu_hdr[0], n,
+                         a.lu_hdr[1], a.lu_hdr[2], a.lu_hdr[3]};
+  lu_solve_shared<NT, M, 1>(d, tab, scr);
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const int r = threadIdx.x * M + k;
+    if (r < n) row[r + r / M] = d[0][k];
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < n; r += NT) {
+    const double x = row[r + r / M];
+    if (a.qout) {
+      a.qout[rowoff + r] = x;
+    } else {
+      const int i = r + i0;
+      store_field_bc<KIND>(g, a.field, i, j, field_at<KIND>(g, a.field, i, j) + x);
     }
   }
 }
 
-template <int KIND, int NT, int M>
-static void cnx_launch(hipStream_t s, const Geo &g, RK rk, const double *dtp, const double *srca,
-                       const double *srcb, const double *srcg, const double *qin, double *field,
-                       double *qout) {
-  hipLaunchKernelGGL((k_cnx<KIND, NT, M>), dim3(g.jtot), dim3(NT), 0, s, g, rk, dtp, srca, srcb,
-                     srcg, qin, field, qout);
+template <int KIND, int NT>
+static void cnx_lu_launch(hipStream_t s, const CnxLuArgs &a) {
+  constexpr int M = 8;
+  const size_t lds = sizeof(double) * (NT * (M + 1) + 4 * (NT / 64));
+  static bool attr = false;
+  if (!attr) {
+    hipFuncSetAttribute((const void *)k_cnx_lu<KIND, NT, M>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_cnx_lu<KIND, NT, M>), dim3(a.g.jtot), dim3(NT), lds, s, a);
 }
 
 template <int KIND>
-static void cnx_dispatch(hipStream_t s, const Geo &g, RK rk, const double *dtp, const double *srca,
-                         const double *srcb, const double *srcg, const double *qin, double *field,
-                         double *qout) {
-  const int n = KIND == KIND_UX ? g.itot - 1 : g.itot;
-  if (n <= 64 * 4) cnx_launch<KIND, 64, 4>(s, g, rk, dtp, srca, srcb, srcg, qin, field, qout);
-  else if (n <= 256 * 4) cnx_launch<KIND, 256, 4>(s, g, rk, dtp, srca, srcb, srcg, qin, field, qout);
-  else if (n <= 256 * 8) cnx_launch<KIND, 256, 8>(s, g, rk, dtp, srca, srcb, srcg, qin, field, qout);
-  else if (n <= 256 * 16) cnx_launch<KIND, 256, 16>(s, g, rk, dtp, srca, srcb, srcg, qin, field, qout);
-  else cnx_launch<KIND, 256, 32>(s, g, rk, dtp, srca, srcb, srcg, qin, field, qout);
+static void cnx_lu_dispatch(hipStream_t s, const CnxLuArgs &a) {
+  switch (lu_row_threads(a.g.itot)) {
+    case 64: cnx_lu_launch<KIND, 64>(s, a); break;
+    case 128: cnx_lu_launch<KIND, 128>(s, a); break;
+    case 256: cnx_lu_launch<KIND, 256>(s, a); break;
+    case 512: cnx_lu_launch<KIND, 512>(s, a); break;
+    default: cnx_lu_launch<KIND, 1024>(s, a); break;
+  }
 }
 
-void launch_cnx(hipStream_t s, const Geo &g, int kind, RK rk, const double *dtp,
-                const double *srca, const double *srcb, const double *srcg, const double *qin,
-                double *field, double *qout) {
-  if (kind == KIND_UX) cnx_dispatch<KIND_UX>(s, g, rk, dtp, srca, srcb, srcg, qin, field, qout);
-  else if (kind == KIND_UY) cnx_dispatch<KIND_UY>(s, g, rk, dtp, srca, srcb, srcg, qin, field, qout);
-  else cnx_dispatch<KIND_TEMP>(s, g, rk, dtp, srca, srcb, srcg, qin, field, qout);
+void launch_cnx_lu(hipStream_t s, int kind, const CnxLuArgs &a) {
+  if (kind == KIND_UX) cnx_lu_dispatch<KIND_UX>(s, a);
+  else if (kind == KIND_UY) cnx_lu_dispatch<KIND_UY>(s, a);
+  else cnx_lu_dispatch<KIND_TEMP>(s, a);
 }
 
 // ------------------------------------------------- Crank-Nicolson in y
