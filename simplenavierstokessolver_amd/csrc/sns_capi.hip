@@ -649,6 +649,11 @@ static int cnx_solve(snsg_ctx *c, int kind, int rkstep, const double *srca, cons
   return run(c, "cnx", [&] { launch_cnx_lu(c->stream, kind, a); });
 }
 
+static int ysolve_prepare(snsg_ctx *c);
+static YFactorArgs yfactor_args(const snsg_ctx *c, const Geo &g);
+static YArgs y_args(const snsg_ctx *c, const Geo &g, int rk, int f, const double *in, double *qf,
+                    double *out);
+
 static int update_field(snsg_ctx *c, int kind, int rkstep, const double *srca, const double *srcb,
                         const double *srcg, double *field) {
   const Geo &g = c->geo;
@@ -669,6 +674,17 @@ static int update_field(snsg_ctx *c, int kind, int rkstep, const double *srca, c
       launch_incr(c->stream, g, kind, rk, c->d_dt, srca, srcb, srcg, nullptr, c->d_q);
     });
   if (rc) return rc;
+  if ((rc = ysolve_prepare(c))) return rc;
+  if (c->ychunk) {
+    // the partitioned solve of the fused step on the x-aligned increments;
+    // its apply adds the solution into the field in place (each thread reads
+    // the interior value it then overwrites)
+    const int f = kind == KIND_TEMP ? 0 : (kind == KIND_UX ? 1 : 2);
+    const YFactorArgs yf = yfactor_args(c, g);
+    if ((rc = run(c, "yfactor", [&] { launch_yfactor(c->stream, yf); }))) return rc;
+    const YArgs ya = y_args(c, g, rkstep, f, field, c->d_q, field);
+    return run(c, "ysolve", [&] { launch_ypart_fields(c->stream, &ya, 1); });
+  }
   if ((rc = run(c, "transpose", [&] { launch_transpose(c->stream, n, g.jtot, 8, c->d_q, c->d_qy); })))
     return rc;
   if ((rc = run(c, "cny", [&] { launch_cny_rows(c->stream, g, kind, rk, c->d_dt, n, c->d_qy); })))
@@ -1047,19 +1063,12 @@ extern "C" int snsg_state_decide_dt(snsg_ctx *c, const snsg_state *st, double *d
 const int kYKinds[3] = {KIND_TEMP, KIND_UX, KIND_UY};
 
 // Scratch of the fused step, allocated on first use
-int step_prepare(snsg_ctx *c) {
-  const Geo &g = c->geo_st;
-  const int itot = g.itot, jtot = g.jtot;
+// Factor tables and interface rows of the partitioned y solve, sized by the
+// chunk length (the launch shapes may have changed it); c->ychunk 0 means
+// the transposed column solves.
+static int ysolve_prepare(snsg_ctx *c) {
+  const int itot = c->geo.itot, jtot = c->geo.jtot;
   int rc;
-  if (c->treat.implicitx && !pred_impx_supported(itot))
-    return set_err(SNSG_E_SIZE, "fused step: implicit x needs itot <= 8192");
-  if (c->treat.implicity && !c->d_qf[0]) {
-    const size_t nb[3] = {(size_t)itot * jtot, (size_t)(itot - 1) * jtot, (size_t)itot * jtot};
-    for (int k = 0; k < 3; k++)
-      if ((rc = ctx_alloc((void **)&c->d_qf[k], sizeof(double) * nb[k], &c->mem))) return rc;
-  }
-  // factor tables and interface rows of the partitioned y solve, sized by
-  // the chunk length (the launch shapes may have changed it)
   auto ybytes = [&](int L, size_t &nt, size_t &nz) {
     nt = sizeof(double) * 9 * ysolve_table_doubles(L);
     // interface values and the first and last interior values of each
@@ -1086,6 +1095,60 @@ int step_prepare(snsg_ctx *c) {
       if ((rc = ctx_alloc((void **)&c->d_qz, nz, &c->mem))) return rc;
     }
   }
+  return 0;
+}
+
+// Factors of the partitioned y solves of all substages and fields (g: the
+// layout the solves run on)
+static YFactorArgs yfactor_args(const snsg_ctx *c, const Geo &g) {
+  const int yL = c->treat.implicity ? c->ychunk : 0;
+  YFactorArgs yf{};
+  yf.g = g;
+  yf.dtp = c->d_dt;
+  for (int k = 0; k < 3; k++) yf.gamma[k] = c->cfg.gamma[k];
+  yf.out = c->d_yfac;
+  yf.L = yL;
+  yf.S = yL ? g.jtot / yL : 0;
+  yf.stride = yL ? (long)ysolve_table_doubles(yL) : 0;
+  return yf;
+}
+
+// The y solve of field f (order T, ux, uy) in substage rk: increments qf
+// (x-aligned), out = in + solution with boundary values
+static YArgs y_args(const snsg_ctx *c, const Geo &g, int rk, int f, const double *in, double *qf,
+                    double *out) {
+  const int yL = c->treat.implicity ? c->ychunk : 0;
+  YArgs y{};
+  y.g = g;
+  y.rk = rk_of(c, rk);
+  y.dtp = c->d_dt;
+  y.kind = kYKinds[f];
+  y.n = kYKinds[f] == KIND_UX ? g.itot - 1 : g.itot;
+  y.in = in;
+  y.qf = qf;
+  y.qy = c->d_qy;
+  y.out = out;
+  y.L = yL;
+  y.S = yL ? g.jtot / yL : 0;
+  y.yfac = yL ? c->d_yfac + (size_t)(rk * 3 + f) * ysolve_table_doubles(yL) : nullptr;
+  y.qz = yL ? c->d_qz + (size_t)f * 3 * (g.jtot / yL) * g.itot : nullptr;
+  y.cols = c->launch.ysolve_cols;
+  y.recompute = c->launch.ysolve_recompute;
+  return y;
+}
+
+int step_prepare(snsg_ctx *c) {
+  const Geo &g = c->geo_st;
+  const int itot = g.itot, jtot = g.jtot;
+  int rc;
+  if (c->treat.implicitx && !pred_impx_supported(itot))
+    return set_err(SNSG_E_SIZE, "fused step: implicit x needs itot <= 8192");
+  if (c->treat.implicity && !c->d_qf[0]) {
+    const size_t nb[3] = {(size_t)itot * jtot, (size_t)(itot - 1) * jtot, (size_t)itot * jtot};
+    for (int k = 0; k < 3; k++)
+      if ((rc = ctx_alloc((void **)&c->d_qf[k], sizeof(double) * nb[k], &c->mem))) return rc;
+  }
+  if ((rc = ysolve_prepare(c))) return rc;
   if (c->treat.implicitx && !c->d_lu) {
     c->lu_ntm = lu_table_entries(itot);
     if ((rc = ctx_alloc((void **)&c->d_lu, sizeof(double) * lu_table_doubles(c->lu_ntm), &c->mem))) return rc;
@@ -1123,15 +1186,7 @@ void step_args(const snsg_ctx *c, const snsg_state *st, bool reduce, StepArgs &s
   sa.bound.ux = st->ux;
   sa.bound.uy = st->uy;
   sa.bound.red = c->d_part;
-  const int yL = impy ? c->ychunk : 0;
-  const size_t ystride = yL ? ysolve_table_doubles(yL) : 0;
-  sa.yf.g = g;
-  sa.yf.dtp = c->d_dt;
-  for (int k = 0; k < 3; k++) sa.yf.gamma[k] = c->cfg.gamma[k];
-  sa.yf.out = c->d_yfac;
-  sa.yf.L = yL;
-  sa.yf.S = yL ? g.jtot / yL : 0;
-  sa.yf.stride = (long)ystride;
+  sa.yf = yfactor_args(c, g);
   double *temp = st->temp, *temp_alt = st->temp_alt;
   // Folded corrections (implicit x, explicit y): substage 1 predicts from
   // (ux, uy, p) into (ux*, uy*) and stops at psi; substage 2's predictor
@@ -1168,24 +1223,7 @@ void step_args(const snsg_ctx *c, const snsg_state *st, bool reduce, StepArgs &s
     pa.ntm = c->lu_ntm;
     const double *ins[3] = {temp, in_u, in_v};
     double *outs[3] = {temp_alt, out_u, out_v};
-    for (int f = 0; f < 3; f++) {
-      YArgs &y = sa.y[rk][f];
-      y.g = g;
-      y.rk = rk_of(c, rk);
-      y.dtp = c->d_dt;
-      y.kind = kYKinds[f];
-      y.n = kYKinds[f] == KIND_UX ? g.itot - 1 : g.itot;
-      y.in = ins[f];
-      y.qf = c->d_qf[f];
-      y.qy = c->d_qy;
-      y.out = outs[f];
-      y.L = yL;
-      y.S = yL ? g.jtot / yL : 0;
-      y.yfac = yL ? c->d_yfac + (size_t)(rk * 3 + f) * ystride : nullptr;
-      y.qz = yL ? c->d_qz + (size_t)f * 3 * (g.jtot / yL) * g.itot : nullptr;
-      y.cols = c->launch.ysolve_cols;
-      y.recompute = c->launch.ysolve_recompute;
-    }
+    for (int f = 0; f < 3; f++) sa.y[rk][f] = y_args(c, g, rk, f, ins[f], c->d_qf[f], outs[f]);
     sa.pois[rk] = pois_args(c, g, c->cfg.gamma[rk], out_u, out_v, st->psi);
     sa.corr_on[rk] = !fold || rk == 2;
     CorrArgs &ca = sa.corr[rk];

@@ -341,11 +341,9 @@ FUSED_CASES = [
 @pytest.mark.parametrize("case", FUSED_CASES)
 def test_fused_step_matches_reference_functions(sns, case):
     """The fused step evaluates the same expressions as the reference-named
-    kernels (the division scheme is exact), and both solve Crank-Nicolson in
-    x with the same LU tables and substitutions, so with explicit diffusion
-    in y the fields agree bit for bit.  With Crank-Nicolson in y the fused
-    step uses the partitioned y solve, which rounds differently from the
-    per-function column solves (1e-12)."""
+    kernels (the division scheme is exact), both solve Crank-Nicolson in x
+    with the same LU tables and substitutions, and Crank-Nicolson in y with
+    the same partitioned solve, so the fields agree bit for bit."""
     itot, jtot, stretch, Ra, Pr, imx, imy = case
     o, inst = seeded_pair(sns, itot, jtot, stretch, Ra, Pr, imx, imy, 9, 1e-2)
     inst.state_alloc()
@@ -358,10 +356,7 @@ def test_fused_step_matches_reference_functions(sns, case):
         inst.integrate()
     for n in ["ux", "uy", "p", "temp"]:
         a, b = inst.state_download(n), inst.download(n)
-        if imy == 0:
-            assert np.array_equal(a, b), (n, np.abs(a - b).max())
-        else:
-            assert_close(interior(a), interior(b), 1e-12, n, demean=(n == "p"))
+        assert np.array_equal(a, b), (n, np.abs(a - b).max())
     inst.close()
 
 
