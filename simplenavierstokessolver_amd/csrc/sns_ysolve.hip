@@ -205,14 +205,24 @@ __device__ __forceinline__ void ychunk_solve(const YArgs &a, const YTab &tb, int
     if (m < L1 - 1) x[m] = x[m] - tb.beta()[m] * x[m + 1];
 }
 
-// Wave w of the workgroup: chunk blockIdx.y*4 + w; lane: column.  The
-// chunk's first and last interior values go to qz rows S+s and 2S+s; STORE:
-// the interior solution also replaces the increments.
-template <bool STORE, class S>
+// Chunk and column of a thread of the per-chunk kernels: the four waves of a
+// workgroup cover CW adjacent runs of 64 columns of 4/CW chunks, so each row
+// access of the workgroup is CW*512 contiguous bytes.
+template <int CW>
+__device__ __forceinline__ void ychunk_of(int &s, int &col) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  s = blockIdx.y * (4 / CW) + wave / CW;
+  col = (blockIdx.x * CW + wave % CW) * 64 + lane;
+}
+
+// Thread of chunk s, column col (ychunk_of).  The chunk's first and last
+// interior values go to qz rows S+s and 2S+s; STORE: the interior solution
+// also replaces the increments.
+template <bool STORE, int CW, class S>
 __global__ void __launch_bounds__(256) k_ylocal(S src) {
   const YArgs a = src.get();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int s = blockIdx.y * 4 + wave, col = blockIdx.x * 64 + lane;
+  int s, col;
+  ychunk_of<CW>(s, col);
   if (s >= a.S || col >= a.n) return;
   const int L1 = a.L - 1;
   const YTab tb{a.yfac, L1};
@@ -398,19 +408,24 @@ static void yreduced_c_launch(hipStream_t s, int n, int SR, S src, int nz) {
 
 // ------------------------------------------------------------- apply
 
-// Chunk s = blockIdx.y*4 + wave, column ii = lane: x = p_s on the interface
-// row, the interior solution (solved again) with the spikes of both
-// interfaces on the others; out = in + x with boundary values, row j = row+1.
-__device__ __forceinline__ void yput(const YArgs &a, int ii, int row, double x) {
+// The substage-start value of unknown ii of row `row` (cell row j = row+1)
+__device__ __forceinline__ double yin(const YArgs &a, int ii, int row) {
+  const Geo &g = a.g;
+  return a.kind == KIND_UX ? a.in[ux_idx(g, ii + 2, row + 1)] : a.in[c_idx(g, ii + 1, row + 1)];
+}
+
+// out at unknown ii of row `row` with boundary values
+__device__ __forceinline__ void yset(const YArgs &a, int ii, int row, double v) {
   const Geo &g = a.g;
   const int j = row + 1;
-  if (a.kind == KIND_UX) {
-    put_ux(g, a.out, ii + 2, j, a.in[ux_idx(g, ii + 2, j)] + x);
-  } else if (a.kind == KIND_UY) {
-    put_c<0>(g, a.out, ii + 1, j, a.in[c_idx(g, ii + 1, j)] + x);
-  } else {
-    put_c<2>(g, a.out, ii + 1, j, a.in[c_idx(g, ii + 1, j)] + x);
-  }
+  if (a.kind == KIND_UX) put_ux(g, a.out, ii + 2, j, v);
+  else if (a.kind == KIND_UY) put_c<0>(g, a.out, ii + 1, j, v);
+  else put_c<2>(g, a.out, ii + 1, j, v);
+}
+
+// out = in + x with boundary values
+__device__ __forceinline__ void yput(const YArgs &a, int ii, int row, double x) {
+  yset(a, ii, row, yin(a, ii, row) + x);
 }
 
 // Row blockIdx.y (j = row+1) after k_ylocal<true>: x = p_s on an interface
@@ -439,25 +454,40 @@ __global__ void __launch_bounds__(256) k_yapply_rows(S src) {
   yput(a, ii, row, x);
 }
 
-template <class S>
+// Chunk s, column ii (ychunk_of): x = p_s on the interface row, the
+// interior solution (solved again) with the spikes of both interfaces on the
+// others; out = in + x with boundary values.  The substage-start values are
+// loaded kYApplyG rows at a time ahead of those rows' stores: out may be in,
+// so the compiler does not move a load past a store by itself.
+constexpr int kYApplyG = 8;
+
+template <int CW, class S>
 __global__ void __launch_bounds__(256) k_yapply(S src) {
   const YArgs a = src.get();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int s = blockIdx.y * 4 + wave, ii = blockIdx.x * 64 + lane;
+  int s, ii;
+  ychunk_of<CW>(s, ii);
   if (s >= a.S || ii >= a.n) return;
-  const int n = a.n, L1 = a.L - 1;
+  const int n = a.n, L1 = a.L - 1, r0 = s * a.L;
   const int s1 = s + 1 == a.S ? 0 : s + 1;
   const YTab tb{a.yfac, L1};
   const double lam = tb.lambda();
   const double zs = a.qz[(size_t)s * n + ii], zn = a.qz[(size_t)s1 * n + ii];
-  double x[kYMaxL - 1];
-  ychunk_solve(a, tb, s, ii, x);
-  yput(a, ii, s * a.L, zs);
+  double x[kYMaxL];
+  ychunk_solve(a, tb, s, ii, *reinterpret_cast<double(*)[kYMaxL - 1]>(x + 1));
+  x[0] = zs;
 #pragma unroll
   for (int m = 1; m < kYMaxL; m++)
-    if (m <= L1)
-      yput(a, ii, s * a.L + m,
-                 x[m - 1] - lam * zs * tb.v()[m - 1] - lam * zn * tb.v()[a.L - 1 - m]);
+    if (m <= L1) x[m] = x[m] - lam * zs * tb.v()[m - 1] - lam * zn * tb.v()[a.L - 1 - m];
+#pragma unroll
+  for (int m0 = 0; m0 < kYMaxL; m0 += kYApplyG) {
+    double v[kYApplyG];
+#pragma unroll
+    for (int k = 0; k < kYApplyG; k++)
+      if (m0 + k <= L1) v[k] = yin(a, ii, r0 + m0 + k);
+#pragma unroll
+    for (int k = 0; k < kYApplyG; k++)
+      if (m0 + k <= L1) yset(a, ii, r0 + m0 + k, v[k] + x[m0 + k]);
+  }
 }
 
 // ------------------------------------------------------------- launch
@@ -499,12 +529,36 @@ static int y_cols(int SR, int chosen) {
 // The partitioned solve of nz argument blocks (instances, or fields of one
 // or more instances; each kernel checks its block's n): n the widest n
 // among them, nrule the instances the recompute rule counts, per field.
+template <int CW, class S>
+static void ylocal_launch(hipStream_t s, int n, int S_, bool recompute, S src, int nz) {
+  const dim3 gc((n + 64 * CW - 1) / (64 * CW), (S_ + 4 / CW - 1) / (4 / CW), nz);
+  if (recompute) hipLaunchKernelGGL((k_ylocal<false, CW, S>), gc, dim3(256), 0, s, src);
+  else hipLaunchKernelGGL((k_ylocal<true, CW, S>), gc, dim3(256), 0, s, src);
+}
+
+template <int CW, class S>
+static void yapply_launch(hipStream_t s, int n, int S_, S src, int nz) {
+  const dim3 gc((n + 64 * CW - 1) / (64 * CW), (S_ + 4 / CW - 1) / (4 / CW), nz);
+  hipLaunchKernelGGL((k_yapply<CW, S>), gc, dim3(256), 0, s, src);
+}
+
+// Runs of 64 columns per workgroup of the per-chunk kernels: 4 from 2048
+// columns (IMP-IMP steps at 4096x8192: 8.30 against 8.20 ms with 4 runs,
+// 2048x4096: 2.23 against 2.20 ms; at 1024x2048 one run is as fast).
+// SNSG_YSOLVE_CW=1, 2 or 4 forces a width (all return the same bits).
+static int y_col_waves(int n) {
+  const char *e = getenv("SNSG_YSOLVE_CW");
+  if (e && (e[0] == '1' || e[0] == '2' || e[0] == '4') && e[1] == 0) return e[0] - '0';
+  return n >= 2048 ? 4 : 1;
+}
+
 template <class S>
 static void ypart(hipStream_t s, const YArgs &shape, int n, S src, int nz, int nrule) {
   const bool recompute = y_recompute((size_t)n * shape.S * nrule, shape.recompute);
-  const dim3 gc((n + 63) / 64, (shape.S + 3) / 4, nz);
-  if (recompute) hipLaunchKernelGGL((k_ylocal<false, S>), gc, dim3(256), 0, s, src);
-  else hipLaunchKernelGGL((k_ylocal<true, S>), gc, dim3(256), 0, s, src);
+  const int cw = y_col_waves(n);
+  if (cw == 4) ylocal_launch<4>(s, n, shape.S, recompute, src, nz);
+  else if (cw == 2) ylocal_launch<2>(s, n, shape.S, recompute, src, nz);
+  else ylocal_launch<1>(s, n, shape.S, recompute, src, nz);
   const int SR = shape.S - 1;
   const int cols = y_cols(SR, shape.cols);
   if (SR > 16 * kYColWaves)
@@ -513,7 +567,9 @@ static void ypart(hipStream_t s, const YArgs &shape, int n, S src, int nz, int n
   else if (cols == 32) yreduced_c_launch<32>(s, n, SR, src, nz);
   else yreduced_c_launch<16>(s, n, SR, src, nz);
   if (recompute) {
-    hipLaunchKernelGGL((k_yapply<S>), gc, dim3(256), 0, s, src);
+    if (cw == 4) yapply_launch<4>(s, n, shape.S, src, nz);
+    else if (cw == 2) yapply_launch<2>(s, n, shape.S, src, nz);
+    else yapply_launch<1>(s, n, shape.S, src, nz);
   } else {
     const dim3 gr((n + 255) / 256, shape.g.jtot, nz);
     hipLaunchKernelGGL((k_yapply_rows<S>), gr, dim3(256), 0, s, src);

@@ -435,6 +435,36 @@ def test_ysolve_recompute_matches_store(sns, monkeypatch, itot, jtot, imx):
         assert_bitwise(out[0][n], out[1][n], n)
 
 
+@pytest.mark.parametrize("itot,jtot,imx", [(130, 512, 1), (2100, 64, 1), (300, 96, 0)])
+@pytest.mark.parametrize("recompute", ["0", "1"])
+def test_ysolve_column_runs_bitwise(sns, monkeypatch, itot, jtot, imx, recompute):
+    """The per-chunk kernels of the partitioned y solve cover 1, 2 or 4 runs
+    of 64 columns per workgroup (SNSG_YSOLVE_CW; partial runs past the last
+    column included).  Each thread does the same work whatever the width, so
+    three steps give the same bits."""
+    monkeypatch.setenv("SNSG_YSOLVE_RECOMPUTE", recompute)
+    out = []
+    for cw in ("1", "2", "4"):
+        monkeypatch.setenv("SNSG_YSOLVE_CW", cw)
+        o = po.Oracle(itot, jtot, stretch=3.0, Ra=1e9, Pr=0.7)
+        o.init_temperature(29, 1e-2, linear=False)
+        o.apply_boundaries()
+        inst = sns.Instance(itot, jtot, stretch=3.0, Ra=1e9, Pr=0.7, implicitx=imx, implicity=1)
+        inst.state_alloc()
+        rng = np.random.default_rng(8)
+        for n in ["ux", "uy", "p", "psi", "temp"]:
+            f = o.f[n]
+            if n in ("ux", "uy"):
+                f = f + 1e-2 * rng.standard_normal(f.shape)
+            inst.state_upload(n, f)
+        inst.advance(3)
+        out.append({n: inst.state_download(n) for n in ["ux", "uy", "p", "temp"]})
+        inst.close()
+    for k in (1, 2):
+        for n in out[0]:
+            assert_bitwise(out[k][n], out[0][n], n)
+
+
 @pytest.mark.parametrize("cols", ["16", "32", "64"])
 @pytest.mark.parametrize("case", [c for c in STEP_CASES if c[6] == 1 and c[1] >= 60])
 def test_ysolve_widths_match_oracle(sns, pinned_gauge, monkeypatch, cols, case):
