@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <atomic>
 #include "sns_device.h"
 
 namespace sns {
@@ -49,6 +50,19 @@ struct ArgBatch {
     return r;
   }
 };
+// Raise kernel f's dynamic LDS limit to `bytes` on the current device, once
+// per device (the attribute belongs to the device's copy of the kernel);
+// `done` holds one bit per device index.
+inline void lds_limit_once(const void *f, size_t bytes, std::atomic<unsigned> &done) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return;  // the launch reports the error
+  const unsigned bit = 1u << (dev & 31);
+  if (done.load(std::memory_order_acquire) & bit) return;
+  if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) ==
+      hipSuccess)
+    done.fetch_or(bit, std::memory_order_release);
+}
+
 template <class A>
 inline ArgBatch<A> arg_batch(const A *dev) {
   ArgBatch<A> b;

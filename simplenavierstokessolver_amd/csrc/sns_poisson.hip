@@ -507,12 +507,8 @@ void launch_poisson_factor(hipStream_t s, const Geo &g, const Tables &tb, const 
 template <int NT, int M, class S>
 static void px_launch(hipStream_t s, const Geo &g, S src, int nz) {
   const size_t lds = sizeof(PxLds<NT, M>);
-  static bool attr = false;
-  if (!attr) {
-    hipFuncSetAttribute((const void *)k_poisson_x3<NT, M, S>,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    attr = true;
-  }
+  static std::atomic<unsigned> attr{0};
+  lds_limit_once((const void *)k_poisson_x3<NT, M, S>, lds, attr);
   hipLaunchKernelGGL((k_poisson_x3<NT, M, S>), dim3(g.jtot / 2 + 1, 1, nz), dim3(NT), lds, s, src);
 }
 

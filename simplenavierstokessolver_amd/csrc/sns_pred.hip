@@ -629,12 +629,8 @@ __global__ void __launch_bounds__(NT) k_pred_rows(S src, int nfld) {
 template <int NT, int M, bool P2, bool FOLD, class S>
 static void pred_rows_launch(hipStream_t s, const PredArgs &shape, S src, int nz) {
   const size_t lds = sizeof(PredRowsLds<NT, M>);
-  static bool attr = false;
-  if (!attr) {
-    hipFuncSetAttribute((const void *)k_pred_rows<NT, M, P2, FOLD, S>,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    attr = true;
-  }
+  static std::atomic<unsigned> attr{0};
+  lds_limit_once((const void *)k_pred_rows<NT, M, P2, FOLD, S>, lds, attr);
   const int nfld = shape.with_temp ? 3 : 2;
   const int npair = (shape.g.jtot + 1) / 2;
   hipLaunchKernelGGL((k_pred_rows<NT, M, P2, FOLD, S>), dim3(npair * nfld, 1, nz), dim3(NT), lds,

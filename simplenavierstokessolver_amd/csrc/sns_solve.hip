@@ -128,12 +128,8 @@ template <int KIND, int NT>
 static void cnx_lu_launch(hipStream_t s, const CnxLuArgs &a) {
   constexpr int M = 8;
   const size_t lds = sizeof(double) * (NT * (M + 1) + 4 * (NT / 64));
-  static bool attr = false;
-  if (!attr) {
-    hipFuncSetAttribute((const void *)k_cnx_lu<KIND, NT, M>,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    attr = true;
-  }
+  static std::atomic<unsigned> attr{0};
+  lds_limit_once((const void *)k_cnx_lu<KIND, NT, M>, lds, attr);
   hipLaunchKernelGGL((k_cnx_lu<KIND, NT, M>), dim3(a.g.jtot), dim3(NT), lds, s, a);
 }
 
