@@ -100,6 +100,39 @@ def test_launch_shapes_match_oracle(sns, pinned_gauge, case):
     inst.close()
 
 
+@pytest.mark.parametrize("shapes", [dict(ysolve_chunk=1), dict(ysolve_chunk=4),
+                                    dict(ysolve_chunk=16, ysolve_cols=16, ysolve_recompute=2),
+                                    dict(ysolve_chunk=8, ysolve_recompute=1)])
+@pytest.mark.parametrize("imx", [0, 1])
+def test_reference_functions_follow_launch_shapes(sns, shapes, imx):
+    """The per-function entry points solve Crank-Nicolson in y with the
+    instance's launch shapes (chunk length or none, interface width,
+    recompute or store), as the fused step does: over three steps the
+    reference-order path and the fused step agree bit for bit."""
+    itot, jtot = 64, 256
+    print("reference functions %s imx %d" % (shapes, imx), flush=True)
+    o = po.Oracle(itot, jtot, stretch=3.0, Ra=1e9, Pr=0.7)
+    o.set_treatment(imx, 1)
+    o.init_temperature(13, 1e-2, linear=False)
+    o.apply_boundaries()
+    inst = sns.Instance(itot, jtot, stretch=3.0, Ra=1e9, Pr=0.7, implicitx=imx, implicity=1)
+    inst.set_launch(**shapes)
+    inst.alloc()
+    inst.state_alloc()
+    for n in ["ux", "uy", "p", "psi", "temp", "tempforcex", "srcuxa", "srcuxb", "srcuxg",
+              "srcuya", "srcuyb", "srcuyg", "srctempa", "srctempb", "srctempg"]:
+        inst.upload(n, o.f[n])
+    for n in ["ux", "uy", "p", "psi", "temp"]:
+        inst.state_upload(n, o.f[n])
+    for _ in range(3):
+        inst.set_dt(1e-3)
+        inst.integrate_reference_order()
+        inst.integrate()
+    for n in ["ux", "uy", "p", "temp"]:
+        assert same(inst.state_download(n), inst.download(n)), (shapes, n)
+    inst.close()
+
+
 def test_fft_factor_reference_functions(sns):
     """The per-function Poisson solve (snsg_fluid_compute_potential) with a
     non-default FFT factor: psi within 1e-11 of the oracle's (after removing
