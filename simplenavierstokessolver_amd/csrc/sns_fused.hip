@@ -327,8 +327,24 @@ template <class S>
 __global__ void __launch_bounds__(kDtThreads) k_dt(S src) {
   const DtArgs a = src.get();
   const double *part = a.part;
-  double m = part[0];
-  for (int k = threadIdx.x; k < a.n; k += kDtThreads) m = fmin(m, part[k]);
+  // eight independent loads in flight per thread (the minimum is exact, so
+  // the grouping does not change it)
+  constexpr int U = 8;
+  double mu[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) mu[u] = part[0];
+  int k = threadIdx.x;
+  for (; k + (U - 1) * kDtThreads < a.n; k += U * kDtThreads) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) v[u] = part[k + u * kDtThreads];
+#pragma unroll
+    for (int u = 0; u < U; u++) mu[u] = fmin(mu[u], v[u]);
+  }
+  for (; k < a.n; k += kDtThreads) mu[0] = fmin(mu[0], part[k]);
+  double m = mu[0];
+#pragma unroll
+  for (int u = 1; u < U; u++) m = fmin(m, mu[u]);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off));
   __shared__ double wm[kDtThreads / 64];
