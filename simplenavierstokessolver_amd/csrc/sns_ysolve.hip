@@ -183,26 +183,28 @@ __global__ void __launch_bounds__(64) k_yfactor(S src) {
 
 // ------------------------------------------------------ interior solves
 
-// Interior solve of chunk s, column col: the L-1 rows in x (Dirichlet
+// Interior solve of chunk s, column col: the L-1 rows in x[OFF..] (Dirichlet
 // block, shared factors), from the increments qf.
+template <int OFF, int N>
 __device__ __forceinline__ void ychunk_solve(const YArgs &a, const YTab &tb, int s, int col,
-                                             double (&x)[kYMaxL - 1]) {
+                                             double (&x)[N]) {
+  static_assert(N >= OFF + kYMaxL - 1, "room for the interior rows");
   const int L1 = a.L - 1;
   const double lam = tb.lambda();
   const double *q = a.qf + ((size_t)s * a.L + 1) * a.n + col;
 #pragma unroll
   for (int m = 0; m < kYMaxL - 1; m++)
-    if (m < L1) x[m] = q[(size_t)m * a.n];
+    if (m < L1) x[OFF + m] = q[(size_t)m * a.n];
   double y = 0.;
 #pragma unroll
   for (int m = 0; m < kYMaxL - 1; m++)
     if (m < L1) {
-      y = (x[m] - lam * y) * tb.r()[m];
-      x[m] = y;
+      y = (x[OFF + m] - lam * y) * tb.r()[m];
+      x[OFF + m] = y;
     }
 #pragma unroll
   for (int m = kYMaxL - 3; m >= 0; m--)
-    if (m < L1 - 1) x[m] = x[m] - tb.beta()[m] * x[m + 1];
+    if (m < L1 - 1) x[OFF + m] = x[OFF + m] - tb.beta()[m] * x[OFF + m + 1];
 }
 
 // Chunk and column of a thread of the per-chunk kernels: the four waves of a
@@ -227,7 +229,7 @@ __global__ void __launch_bounds__(256) k_ylocal(S src) {
   const int L1 = a.L - 1;
   const YTab tb{a.yfac, L1};
   double x[kYMaxL - 1];
-  ychunk_solve(a, tb, s, col, x);
+  ychunk_solve<0>(a, tb, s, col, x);
   double last = x[0];
 #pragma unroll
   for (int m = 1; m < kYMaxL - 1; m++)
@@ -473,7 +475,7 @@ __global__ void __launch_bounds__(256) k_yapply(S src) {
   const double lam = tb.lambda();
   const double zs = a.qz[(size_t)s * n + ii], zn = a.qz[(size_t)s1 * n + ii];
   double x[kYMaxL];
-  ychunk_solve(a, tb, s, ii, *reinterpret_cast<double(*)[kYMaxL - 1]>(x + 1));
+  ychunk_solve<1>(a, tb, s, ii, x);
   x[0] = zs;
 #pragma unroll
   for (int m = 1; m < kYMaxL; m++)
