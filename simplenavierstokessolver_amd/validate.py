@@ -48,7 +48,9 @@ def published(pr):
 
 def run(sns, pr, t_spin=500.0, t_avg=500.0, chunk=200, seed=4, log=sys.stderr):
     """Advance the case to t_spin + t_avg; from t_spin on, sample the wall
-    Nusselt number and the statistics each time t passes a whole time unit."""
+    Nusselt number and the statistics each time t passes a whole time unit
+    (the steps between samples are sized from the last dt to span about half
+    a unit, at most `chunk`)."""
     itot, jtot = 512, 1024
     inst = sns.Instance(itot, jtot, ly=2.0, stretch=3.0, Ra=1e8, Pr=pr, implicitx=1, implicity=1)
     inst.state_alloc()
@@ -58,10 +60,12 @@ def run(sns, pr, t_spin=500.0, t_avg=500.0, chunk=200, seed=4, log=sys.stderr):
         inst.upload(n, np.zeros(inst.shape(n)))
     t0 = time.perf_counter()
     t, steps, nus, next_sample, last_log = 0.0, 0, [], t_spin, -1
+    n = chunk
     while t < t_spin + t_avg:
-        inst.advance(chunk)
-        steps += chunk
+        inst.advance(n)
+        steps += n
         t, dt = inst.get_time()
+        n = max(1, min(chunk, int(0.5 / dt)))
         if t >= next_sample:
             nus.append(inst.diagnostics(state=True)["nu_wall"])
             inst.statistics_collect(state=True)
