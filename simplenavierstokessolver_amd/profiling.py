@@ -12,6 +12,10 @@
   python -m simplenavierstokessolver_amd.profiling timeline KERNEL_TRACE_CSV
         [--marker NAME] [--steps N] [--out FILE] [--command TEXT ...]
 
+  python -m simplenavierstokessolver_amd.profiling steps --itot N --jtot N
+        [--implicitx 0|1] [--implicity 0|1] [--stretch S] [--ra RA] [--pr PR]
+        [--steps N] [--warmup N]
+
 `stats` condenses rocprofv3's kernel_stats.csv into a table of kernel groups
 (the names bench.py and the C-ABI timing use).  `traffic` averages the
 FETCH_SIZE and WRITE_SIZE counters (KiB per dispatch) of every kernel over
@@ -22,6 +26,10 @@ with the kernel's algorithmic bytes (bench.KERNEL_BYTES_PER_CELL x cells).
 `counters` tabulates every counter of the given counter_collection.csv
 files as its mean per dispatch of each kernel, with the L2 hit rate
 TCC_HIT / (TCC_HIT + TCC_MISS) when both are there.
+`steps` is a program to profile: it advances one instance of the given
+size and treatment (seeded temperature noise 1e-3, the time step decided on
+the device) by --warmup and then --steps fused steps, so that a
+rocprofv3 run of it sees (warmup + steps) launches of each step kernel.
 `timeline` reads a rocprofv3 kernel_trace.csv and takes the window between
 the last two dispatches of the marker kernel (a 16-byte k_copy the traced
 program issues before and after the steps it wants measured): the share of
@@ -184,6 +192,22 @@ def timeline(path, marker="k_copy"):
     return out
 
 
+def run_steps(a):
+    import simplenavierstokessolver_amd as sns
+    stretch = float(a.itot) if a.stretch is None else a.stretch
+    inst = sns.Instance(a.itot, a.jtot, stretch=stretch, Ra=a.ra, Pr=a.pr,
+                        implicitx=a.implicitx, implicity=a.implicity)
+    inst.state_alloc()
+    inst.state_init_temperature(3, 1e-3)
+    inst.advance(a.warmup)
+    inst.sync()
+    inst.advance(a.steps)
+    inst.sync()
+    print("steps: %dx%d implicitx %d implicity %d, %d + %d steps" % (
+        a.itot, a.jtot, a.implicitx, a.implicity, a.warmup, a.steps), flush=True)
+    inst.close()
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -212,7 +236,20 @@ def main(argv=None):
     s4.add_argument("--out")
     s4.add_argument("--json")
     s4.add_argument("--command", action="append", default=[])
+    s5 = sub.add_parser("steps")
+    s5.add_argument("--itot", type=int, required=True)
+    s5.add_argument("--jtot", type=int, required=True)
+    s5.add_argument("--implicitx", type=int, default=1)
+    s5.add_argument("--implicity", type=int, default=0)
+    s5.add_argument("--stretch", type=float, default=None, help="default: uniform (itot)")
+    s5.add_argument("--ra", type=float, default=1e11)
+    s5.add_argument("--pr", type=float, default=0.7)
+    s5.add_argument("--steps", type=int, default=3)
+    s5.add_argument("--warmup", type=int, default=1)
     a = ap.parse_args(argv)
+    if a.cmd == "steps":
+        run_steps(a)
+        return
     lines = []
     if a.command:
         lines += ["Commands:", ""] + ["    " + c for c in a.command] + [""]
