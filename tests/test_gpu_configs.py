@@ -119,6 +119,41 @@ def test_large_instance_full_size(sns, pinned_gauge):
     inst.close()
 
 
+@pytest.mark.timeout(300)
+def test_large_instance_full_length_invariants(sns):
+    """The large single instance over its full 200 steps (fused path, the
+    time step decided on the device): no oracle at this length, the
+    invariants instead -- discrete divergence and net x-momentum at rounding
+    level, a temperature inside the wall values up to rounding, finite
+    energies, and the time equal to the sum of the steps' dt."""
+    itot, jtot = 4096, 8192
+    inst = sns.Instance(itot, jtot, ly=2.0, stretch=float(itot), Ra=1e11, Pr=0.7, implicitx=1,
+                        implicity=0)
+    inst.state_alloc()
+    inst.state_init_temperature(3, 1e-3)
+    inst.set_time(0.0)
+    total = 0.0
+    for chunk in range(4):
+        print("4096x8192 steps %d..%d" % (50 * chunk + 1, 50 * chunk + 50), flush=True)
+        for _ in range(50):
+            inst.advance(1)
+            total += inst.get_time()[1]
+    t, dt = inst.get_time()
+    assert abs(t - total) <= 1e-12 * t, (t, total)
+    d = inst.diagnostics(state=True)
+    ux, uy, temp = (inst.state_download(n) for n in ("ux", "uy", "temp"))
+    umax = max(np.abs(ux).max(), np.abs(uy).max())
+    print("t %.4g dt %.3g divmax %.2e mom_x %.2e umax %.3e Tmax %.17g" % (
+        t, dt, d["divmax"], d["mom_x"], umax, np.abs(temp).max()), flush=True)
+    assert np.isfinite(ux).all() and np.isfinite(uy).all() and np.isfinite(temp).all()
+    assert 0.0 < umax < 1.0
+    assert d["divmax"] <= 1e3 * np.finfo(float).eps * umax * itot
+    assert abs(d["mom_x"]) <= 1e-13 * umax * 2.0
+    assert np.abs(temp).max() <= 0.5 + 1e-12
+    assert all(np.isfinite(d[k]) for k in ("energy_ux", "energy_uy", "energy_t"))
+    inst.close()
+
+
 def with_treatments(sns, jobs):
     from simplenavierstokessolver_amd import ensemble as ens
     costs = ens.load_costs()
