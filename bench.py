@@ -111,7 +111,7 @@ def parse():
                     help="steps per job of the mixed ensemble (0: skip it)")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the served request stream (0: skip it)")
-    ap.add_argument("--serve-slots", type=int, default=32,
+    ap.add_argument("--serve-slots", type=int, default=16,
                     help="most jobs on the GPU at once in the served stream (0: memory budget "
                          "alone)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
