@@ -211,6 +211,28 @@ def grid(itot, jtot, ly=2.0, stretch=3.0, lx=1.0):
     return g
 
 
+def tanh_grid(itot, jtot, ly=2.0, stretch=3.0, lx=1.0):
+    """Wall-refined x coordinates from a hyperbolic tangent, an option with
+    no counterpart in the reference (whose stretched grid is the clipped
+    Chebyshev one, src/param/init.c:144-165):
+        XF(i) = lx/2 (1 + tanh(s (2 (i-1)/itot - 1)) / tanh(s)),  i = 1..itot+1,
+    s = stretch, with XF(1) = 0 and XF(itot+1) = lx exactly; the cell centres,
+    spacings and y coordinates follow from XF as in the reference
+    (src/param/init.c:170-187).  The instance takes them through
+    Instance(..., coords=tanh_grid(...))."""
+    g = grid(itot, jtot, ly, float(itot), lx)  # y as the reference; x replaced
+    xi = np.arange(itot + 1) / itot
+    xf = 0.5 * lx * (1.0 + np.tanh(stretch * (2.0 * xi - 1.0)) / np.tanh(stretch))
+    xf[0], xf[itot] = 0.0, lx
+    xc = np.empty(itot + 2)
+    xc[0], xc[itot + 1] = 0.0, lx
+    xc[1:itot + 1] = 0.5 * (xf[:-1] + xf[1:])
+    g["xf"], g["xc"] = xf, xc
+    g["dxf"] = xf[1:] - xf[:-1]
+    g["dxc"] = xc[1:] - xc[:-1]
+    return g
+
+
 def rk_coefs():
     a, b, c = np.zeros(3), np.zeros(3), np.zeros(3)
     _load_host().sns_rk_coefs(_p(a), _p(b), _p(c))
@@ -314,10 +336,13 @@ class Instance:
     the reference-named operations on them (through include/sns_hip.h)."""
 
     def __init__(self, itot, jtot, ly=2.0, stretch=3.0, Ra=1e8, Pr=1.0, implicitx=0,
-                 implicity=0, with_temperature=True, with_thermal_forcing=True, device=0):
+                 implicity=0, with_temperature=True, with_thermal_forcing=True, device=0,
+                 coords=None):
+        """coords: x coordinates other than the reference's (a dict of grid()'s
+        keys, e.g. tanh_grid()); stretch is then unused."""
         self.L = _load_hip()
         self.itot, self.jtot = itot, jtot
-        self.g = grid(itot, jtot, ly, stretch)
+        self.g = grid(itot, jtot, ly, stretch) if coords is None else coords
         self.alpha, self.beta, self.gamma = rk_coefs()
         cfg = make_config(itot, jtot, ly, Ra, Pr, implicitx, implicity, with_temperature,
                           with_thermal_forcing)

@@ -154,6 +154,36 @@ def test_large_instance_full_length_invariants(sns):
     inst.close()
 
 
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("case", [(128, 256, 10), (1024, 2048, 3)])
+def test_tanh_grid_against_oracle(sns, pinned_gauge, case):
+    """The single-instance configuration with tanh-stretched x spacing, as
+    the benchmark list words it (stretch factor 3, Ra=1e10, Pr=1, Crank-
+    Nicolson in x and y; the reference itself only makes the clipped
+    Chebyshev grid): the oracle on the same coordinates, fused steps within
+    1e-12."""
+    itot, jtot, steps = case
+    g = sns.tanh_grid(itot, jtot, 2.0, 3.0)
+    o = po.Oracle(itot, jtot, ly=2.0, stretch=3.0, Ra=1e10, Pr=1.0)
+    for n in ("xf", "xc", "dxf", "dxc"):
+        np.copyto(getattr(o, n), g[n])
+    o.set_treatment(1, 1)
+    o.init_temperature(2, 1e-3, linear=False)
+    o.apply_boundaries()
+    inst = sns.Instance(itot, jtot, ly=2.0, Ra=1e10, Pr=1.0, implicitx=1, implicity=1, coords=g)
+    inst.state_alloc()
+    for n in ("ux", "uy", "p", "psi", "temp"):
+        inst.state_upload(n, o.f[n])
+    inst.set_time(0.0)
+    print("tanh grid %dx%d, %d steps" % (itot, jtot, steps), flush=True)
+    o.run(steps)
+    inst.advance(steps)
+    t, dt = inst.get_time()
+    assert abs(dt - o.param.dt) <= 1e-12 * o.param.dt, (dt, o.param.dt)
+    check_fields("tanh %dx%d" % (itot, jtot), inst.state_download, o)
+    inst.close()
+
+
 def with_treatments(sns, jobs):
     from simplenavierstokessolver_amd import ensemble as ens
     costs = ens.load_costs()
